@@ -1,0 +1,795 @@
+// Host runtime and C ABI of the MI355X stockholder-partitioning library.
+//
+// The outer fixed-point loop of AbstractISAWPart.do_partitioning
+// (core/iterstock.py:159-193) runs here, natively: per iteration one launch of
+// the fused weights pass, one launch of the per-atom solver and one
+// finalisation launch, then a 32-byte read-back of (change, entropy, status)
+// that decides convergence exactly as iterstock.py:187 does.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "hortonpart.h"
+#include "hp_device.h"
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const std::string &msg) {
+  g_err = msg;
+  return code;
+}
+
+#define HP_HIP(expr)                                                                        \
+  do {                                                                                      \
+    hipError_t e_ = (expr);                                                                 \
+    if (e_ != hipSuccess) return fail(HP_E_HIP, std::string(#expr ": ") + hipGetErrorString(e_)); \
+  } while (0)
+
+#define HP_LAUNCHED()                                                                       \
+  do {                                                                                      \
+    hipError_t e_ = hipGetLastError();                                                      \
+    if (e_ != hipSuccess) return fail(HP_E_HIP, std::string("kernel launch: ") + hipGetErrorString(e_)); \
+  } while (0)
+
+template <typename T>
+int dalloc(T **p, size_t n) {
+  *p = nullptr;
+  if (n == 0) n = 1;
+  hipError_t e = hipMalloc((void **)p, n * sizeof(T));
+  if (e != hipSuccess) return fail(HP_E_HIP, std::string("hipMalloc: ") + hipGetErrorString(e));
+  return 0;
+}
+
+template <typename T>
+int dupload(T **p, const T *host, size_t n, hipStream_t s) {
+  int rc = dalloc(p, n);
+  if (rc) return rc;
+  if (n) HP_HIP(hipMemcpyAsync(*p, host, n * sizeof(T), hipMemcpyHostToDevice, s));
+  return 0;
+}
+
+}  // namespace
+
+struct hp_part {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  hipEvent_t ev[6] = {};
+  int natom = 0, nshell = 0, nfn = 0, npar = 0;
+  int64_t npoint = 0;
+  int scheme = 0, solver = 0, grid_type = 1, fnkind = HP_FN_SLATER;
+  int maxL = 0, maxR = 0, maxK = 0;
+  double threshold = 1e-6, inner_threshold = 1e-8, dc = 1e-15, neg_cutoff = -1e-12, pop_cutoff = 1e-4;
+  double rc = 0.0, tau = 1.0;
+  int maxiter = 500, inner_maxiter = 2000, nlin = 20, check_mono = 0;
+  std::vector<int64_t> h_shell_off;
+  std::vector<int> h_atom_shell_off;
+  // device: molecule
+  double *cx = nullptr, *cy = nullptr, *cz = nullptr, *px = nullptr, *py = nullptr, *pz = nullptr;
+  double *rho = nullptr, *wmol = nullptr, *wat = nullptr, *wsph = nullptr, *shell_r = nullptr, *shell_w = nullptr,
+         *pseudo = nullptr;
+  int *shell_atom = nullptr, *atom_shell_off = nullptr;
+  int64_t *shell_off = nullptr;
+  // device: scheme
+  int *fn_off = nullptr, *nb_list = nullptr, *shell_nb = nullptr;
+  double *fn_norm = nullptr, *fn_alpha = nullptr, *fn_order = nullptr, *coef = nullptr, *alpha = nullptr,
+         *qpP = nullptr;
+  int64_t *qp_off = nullptr;
+  // device: state
+  double *par = nullptr, *old_par = nullptr, *rhobar = nullptr, *ent_part = nullptr, *promol = nullptr,
+         *charges = nullptr, *change_part = nullptr, *out = nullptr, *mom_part = nullptr, *mom_out = nullptr,
+         *scratch_pts = nullptr;
+  int *inner_niter = nullptr, *status = nullptr, *status_all = nullptr;
+  double *hpar = nullptr, *hch = nullptr, *hent = nullptr, *hchg = nullptr;
+  int hist_cap = 0;
+  double *h_out = nullptr;  // pinned
+  // host state
+  int niter = 0;
+  double last_change = 0.0, last_entropy = 0.0;
+  double k_ms[3] = {0, 0, 0};
+  int64_t k_n[3] = {0, 0, 0};
+
+  DevMol mol() const {
+    DevMol m;
+    m.natom = natom;
+    m.nshell = nshell;
+    m.npoint = npoint;
+    m.cx = cx;
+    m.cy = cy;
+    m.cz = cz;
+    m.px = px;
+    m.py = py;
+    m.pz = pz;
+    m.rho = rho;
+    m.wmol = wmol;
+    m.wat = wat;
+    m.wsph = wsph;
+    m.shell_atom = shell_atom;
+    m.shell_off = shell_off;
+    m.shell_r = shell_r;
+    m.shell_w = shell_w;
+    m.atom_shell_off = atom_shell_off;
+    m.pseudo = pseudo;
+    return m;
+  }
+  DevFn fn() const {
+    DevFn f;
+    f.fn_off = fn_off;
+    f.coef = coef;
+    f.alpha = alpha;
+    f.order = fn_order;
+    f.nb_list = rc > 0.0 ? nb_list : nullptr;
+    f.shell_nb = shell_nb;
+    f.rc = rc;
+    return f;
+  }
+};
+
+namespace {
+
+void free_part(hp_part *p) {
+  if (!p) return;
+  (void)hipSetDevice(p->device);
+  void *bufs[] = {p->cx,      p->cy,          p->cz,      p->px,        p->py,          p->pz,       p->rho,
+                  p->wmol,    p->wat,         p->wsph,    p->shell_r,   p->shell_w,     p->pseudo,   p->shell_atom,
+                  p->atom_shell_off, p->shell_off, p->fn_off, p->nb_list, p->shell_nb,    p->fn_norm,  p->fn_alpha,
+                  p->fn_order, p->coef,       p->alpha,   p->qpP,       p->qp_off,      p->par,      p->old_par,
+                  p->rhobar,  p->ent_part,    p->promol,  p->charges,   p->change_part, p->out,      p->mom_part,
+                  p->mom_out, p->scratch_pts, p->inner_niter, p->status, p->status_all, p->hpar,    p->hch,
+                  p->hent,    p->hchg};
+  for (void *b : bufs)
+    if (b) (void)hipFree(b);
+  if (p->h_out) (void)hipHostFree(p->h_out);
+  for (auto &e : p->ev)
+    if (e) (void)hipEventDestroy(e);
+  if (p->stream) (void)hipStreamDestroy(p->stream);
+  delete p;
+}
+
+int pick_ppl(int R) {
+  if (R <= 128) return 2;
+  if (R <= 256) return 4;
+  if (R <= 512) return 8;
+  return 0;
+}
+
+// ---- kernel launch helpers -------------------------------------------------
+
+template <typename R, int FK>
+void launch_weights_fk(hp_part *p, bool gt2) {
+  int bs = ((p->maxL + 63) / 64) * 64;
+  bs = std::max(64, std::min(512, bs));
+  if (gt2)
+    hipLaunchKernelGGL((k_weights<R, FK, true>), dim3(p->nshell), dim3(bs), 0, p->stream, p->mol(), p->fn(), p->rhobar,
+                       p->ent_part, p->promol, p->dc);
+  else
+    hipLaunchKernelGGL((k_weights<R, FK, false>), dim3(p->nshell), dim3(bs), 0, p->stream, p->mol(), p->fn(),
+                       p->rhobar, p->ent_part, p->promol, p->dc);
+}
+
+template <typename R>
+void launch_weights(hp_part *p, bool gt2) {
+  if (p->fnkind == HP_FN_SLATER)
+    launch_weights_fk<R, HP_FN_SLATER>(p, gt2);
+  else if (p->fnkind == HP_FN_GAUSS)
+    launch_weights_fk<R, HP_FN_GAUSS>(p, gt2);
+  else
+    launch_weights_fk<R, HP_FN_GENERAL>(p, gt2);
+}
+
+template <int PPL>
+int launch_inner_ppl(hp_part *p) {
+  const dim3 grid((p->natom + 3) / 4), block(256);
+  if (p->scheme == HP_SCHEME_MBIS) {
+    if (p->maxK <= 3)
+      hipLaunchKernelGGL((k_mbis_inner<PPL, 3>), grid, block, 0, p->stream, p->mol(), p->rhobar, p->par, p->old_par,
+                         p->fn_off, p->inner_threshold, p->inner_maxiter, p->inner_niter, p->status, p->charges,
+                         p->change_part);
+    else
+      hipLaunchKernelGGL((k_mbis_inner<PPL, 7>), grid, block, 0, p->stream, p->mol(), p->rhobar, p->par, p->old_par,
+                         p->fn_off, p->inner_threshold, p->inner_maxiter, p->inner_niter, p->status, p->charges,
+                         p->change_part);
+  } else if (p->solver == HP_SOLVER_QP) {
+    hipLaunchKernelGGL((k_gisa_qp<PPL>), grid, block, 0, p->stream, p->mol(), p->rhobar, p->par, p->old_par, p->fn_off,
+                       p->fn_norm, p->fn_alpha, p->fn_order, p->qpP, p->qp_off, p->pop_cutoff, p->neg_cutoff,
+                       p->inner_niter, p->status, p->charges, p->change_part);
+  } else {
+#define HP_LISA(KMV)                                                                                             \
+  hipLaunchKernelGGL((k_lisa_inner<PPL, KMV>), grid, block, 0, p->stream, p->mol(), p->rhobar, p->par, p->old_par, \
+                     p->fn_off, p->fn_norm, p->fn_alpha, p->fn_order, p->solver, p->inner_threshold,             \
+                     p->inner_maxiter, p->dc, p->neg_cutoff, p->pop_cutoff, p->tau, p->nlin, p->check_mono,       \
+                     p->inner_niter, p->status, p->charges, p->change_part)
+    if (p->maxK <= 4)
+      HP_LISA(4);
+    else if (p->maxK <= 6)
+      HP_LISA(6);
+    else if (p->maxK <= 8)
+      HP_LISA(8);
+    else if (p->maxK <= 12)
+      HP_LISA(12);
+    else
+      return fail(HP_E_UNSUPP, "LISA on the device supports at most 12 functions per atom");
+#undef HP_LISA
+  }
+  return 0;
+}
+
+int launch_inner(hp_part *p) {
+  const int ppl = pick_ppl(p->maxR);
+  if (ppl == 2) return launch_inner_ppl<2>(p);
+  if (ppl == 4) return launch_inner_ppl<4>(p);
+  if (ppl == 8) return launch_inner_ppl<8>(p);
+  return fail(HP_E_UNSUPP, "more than 512 radial points per atom");
+}
+
+// One outer iteration: update_at_weights + per-atom updates + entropy + change.
+int outer_step(hp_part *p, int precision, int iter) {
+  hipStream_t s = p->stream;
+  HP_HIP(hipMemcpyAsync(p->old_par, p->par, sizeof(double) * p->npar, hipMemcpyDeviceToDevice, s));
+  hipLaunchKernelGGL(k_build_fn, dim3((p->nfn + 255) / 256), dim3(256), 0, s, p->scheme == HP_SCHEME_MBIS ? 0 : 1,
+                     p->nfn, p->par, p->fn_norm, p->fn_alpha, p->coef, p->alpha);
+  HP_LAUNCHED();
+  HP_HIP(hipEventRecord(p->ev[0], s));
+  if (precision == HP_PREC_MIXED)
+    launch_weights<float>(p, false);
+  else
+    launch_weights<double>(p, false);
+  HP_LAUNCHED();
+  HP_HIP(hipEventRecord(p->ev[1], s));
+  int rc = launch_inner(p);
+  if (rc) return rc;
+  HP_LAUNCHED();
+  HP_HIP(hipEventRecord(p->ev[2], s));
+  hipLaunchKernelGGL(k_finalize, dim3(1), dim3(256), 0, s, p->nshell, p->ent_part, p->natom, p->change_part, p->status,
+                     p->status_all, p->par, p->npar, p->charges, iter, p->hpar, p->hch, p->hent, p->hchg, p->out);
+  HP_LAUNCHED();
+  HP_HIP(hipMemcpyAsync(p->h_out, p->out, 4 * sizeof(double), hipMemcpyDeviceToHost, s));
+  HP_HIP(hipStreamSynchronize(s));
+  float ms = 0.f;
+  HP_HIP(hipEventElapsedTime(&ms, p->ev[0], p->ev[1]));
+  p->k_ms[HP_KERNEL_WEIGHTS] += ms;
+  p->k_n[HP_KERNEL_WEIGHTS] += 1;
+  HP_HIP(hipEventElapsedTime(&ms, p->ev[1], p->ev[2]));
+  p->k_ms[HP_KERNEL_INNER] += ms;
+  p->k_n[HP_KERNEL_INNER] += 1;
+  p->last_change = p->h_out[0];
+  p->last_entropy = p->h_out[1];
+  if ((int)p->h_out[2] != 0) return fail(HP_E_SOLVER, "a per-atom solver raised; see hp_part_status");
+  return 0;
+}
+
+}  // namespace
+
+extern "C" {
+
+int hp_abi_version(void) { return HP_ABI_VERSION; }
+
+const char *hp_last_error(void) { return g_err.c_str(); }
+
+int hp_device_count(int *count) {
+  if (!count) return fail(HP_E_ARG, "count is NULL");
+  int n = 0;
+  hipError_t e = hipGetDeviceCount(&n);
+  if (e != hipSuccess) {
+    *count = 0;
+    return fail(HP_E_HIP, std::string("hipGetDeviceCount: ") + hipGetErrorString(e));
+  }
+  *count = n;
+  return 0;
+}
+
+int hp_part_create(int device, const hp_molecule_desc *md, const hp_scheme_desc *sd, hp_part **out) {
+  g_err.clear();
+  if (!md || !sd || !out) return fail(HP_E_ARG, "NULL argument");
+  *out = nullptr;
+  const int A = md->natom, S = md->nshell;
+  const int64_t N = md->npoint;
+  if (A <= 0 || S <= 0 || N <= 0) return fail(HP_E_ARG, "natom, nshell and npoint must be positive");
+  if (!md->coordinates || !md->pseudo_numbers || !md->points || !md->density || !md->mol_weights ||
+      !md->atom_weights || !md->sph_weights || !md->shell_atom || !md->shell_offsets || !md->shell_radius ||
+      !md->shell_rweight)
+    return fail(HP_E_ARG, "molecule description has a NULL array");
+  if (sd->scheme < 0 || sd->scheme > 2) return fail(HP_E_ARG, "unknown scheme");
+  if (sd->grid_type != 1) return fail(HP_E_UNSUPP, "the device path implements grid_type 1");
+  if (!sd->fn_offsets) return fail(HP_E_ARG, "fn_offsets is NULL");
+  const bool mbis = sd->scheme == HP_SCHEME_MBIS;
+  if (mbis && sd->solver != HP_SOLVER_MBIS) return fail(HP_E_ARG, "MBIS uses HP_SOLVER_MBIS");
+  if (sd->scheme == HP_SCHEME_GISA && sd->solver != HP_SOLVER_QP) return fail(HP_E_ARG, "GISA uses HP_SOLVER_QP");
+  if (sd->scheme == HP_SCHEME_LISA &&
+      !(sd->solver == HP_SOLVER_SC || sd->solver == HP_SOLVER_NEWTON || sd->solver == HP_SOLVER_MNEWTON))
+    return fail(HP_E_UNSUPP, "LISA solver not available on the device");
+  if (sd->solver == HP_SOLVER_QP && !sd->qp_matrix) return fail(HP_E_ARG, "qp_matrix is NULL");
+  if (!mbis && (!sd->fn_alpha || !sd->fn_order || !sd->fn_norm)) return fail(HP_E_ARG, "basis arrays are NULL");
+  if (sd->maxiter < 1 || sd->inner_maxiter < 1) return fail(HP_E_ARG, "iteration caps must be positive");
+  if (sd->solver == HP_SOLVER_MNEWTON && sd->newton_linspace < 2) return fail(HP_E_ARG, "linspace size < 2");
+
+  // validate grid structure: shells grouped by atom in order, offsets monotone
+  std::vector<int> atom_shell_off(A + 1, 0);
+  {
+    int prev = -1;
+    for (int s = 0; s < S; ++s) {
+      const int a = md->shell_atom[s];
+      if (a < 0 || a >= A || a < prev) return fail(HP_E_ARG, "shell_atom must be sorted atom indices");
+      atom_shell_off[a + 1] += 1;
+      prev = a;
+    }
+    for (int a = 0; a < A; ++a) {
+      if (atom_shell_off[a + 1] == 0) return fail(HP_E_ARG, "every atom needs at least one radial shell");
+      atom_shell_off[a + 1] += atom_shell_off[a];
+    }
+    if (md->shell_offsets[0] != 0 || md->shell_offsets[S] != N) return fail(HP_E_ARG, "shell_offsets must span the grid");
+  }
+  int maxL = 0, maxR = 0, maxK = 0;
+  for (int s = 0; s < S; ++s) {
+    const int64_t L = md->shell_offsets[s + 1] - md->shell_offsets[s];
+    if (L <= 0) return fail(HP_E_ARG, "empty radial shell");
+    maxL = std::max<int64_t>(maxL, L);
+  }
+  for (int a = 0; a < A; ++a) maxR = std::max(maxR, atom_shell_off[a + 1] - atom_shell_off[a]);
+  if (pick_ppl(maxR) == 0) return fail(HP_E_UNSUPP, "more than 512 radial points per atom");
+  const int nfn = sd->fn_offsets[A];
+  if (sd->fn_offsets[0] != 0) return fail(HP_E_ARG, "fn_offsets[0] must be 0");
+  for (int a = 0; a < A; ++a) {
+    const int K = sd->fn_offsets[a + 1] - sd->fn_offsets[a];
+    if (K <= 0) return fail(HP_E_ARG, "every atom needs at least one function");
+    maxK = std::max(maxK, K);
+  }
+  if (mbis && maxK > 7) return fail(HP_E_UNSUPP, "MBIS supports at most 7 shells per atom");
+  if (maxK > HP_KMAX) return fail(HP_E_UNSUPP, "too many functions per atom");
+  int fnkind = HP_FN_SLATER;
+  if (!mbis) {
+    bool all1 = true, all2 = true;
+    for (int f = 0; f < nfn; ++f) {
+      if (!(sd->fn_order[f] > 0.0) || !(sd->fn_alpha[f] >= 0.0))
+        return fail(HP_E_ARG, "basis orders must be positive and exponents non-negative");
+      all1 = all1 && sd->fn_order[f] == 1.0;
+      all2 = all2 && sd->fn_order[f] == 2.0;
+    }
+    fnkind = all1 ? HP_FN_SLATER : (all2 ? HP_FN_GAUSS : HP_FN_GENERAL);
+  }
+
+  hp_part *p = new hp_part();
+  p->device = device;
+  p->natom = A;
+  p->nshell = S;
+  p->npoint = N;
+  p->nfn = nfn;
+  p->npar = mbis ? 2 * nfn : nfn;
+  p->scheme = sd->scheme;
+  p->solver = sd->solver;
+  p->grid_type = sd->grid_type;
+  p->fnkind = fnkind;
+  p->maxL = maxL;
+  p->maxR = maxR;
+  p->maxK = maxK;
+  p->threshold = sd->threshold;
+  p->inner_threshold = sd->inner_threshold;
+  p->dc = sd->density_cutoff;
+  p->neg_cutoff = sd->negative_cutoff;
+  p->pop_cutoff = sd->population_cutoff;
+  p->rc = sd->cutoff_radius;
+  p->tau = sd->newton_tau;
+  p->nlin = sd->newton_linspace;
+  p->check_mono = sd->check_mono;
+  p->maxiter = sd->maxiter;
+  p->inner_maxiter = sd->inner_maxiter;
+  p->h_shell_off.assign(md->shell_offsets, md->shell_offsets + S + 1);
+  p->h_atom_shell_off = atom_shell_off;
+
+  auto bail = [&](int rc) {
+    free_part(p);
+    return rc;
+  };
+  hipError_t e = hipSetDevice(device);
+  if (e != hipSuccess) return bail(fail(HP_E_HIP, std::string("hipSetDevice: ") + hipGetErrorString(e)));
+  e = hipStreamCreateWithFlags(&p->stream, hipStreamNonBlocking);
+  if (e != hipSuccess) return bail(fail(HP_E_HIP, std::string("hipStreamCreate: ") + hipGetErrorString(e)));
+  for (auto &ev : p->ev) {
+    e = hipEventCreate(&ev);
+    if (e != hipSuccess) return bail(fail(HP_E_HIP, "hipEventCreate failed"));
+  }
+  e = hipHostMalloc((void **)&p->h_out, 4 * sizeof(double), hipHostMallocDefault);
+  if (e != hipSuccess) return bail(fail(HP_E_HIP, "hipHostMalloc failed"));
+  hipStream_t s = p->stream;
+
+  // SoA copies of coordinates and points
+  std::vector<double> ax(A), ay(A), az(A);
+  for (int a = 0; a < A; ++a) {
+    ax[a] = md->coordinates[3 * a];
+    ay[a] = md->coordinates[3 * a + 1];
+    az[a] = md->coordinates[3 * a + 2];
+  }
+  std::vector<double> gx(N), gy(N), gz(N);
+  for (int64_t i = 0; i < N; ++i) {
+    gx[i] = md->points[3 * i];
+    gy[i] = md->points[3 * i + 1];
+    gz[i] = md->points[3 * i + 2];
+  }
+  std::vector<int> shell_atom(md->shell_atom, md->shell_atom + S);
+  int rc = 0;
+  if ((rc = dupload(&p->cx, ax.data(), A, s)) || (rc = dupload(&p->cy, ay.data(), A, s)) ||
+      (rc = dupload(&p->cz, az.data(), A, s)) || (rc = dupload(&p->px, gx.data(), N, s)) ||
+      (rc = dupload(&p->py, gy.data(), N, s)) || (rc = dupload(&p->pz, gz.data(), N, s)) ||
+      (rc = dupload(&p->rho, md->density, N, s)) || (rc = dupload(&p->wmol, md->mol_weights, N, s)) ||
+      (rc = dupload(&p->wat, md->atom_weights, N, s)) || (rc = dupload(&p->wsph, md->sph_weights, N, s)) ||
+      (rc = dupload(&p->shell_r, md->shell_radius, S, s)) || (rc = dupload(&p->shell_w, md->shell_rweight, S, s)) ||
+      (rc = dupload(&p->pseudo, md->pseudo_numbers, A, s)) || (rc = dupload(&p->shell_atom, shell_atom.data(), S, s)) ||
+      (rc = dupload(&p->shell_off, md->shell_offsets, S + 1, s)) ||
+      (rc = dupload(&p->atom_shell_off, atom_shell_off.data(), A + 1, s)))
+    return bail(rc);
+  std::vector<int> fn_off(sd->fn_offsets, sd->fn_offsets + A + 1);
+  if ((rc = dupload(&p->fn_off, fn_off.data(), A + 1, s))) return bail(rc);
+  if (!mbis) {
+    if ((rc = dupload(&p->fn_norm, sd->fn_norm, nfn, s)) || (rc = dupload(&p->fn_alpha, sd->fn_alpha, nfn, s)) ||
+        (rc = dupload(&p->fn_order, sd->fn_order, nfn, s)))
+      return bail(rc);
+  } else {
+    std::vector<double> ones(nfn, 1.0);
+    if ((rc = dupload(&p->fn_order, ones.data(), nfn, s))) return bail(rc);
+  }
+  if (sd->solver == HP_SOLVER_QP) {
+    std::vector<int64_t> qoff(A + 1, 0);
+    for (int a = 0; a < A; ++a) {
+      const int64_t K = fn_off[a + 1] - fn_off[a];
+      qoff[a + 1] = qoff[a] + K * K;
+    }
+    if ((rc = dupload(&p->qpP, sd->qp_matrix, qoff[A], s)) || (rc = dupload(&p->qp_off, qoff.data(), A + 1, s)))
+      return bail(rc);
+  }
+  // neighbour lists for the cut-off: atoms sorted by distance to each atom,
+  // and per shell the number of atoms that can reach any of its points
+  if (p->rc > 0.0) {
+    std::vector<int> nb((size_t)A * A);
+    std::vector<double> dist((size_t)A * A);
+    for (int a = 0; a < A; ++a) {
+      std::vector<std::pair<double, int>> v(A);
+      for (int b = 0; b < A; ++b) {
+        const double dx = ax[a] - ax[b], dy = ay[a] - ay[b], dz = az[a] - az[b];
+        v[b] = {std::sqrt(dx * dx + dy * dy + dz * dz), b};
+      }
+      std::sort(v.begin(), v.end());
+      for (int t = 0; t < A; ++t) {
+        nb[(size_t)a * A + t] = v[t].second;
+        dist[(size_t)a * A + t] = v[t].first;
+      }
+    }
+    std::vector<int> shell_nb(S);
+    for (int si = 0; si < S; ++si) {
+      const int a = shell_atom[si];
+      double rmax = 0.0;
+      for (int64_t i = md->shell_offsets[si]; i < md->shell_offsets[si + 1]; ++i) {
+        const double dx = gx[i] - ax[a], dy = gy[i] - ay[a], dz = gz[i] - az[a];
+        rmax = std::max(rmax, std::sqrt(dx * dx + dy * dy + dz * dz));
+      }
+      const double reach = rmax + p->rc + 1e-9 * (1.0 + rmax + p->rc);
+      const double *d = &dist[(size_t)a * A];
+      shell_nb[si] = (int)(std::upper_bound(d, d + A, reach) - d);
+    }
+    if ((rc = dupload(&p->nb_list, nb.data(), nb.size(), s)) || (rc = dupload(&p->shell_nb, shell_nb.data(), S, s)))
+      return bail(rc);
+  }
+  // state
+  p->hist_cap = p->maxiter + 64;
+  if ((rc = dalloc(&p->coef, nfn)) || (rc = dalloc(&p->alpha, nfn)) || (rc = dalloc(&p->par, p->npar)) ||
+      (rc = dalloc(&p->old_par, p->npar)) || (rc = dalloc(&p->rhobar, S)) || (rc = dalloc(&p->ent_part, S)) ||
+      (rc = dalloc(&p->charges, A)) || (rc = dalloc(&p->change_part, A)) || (rc = dalloc(&p->out, 4)) ||
+      (rc = dalloc(&p->inner_niter, A)) || (rc = dalloc(&p->status, A)) || (rc = dalloc(&p->status_all, A)) ||
+      (rc = dalloc(&p->hpar, (size_t)p->hist_cap * p->npar)) || (rc = dalloc(&p->hch, (size_t)p->hist_cap * A)) ||
+      (rc = dalloc(&p->hent, p->hist_cap)) || (rc = dalloc(&p->hchg, p->hist_cap)))
+    return bail(rc);
+  e = hipMemsetAsync(p->status_all, 0, sizeof(int) * A, s);
+  if (e == hipSuccess) e = hipMemsetAsync(p->status, 0, sizeof(int) * A, s);
+  if (e == hipSuccess) e = hipMemsetAsync(p->inner_niter, 0, sizeof(int) * A, s);
+  if (e == hipSuccess) e = hipMemsetAsync(p->charges, 0, sizeof(double) * A, s);
+  if (e == hipSuccess) e = hipStreamSynchronize(s);
+  if (e != hipSuccess) return bail(fail(HP_E_HIP, std::string("initialisation: ") + hipGetErrorString(e)));
+  *out = p;
+  return 0;
+}
+
+int hp_part_destroy(hp_part *part) {
+  if (!part) return fail(HP_E_ARG, "NULL part");
+  (void)hipSetDevice(part->device);
+  (void)hipStreamSynchronize(part->stream);
+  free_part(part);
+  return 0;
+}
+
+int hp_part_set_propars(hp_part *p, const double *propars, int64_t n) {
+  if (!p || !propars) return fail(HP_E_ARG, "NULL argument");
+  if (n != p->npar) return fail(HP_E_ARG, "propars length mismatch");
+  HP_HIP(hipSetDevice(p->device));
+  HP_HIP(hipMemcpyAsync(p->par, propars, sizeof(double) * n, hipMemcpyHostToDevice, p->stream));
+  HP_HIP(hipMemcpyAsync(p->old_par, propars, sizeof(double) * n, hipMemcpyHostToDevice, p->stream));
+  HP_HIP(hipStreamSynchronize(p->stream));
+  p->niter = 0;
+  return 0;
+}
+
+int hp_part_run(hp_part *p, int precision, int n_polish, int *niter) {
+  if (!p) return fail(HP_E_ARG, "NULL part");
+  if (precision != HP_PREC_FP64 && precision != HP_PREC_MIXED) return fail(HP_E_ARG, "unknown precision");
+  HP_HIP(hipSetDevice(p->device));
+  for (int k = 0; k < 3; ++k) {
+    p->k_ms[k] = 0.0;
+    p->k_n[k] = 0;
+  }
+  HP_HIP(hipMemsetAsync(p->status_all, 0, sizeof(int) * p->natom, p->stream));
+  int counter = 0;
+  int rc = 0;
+  // main loop (iterstock.py:172-188)
+  while (true) {
+    if (counter >= p->hist_cap) return fail(HP_E_ARG, "history capacity exceeded");
+    rc = outer_step(p, precision, counter);
+    counter += 1;
+    p->niter = counter;
+    if (rc) break;
+    if (p->last_change < p->threshold || counter >= p->maxiter) break;
+  }
+  if (!rc && precision == HP_PREC_MIXED) {
+    const int np = std::max(1, n_polish);
+    for (int i = 0; i < np && !rc; ++i) {
+      if (counter >= p->hist_cap) return fail(HP_E_ARG, "history capacity exceeded");
+      rc = outer_step(p, HP_PREC_FP64, counter);
+      counter += 1;
+      p->niter = counter;
+    }
+  }
+  if (niter) *niter = counter;
+  return rc;
+}
+
+int hp_part_results(hp_part *p, double *propars, double *charges, double *change, double *entropy) {
+  if (!p) return fail(HP_E_ARG, "NULL part");
+  HP_HIP(hipSetDevice(p->device));
+  if (propars) HP_HIP(hipMemcpyAsync(propars, p->par, sizeof(double) * p->npar, hipMemcpyDeviceToHost, p->stream));
+  if (charges) HP_HIP(hipMemcpyAsync(charges, p->charges, sizeof(double) * p->natom, hipMemcpyDeviceToHost, p->stream));
+  HP_HIP(hipStreamSynchronize(p->stream));
+  if (change) *change = p->last_change;
+  if (entropy) *entropy = p->last_entropy;
+  return 0;
+}
+
+int hp_part_history(hp_part *p, double *hpropars, double *hcharges, double *hentropies, double *hchanges) {
+  if (!p) return fail(HP_E_ARG, "NULL part");
+  HP_HIP(hipSetDevice(p->device));
+  const size_t n = (size_t)p->niter;
+  if (hpropars && n)
+    HP_HIP(hipMemcpyAsync(hpropars, p->hpar, sizeof(double) * n * p->npar, hipMemcpyDeviceToHost, p->stream));
+  if (hcharges && n)
+    HP_HIP(hipMemcpyAsync(hcharges, p->hch, sizeof(double) * n * p->natom, hipMemcpyDeviceToHost, p->stream));
+  if (hentropies && n) HP_HIP(hipMemcpyAsync(hentropies, p->hent, sizeof(double) * n, hipMemcpyDeviceToHost, p->stream));
+  if (hchanges && n) HP_HIP(hipMemcpyAsync(hchanges, p->hchg, sizeof(double) * n, hipMemcpyDeviceToHost, p->stream));
+  HP_HIP(hipStreamSynchronize(p->stream));
+  return 0;
+}
+
+int hp_part_status(hp_part *p, int32_t *inner_niter, int32_t *status_last, int32_t *status_all) {
+  if (!p) return fail(HP_E_ARG, "NULL part");
+  HP_HIP(hipSetDevice(p->device));
+  if (inner_niter)
+    HP_HIP(hipMemcpyAsync(inner_niter, p->inner_niter, sizeof(int) * p->natom, hipMemcpyDeviceToHost, p->stream));
+  if (status_last)
+    HP_HIP(hipMemcpyAsync(status_last, p->status, sizeof(int) * p->natom, hipMemcpyDeviceToHost, p->stream));
+  if (status_all)
+    HP_HIP(hipMemcpyAsync(status_all, p->status_all, sizeof(int) * p->natom, hipMemcpyDeviceToHost, p->stream));
+  HP_HIP(hipStreamSynchronize(p->stream));
+  return 0;
+}
+
+int hp_part_spherical_averages(hp_part *p, double *rhobar) {
+  if (!p || !rhobar) return fail(HP_E_ARG, "NULL argument");
+  HP_HIP(hipSetDevice(p->device));
+  HP_HIP(hipMemcpyAsync(rhobar, p->rhobar, sizeof(double) * p->nshell, hipMemcpyDeviceToHost, p->stream));
+  HP_HIP(hipStreamSynchronize(p->stream));
+  return 0;
+}
+
+static int point_weights(hp_part *p, int atom, int64_t p0, int64_t n, double *out) {
+  HP_HIP(hipSetDevice(p->device));
+  if (!p->scratch_pts) {
+    int rc = dalloc(&p->scratch_pts, (size_t)p->npoint);
+    if (rc) return rc;
+  }
+  const dim3 grid((unsigned)((n + 255) / 256)), block(256);
+  DevFn f = p->fn();
+  if (p->fnkind == HP_FN_SLATER)
+    hipLaunchKernelGGL((k_point_weights<HP_FN_SLATER>), grid, block, 0, p->stream, p->mol(), f, atom, p0, n,
+                       p->scratch_pts);
+  else if (p->fnkind == HP_FN_GAUSS)
+    hipLaunchKernelGGL((k_point_weights<HP_FN_GAUSS>), grid, block, 0, p->stream, p->mol(), f, atom, p0, n,
+                       p->scratch_pts);
+  else
+    hipLaunchKernelGGL((k_point_weights<HP_FN_GENERAL>), grid, block, 0, p->stream, p->mol(), f, atom, p0, n,
+                       p->scratch_pts);
+  HP_LAUNCHED();
+  HP_HIP(hipMemcpyAsync(out, p->scratch_pts, sizeof(double) * n, hipMemcpyDeviceToHost, p->stream));
+  HP_HIP(hipStreamSynchronize(p->stream));
+  return 0;
+}
+
+int hp_part_at_weights(hp_part *p, int atom, double *out, int64_t n) {
+  if (!p || !out) return fail(HP_E_ARG, "NULL argument");
+  if (atom < 0 || atom >= p->natom) return fail(HP_E_ARG, "atom index out of range");
+  if (p->niter == 0) return fail(HP_E_ARG, "no weights yet: run the partitioning first");
+  const int64_t p0 = p->h_shell_off[p->h_atom_shell_off[atom]];
+  const int64_t p1 = p->h_shell_off[p->h_atom_shell_off[atom + 1]];
+  if (n != p1 - p0) return fail(HP_E_ARG, "output length must equal the atom-grid size");
+  return point_weights(p, atom, p0, n, out);
+}
+
+int hp_part_promoldens(hp_part *p, double *out, int64_t n) {
+  if (!p || !out) return fail(HP_E_ARG, "NULL argument");
+  if (n != p->npoint) return fail(HP_E_ARG, "output length must equal npoint");
+  if (p->niter == 0) return fail(HP_E_ARG, "no promolecule yet: run the partitioning first");
+  return point_weights(p, -1, 0, n, out);
+}
+
+int hp_part_moments(hp_part *p, int lmax, double *cart, double *pure, double *radial) {
+  if (!p) return fail(HP_E_ARG, "NULL part");
+  if (lmax != 3) return fail(HP_E_UNSUPP, "the device moments are implemented for lmax = 3");
+  if (p->niter == 0) return fail(HP_E_ARG, "run the partitioning first");
+  HP_HIP(hipSetDevice(p->device));
+  int rc = 0;
+  if (!p->mom_part && ((rc = dalloc(&p->mom_part, (size_t)p->nshell * HP_NMOM)) ||
+                       (rc = dalloc(&p->mom_out, (size_t)p->natom * HP_NMOM))))
+    return rc;
+  DevFn f = p->fn();
+  if (p->fnkind == HP_FN_SLATER)
+    hipLaunchKernelGGL((k_moments<HP_FN_SLATER>), dim3(p->nshell), dim3(256), 0, p->stream, p->mol(), f, p->mom_part);
+  else if (p->fnkind == HP_FN_GAUSS)
+    hipLaunchKernelGGL((k_moments<HP_FN_GAUSS>), dim3(p->nshell), dim3(256), 0, p->stream, p->mol(), f, p->mom_part);
+  else
+    hipLaunchKernelGGL((k_moments<HP_FN_GENERAL>), dim3(p->nshell), dim3(256), 0, p->stream, p->mol(), f, p->mom_part);
+  HP_LAUNCHED();
+  hipLaunchKernelGGL(k_moments_reduce, dim3(p->natom), dim3(64), 0, p->stream, p->mol(), p->mom_part, p->mom_out);
+  HP_LAUNCHED();
+  std::vector<double> h((size_t)p->natom * HP_NMOM);
+  std::vector<double> z(p->natom);
+  HP_HIP(hipMemcpyAsync(h.data(), p->mom_out, sizeof(double) * h.size(), hipMemcpyDeviceToHost, p->stream));
+  HP_HIP(hipMemcpyAsync(z.data(), p->pseudo, sizeof(double) * p->natom, hipMemcpyDeviceToHost, p->stream));
+  HP_HIP(hipStreamSynchronize(p->stream));
+  for (int a = 0; a < p->natom; ++a) {
+    const double *v = &h[(size_t)a * HP_NMOM];
+    if (cart) {
+      for (int i = 0; i < 20; ++i) cart[a * 20 + i] = -v[i];
+      cart[a * 20] += z[a];
+    }
+    if (pure) {
+      for (int i = 0; i < 16; ++i) pure[a * 16 + i] = -v[20 + i];
+      pure[a * 16] += z[a];
+    }
+    if (radial)
+      for (int i = 0; i < 4; ++i) radial[a * 4 + i] = v[36 + i];
+  }
+  return 0;
+}
+
+int hp_part_kernel_time(hp_part *p, int kernel, double *total_ms, int64_t *launches) {
+  if (!p || kernel < 0 || kernel > 2) return fail(HP_E_ARG, "bad argument");
+  if (total_ms) *total_ms = p->k_ms[kernel];
+  if (launches) *launches = p->k_n[kernel];
+  return 0;
+}
+
+}  // extern "C"
+
+// ---- grid utilities ----------------------------------------------------------
+
+namespace {
+struct GridUpload {
+  double *cx = nullptr, *cy = nullptr, *cz = nullptr, *px = nullptr, *py = nullptr, *pz = nullptr, *out = nullptr;
+  ~GridUpload() {
+    void *b[] = {cx, cy, cz, px, py, pz, out};
+    for (void *q : b)
+      if (q) (void)hipFree(q);
+  }
+};
+
+int upload_points(GridUpload &g, int natom, const double *coords, int64_t npoint, const double *points) {
+  std::vector<double> ax(natom), ay(natom), az(natom);
+  for (int a = 0; a < natom; ++a) {
+    ax[a] = coords[3 * a];
+    ay[a] = coords[3 * a + 1];
+    az[a] = coords[3 * a + 2];
+  }
+  std::vector<double> gx(npoint), gy(npoint), gz(npoint);
+  for (int64_t i = 0; i < npoint; ++i) {
+    gx[i] = points[3 * i];
+    gy[i] = points[3 * i + 1];
+    gz[i] = points[3 * i + 2];
+  }
+  int rc = 0;
+  if ((rc = dupload(&g.cx, ax.data(), natom, nullptr)) || (rc = dupload(&g.cy, ay.data(), natom, nullptr)) ||
+      (rc = dupload(&g.cz, az.data(), natom, nullptr)) || (rc = dupload(&g.px, gx.data(), npoint, nullptr)) ||
+      (rc = dupload(&g.py, gy.data(), npoint, nullptr)) || (rc = dupload(&g.pz, gz.data(), npoint, nullptr)) ||
+      (rc = dalloc(&g.out, npoint)))
+    return rc;
+  return 0;
+}
+}  // namespace
+
+extern "C" {
+
+int hp_becke_weights(int device, int natom, const double *coordinates, const double *radii, int64_t npoint,
+                     const double *points, const int64_t *atom_offsets, double *out) {
+  g_err.clear();
+  if (natom <= 0 || npoint <= 0 || !coordinates || !radii || !points || !atom_offsets || !out)
+    return fail(HP_E_ARG, "bad argument");
+  if (atom_offsets[0] != 0 || atom_offsets[natom] != npoint) return fail(HP_E_ARG, "atom_offsets must span the points");
+  for (int a = 0; a < natom; ++a)
+    if (atom_offsets[a + 1] < atom_offsets[a]) return fail(HP_E_ARG, "atom_offsets must be non-decreasing");
+  // size adjustment a_ij from the radius ratio chi = R_i / R_j, |a_ij| <= 1/2
+  std::vector<double> adj((size_t)natom * natom, 0.0), invr((size_t)natom * natom, 0.0);
+  for (int i = 0; i < natom; ++i)
+    for (int j = 0; j < natom; ++j) {
+      if (i == j) continue;
+      const double chi = radii[i] / radii[j];
+      const double u = (chi - 1.0) / (chi + 1.0);
+      const double den = u * u - 1.0;
+      double aij = den != 0.0 ? u / den : 0.0;
+      aij = std::min(0.5, std::max(-0.5, aij));
+      adj[(size_t)i * natom + j] = aij;
+      const double dx = coordinates[3 * i] - coordinates[3 * j], dy = coordinates[3 * i + 1] - coordinates[3 * j + 1],
+                   dz = coordinates[3 * i + 2] - coordinates[3 * j + 2];
+      const double d = std::sqrt(dx * dx + dy * dy + dz * dz);
+      if (d == 0.0) return fail(HP_E_ARG, "two atoms at the same position");
+      invr[(size_t)i * natom + j] = 1.0 / d;
+    }
+  HP_HIP(hipSetDevice(device));
+  GridUpload g;
+  double *dadj = nullptr, *dinvr = nullptr;
+  int64_t *doff = nullptr;
+  int rc = upload_points(g, natom, coordinates, npoint, points);
+  if (!rc) rc = dupload(&dadj, adj.data(), adj.size(), nullptr);
+  if (!rc) rc = dupload(&dinvr, invr.data(), invr.size(), nullptr);
+  if (!rc) rc = dupload(&doff, atom_offsets, natom + 1, nullptr);
+  if (!rc) {
+    hipLaunchKernelGGL(k_becke, dim3((unsigned)((npoint + 255) / 256)), dim3(256), 0, nullptr, natom, g.cx, g.cy, g.cz,
+                       dadj, dinvr, npoint, g.px, g.py, g.pz, doff, 3, g.out);
+    hipError_t e = hipGetLastError();
+    if (e == hipSuccess) e = hipMemcpy(out, g.out, sizeof(double) * npoint, hipMemcpyDeviceToHost);
+    if (e != hipSuccess) rc = fail(HP_E_HIP, std::string("becke: ") + hipGetErrorString(e));
+  }
+  if (dadj) (void)hipFree(dadj);
+  if (dinvr) (void)hipFree(dinvr);
+  if (doff) (void)hipFree(doff);
+  return rc;
+}
+
+int hp_slater_density(int device, int natom, const double *coordinates, const int32_t *fn_offsets,
+                      const double *coef, const double *alpha, int64_t npoint, const double *points, double *out) {
+  g_err.clear();
+  if (natom <= 0 || npoint <= 0 || !coordinates || !fn_offsets || !coef || !alpha || !points || !out)
+    return fail(HP_E_ARG, "bad argument");
+  const int nfn = fn_offsets[natom];
+  if (fn_offsets[0] != 0 || nfn <= 0) return fail(HP_E_ARG, "bad fn_offsets");
+  HP_HIP(hipSetDevice(device));
+  GridUpload g;
+  int *doff = nullptr;
+  double *dcoef = nullptr, *dalpha = nullptr;
+  std::vector<int> off(fn_offsets, fn_offsets + natom + 1);
+  int rc = upload_points(g, natom, coordinates, npoint, points);
+  if (!rc) rc = dupload(&doff, off.data(), natom + 1, nullptr);
+  if (!rc) rc = dupload(&dcoef, coef, nfn, nullptr);
+  if (!rc) rc = dupload(&dalpha, alpha, nfn, nullptr);
+  if (!rc) {
+    hipLaunchKernelGGL(k_slater_sum, dim3((unsigned)((npoint + 255) / 256)), dim3(256), 0, nullptr, natom, g.cx, g.cy,
+                       g.cz, doff, dcoef, dalpha, npoint, g.px, g.py, g.pz, g.out);
+    hipError_t e = hipGetLastError();
+    if (e == hipSuccess) e = hipMemcpy(out, g.out, sizeof(double) * npoint, hipMemcpyDeviceToHost);
+    if (e != hipSuccess) rc = fail(HP_E_HIP, std::string("slater density: ") + hipGetErrorString(e));
+  }
+  if (doff) (void)hipFree(doff);
+  if (dcoef) (void)hipFree(dcoef);
+  if (dalpha) (void)hipFree(dalpha);
+  return rc;
+}
+
+}  // extern "C"

@@ -1,0 +1,1060 @@
+// Device code of the stockholder-partitioning library (gfx950, wave64).
+//
+// Kernels
+//   k_build_fn        pro-atom function table (coefficient, exponent) from the
+//                     current parameters (mbis.py:284-289, basis.py:161-171)
+//   k_weights<R,F,G>  fused pass over one radial shell per workgroup:
+//                     pro-atoms of every (or every neighbouring) atom,
+//                     promolecule with the +1e-100 floor, own-atom weight
+//                     clipped to [0,1], spherical average of w*rho and the
+//                     entropy partial (stockholder.py:145-175, 352-384;
+//                     mbis.py:176-184).  R = float gives the mixed path.
+//   k_mbis_inner      one wavefront per atom: MBIS fixed point (mbis.py:81-163)
+//   k_gisa_qp         one wavefront per atom: QP over the simplex (gisa.py:348-421)
+//   k_lisa_inner      one wavefront per atom: sc / newton / m-newton on the
+//                     radial grid (alisa.py:193-291, 753-938)
+//   k_finalize        entropy and change sums, histories (iterstock.py:32-58)
+//   k_moments*        AIM multipoles (core/base.py:329-402)
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#define HP_KMAX 16      // functions per atom handled by the per-atom kernels
+
+// per-atom status bits (same values as HP_ST_* in include/hortonpart.h)
+#define HP_ST_NOCONV 1
+#define HP_ST_POP 2
+#define HP_ST_NEGDENS 4
+#define HP_ST_LINESEARCH 8
+#define HP_ST_NEWTONMAX 16
+#define HP_ST_QPFAIL 32
+#define HP_ST_NEGPARS 64
+#define HP_ST_NOTMONO 128
+#define HP_ST_SINGULAR 256
+#define HP_ST_NONFINITE 512
+#define HP_PI 3.14159265358979323846
+
+struct DevMol {
+  int natom;
+  int nshell;
+  int64_t npoint;
+  const double *cx, *cy, *cz;        // atom centres
+  const double *px, *py, *pz;        // grid points (SoA)
+  const double *rho, *wmol, *wat, *wsph;
+  const int *shell_atom;
+  const int64_t *shell_off;
+  const double *shell_r, *shell_w;
+  const int *atom_shell_off;         // natom+1 radial-shell offsets per atom
+  const double *pseudo;
+};
+
+struct DevFn {
+  const int *fn_off;     // natom+1
+  const double *coef;    // per function: prefactor (population included)
+  const double *alpha;   // per function: exponent
+  const double *order;   // per function: power of r
+  const int *nb_list;    // natom*natom neighbour lists sorted by distance (culling)
+  const int *shell_nb;   // nshell: candidate count of each shell (culling)
+  double rc;             // cutoff radius; <= 0 disables culling
+};
+
+// Function kinds of the promolecule pass.
+#define HP_FN_SLATER 0   // all functions exp(-a r)
+#define HP_FN_GAUSS 1    // all functions exp(-a r^2)
+#define HP_FN_GENERAL 2  // exp(-a r^n), n per function
+
+// ---------------------------------------------------------------------------
+// reductions (deterministic: fixed tree, result broadcast from lane 0)
+// ---------------------------------------------------------------------------
+
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_down(v, o, 64);
+  return __shfl(v, 0, 64);
+}
+
+__device__ __forceinline__ double wave_min(double v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v = fmin(v, __shfl_down(v, o, 64));
+  return __shfl(v, 0, 64);
+}
+
+// Sum of n values per lane over the wave, n known at compile time.
+template <int N>
+__device__ __forceinline__ void wave_sum_n(double *v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) {
+#pragma unroll
+    for (int i = 0; i < N; ++i) v[i] += __shfl_down(v[i], o, 64);
+  }
+#pragma unroll
+  for (int i = 0; i < N; ++i) v[i] = __shfl(v[i], 0, 64);
+}
+
+// Block sum of two values (blockDim multiple of 64, <= 1024); thread 0 gets it.
+__device__ __forceinline__ void block_sum2(double &a, double &b, double *lds) {
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6, nw = blockDim.x >> 6;
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) {
+    a += __shfl_down(a, o, 64);
+    b += __shfl_down(b, o, 64);
+  }
+  if (lane == 0) {
+    lds[2 * wv] = a;
+    lds[2 * wv + 1] = b;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double sa = 0.0, sb = 0.0;
+    for (int w = 0; w < nw; ++w) {
+      sa += lds[2 * w];
+      sb += lds[2 * w + 1];
+    }
+    a = sa;
+    b = sb;
+  }
+}
+
+template <typename R>
+__device__ __forceinline__ R hp_exp(R x);
+template <>
+__device__ __forceinline__ double hp_exp<double>(double x) { return exp(x); }
+template <>
+__device__ __forceinline__ float hp_exp<float>(float x) { return __expf(x); }
+
+template <typename R>
+__device__ __forceinline__ R hp_sqrt(R x);
+template <>
+__device__ __forceinline__ double hp_sqrt<double>(double x) { return sqrt(x); }
+template <>
+__device__ __forceinline__ float hp_sqrt<float>(float x) { return __fsqrt_rn(x); }
+
+template <typename R>
+__device__ __forceinline__ R hp_floor_add();
+template <>
+__device__ __forceinline__ double hp_floor_add<double>() { return 1e-100; }
+template <>
+__device__ __forceinline__ float hp_floor_add<float>() { return 0.0f; }
+
+// ---------------------------------------------------------------------------
+// function table
+// ---------------------------------------------------------------------------
+
+// MBIS: par[2f] = N, par[2f+1] = S -> coef = N S^3 / (8 pi), alpha = S.
+// Linear bases: coef = c_f * norm_f, alpha fixed.
+__global__ void k_build_fn(int scheme, int nfn, const double *__restrict__ par, const double *__restrict__ norm,
+                           const double *__restrict__ alpha_fixed, double *__restrict__ coef, double *__restrict__ alpha) {
+  const int f = blockIdx.x * blockDim.x + threadIdx.x;
+  if (f >= nfn) return;
+  if (scheme == 0) {
+    const double N = par[2 * f], S = par[2 * f + 1];
+    coef[f] = N * (S * S * S) / (8.0 * HP_PI);
+    alpha[f] = S;
+  } else {
+    coef[f] = par[f] * norm[f];
+    alpha[f] = alpha_fixed[f];
+  }
+}
+
+// Pro-atom density of atom b at squared distance r2 (and distance r).
+template <typename R, int FK>
+__device__ __forceinline__ R proatom_value(const DevFn &fn, int b, R r, R r2) {
+  R pb = (R)0;
+  const int f1 = fn.fn_off[b + 1];
+  for (int f = fn.fn_off[b]; f < f1; ++f) {
+    const R c = (R)fn.coef[f];
+    const R al = (R)fn.alpha[f];
+    R arg;
+    if (FK == HP_FN_SLATER) {
+      arg = r;
+    } else if (FK == HP_FN_GAUSS) {
+      arg = r2;
+    } else {
+      const double n = fn.order[f];
+      arg = (n == 1.0) ? r : ((n == 2.0) ? r2 : (R)pow((double)r, n));
+    }
+    pb += c * hp_exp<R>(-al * arg);
+  }
+  return pb;
+}
+
+// Promolecule and own-atom pro-atom at one point (reference accumulation
+// order: atom index order, +1e-100 after each atom, stockholder.py:167-170).
+template <typename R, int FK>
+__device__ __forceinline__ void promol_at(const DevMol &m, const DevFn &fn, int a, int nb_n, const int *nb, double x,
+                                          double y, double z, R &promol, R &own) {
+  promol = (R)0;
+  own = (R)0;
+  const bool cull = fn.rc > 0.0;
+  const R rc = (R)fn.rc;
+  for (int t = 0; t < nb_n; ++t) {
+    const int b = nb ? nb[t] : t;
+    const double dx = x - m.cx[b], dy = y - m.cy[b], dz = z - m.cz[b];
+    const R r2 = (R)(dx * dx + dy * dy + dz * dz);
+    const R r = (FK == HP_FN_GAUSS && !cull) ? (R)0 : hp_sqrt<R>(r2);
+    R pb = (R)0;
+    if (!cull || r <= rc) pb = proatom_value<R, FK>(fn, b, r, r2);
+    promol += pb;
+    promol += hp_floor_add<R>();
+    if (b == a) own = pb;
+  }
+  if (nb) {
+    // culled atoms still contribute their 1e-100 floor
+    for (int t = nb_n; t < m.natom; ++t) promol += hp_floor_add<R>();
+  }
+}
+
+// ---------------------------------------------------------------------------
+// K1: fused weights pass, one workgroup per radial shell
+// ---------------------------------------------------------------------------
+template <typename R, int FK, bool GT2>
+__global__ __launch_bounds__(512) void k_weights(DevMol m, DevFn fn, double *__restrict__ rhobar,
+                                                 double *__restrict__ ent_part, double *__restrict__ promol_out,
+                                                 double dc) {
+  __shared__ double lds[32];
+  const int s = blockIdx.x;
+  const int a = m.shell_atom[s];
+  const int64_t p0 = m.shell_off[s], p1 = m.shell_off[s + 1];
+  int nb_n = m.natom;
+  const int *nb = nullptr;
+  if (fn.rc > 0.0) {
+    nb = fn.nb_list + (int64_t)a * m.natom;
+    nb_n = fn.shell_nb[s];
+  }
+  double acc_avg = 0.0, acc_ent = 0.0;
+  for (int64_t p = p0 + threadIdx.x; p < p1; p += blockDim.x) {
+    R promol, own;
+    promol_at<R, FK>(m, fn, a, nb_n, nb, m.px[p], m.py[p], m.pz[p], promol, own);
+    const double pm = (double)promol;
+    const double rho = m.rho[p];
+    if (!GT2) {
+      double w = (promol > (R)0) ? (double)(own / promol) : 0.0;
+      w = fmin(fmax(w, 0.0), 1.0);
+      const double f = w * rho;
+      acc_avg += m.wsph[p] * f;
+    } else {
+      promol_out[p] = pm;
+    }
+    if (!(pm < dc || rho < dc)) acc_ent += m.wmol[p] * rho * log(rho / pm);
+  }
+  block_sum2(acc_avg, acc_ent, lds);
+  if (threadIdx.x == 0) {
+    if (!GT2) rhobar[s] = acc_avg;
+    ent_part[s] = acc_ent;
+  }
+}
+
+// Promolecule (stored) or AIM weights of one atom over a point range.
+template <int FK>
+__global__ void k_point_weights(DevMol m, DevFn fn, int atom, int64_t p0, int64_t n, double *__restrict__ out) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const int64_t p = p0 + i;
+  double promol, own;
+  // culling lists are per owning atom and shell; evaluate without them here
+  DevFn f2 = fn;
+  f2.nb_list = nullptr;
+  const int a = atom < 0 ? 0 : atom;
+  promol_at<double, FK>(m, f2, a, m.natom, nullptr, m.px[p], m.py[p], m.pz[p], promol, own);
+  if (atom < 0) {
+    out[i] = promol;
+  } else {
+    double w = own / promol;
+    out[i] = fmin(fmax(w, 0.0), 1.0);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// per-atom radial data shared by the inner solvers
+// ---------------------------------------------------------------------------
+template <int PPL>
+struct RadialLane {
+  double r[PPL], w[PPL], wr[PPL], rho[PPL];
+};
+
+template <int PPL>
+__device__ __forceinline__ void load_radial(const DevMol &m, const double *rhobar, int s0, int R, int lane,
+                                            RadialLane<PPL> &d) {
+#pragma unroll
+  for (int j = 0; j < PPL; ++j) {
+    const int i = lane + 64 * j;
+    if (i < R) {
+      const double r = m.shell_r[s0 + i];
+      d.r[j] = r;
+      d.wr[j] = m.shell_w[s0 + i];
+      d.w[j] = 4.0 * HP_PI * (r * r) * d.wr[j];
+      d.rho[j] = rhobar[s0 + i];
+    } else {
+      d.r[j] = 0.0;
+      d.wr[j] = 0.0;
+      d.w[j] = 0.0;
+      d.rho[j] = 0.0;
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// K2: MBIS inner fixed point, one wavefront per atom (mbis.py:81-163)
+// KM: compile-time bound on shells per atom (loops fully unrolled, guarded by
+// the wave-uniform K so that every per-lane array stays in registers).
+// ---------------------------------------------------------------------------
+template <int PPL, int KM>
+__global__ __launch_bounds__(256) void k_mbis_inner(DevMol m, const double *__restrict__ rhobar,
+                                                    double *__restrict__ par, const double *__restrict__ old_par,
+                                                    const int *__restrict__ fn_off, double thr, int maxit,
+                                                    int *__restrict__ inner_niter, int *__restrict__ status,
+                                                    double *__restrict__ charges, double *__restrict__ change_part) {
+  const int lane = threadIdx.x & 63;
+  const int a = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+  if (a >= m.natom) return;
+  const int s0 = m.atom_shell_off[a], R = m.atom_shell_off[a + 1] - s0;
+  const int f0 = fn_off[a], K = fn_off[a + 1] - f0;
+  RadialLane<PPL> d;
+  load_radial<PPL>(m, rhobar, s0, R, lane, d);
+  double pop = 0.0;
+#pragma unroll
+  for (int j = 0; j < PPL; ++j) pop += d.w[j] * d.rho[j];
+  pop = wave_sum(pop);
+
+  double Np[KM], Sp[KM];
+#pragma unroll
+  for (int k = 0; k < KM; ++k) {
+    Np[k] = k < K ? par[2 * (f0 + k)] : 0.0;
+    Sp[k] = k < K ? par[2 * (f0 + k) + 1] : 1.0;
+  }
+  double oldpro[PPL];
+#pragma unroll
+  for (int j = 0; j < PPL; ++j) oldpro[j] = 0.0;
+  bool have_old = false, conv = false;
+  int it = 0;
+  const double dc = 1e-15;  // opt_mbis_propars default density_cutoff (mbis.py:82)
+  for (it = 0; it < maxit; ++it) {
+    double t[KM][PPL];
+    double pro[PPL];
+#pragma unroll
+    for (int j = 0; j < PPL; ++j) pro[j] = 0.0;
+#pragma unroll
+    for (int k = 0; k < KM; ++k) {
+      const double NS3 = Np[k] * (Sp[k] * Sp[k] * Sp[k]);
+#pragma unroll
+      for (int j = 0; j < PPL; ++j) {
+        if (k < K) {
+          t[k][j] = NS3 * exp(-Sp[k] * d.r[j]) / (8.0 * HP_PI);
+          pro[j] += t[k][j];
+        } else {
+          t[k][j] = 0.0;
+        }
+      }
+    }
+    double red[2 * KM + 1];
+#pragma unroll
+    for (int q = 0; q < 2 * KM + 1; ++q) red[q] = 0.0;
+#pragma unroll
+    for (int j = 0; j < PPL; ++j) {
+      const bool sick = (d.rho[j] < dc) || (pro[j] < dc);
+      const double ratio = sick ? 0.0 : d.rho[j] / pro[j];
+#pragma unroll
+      for (int k = 0; k < KM; ++k) {
+        const double wt = d.w[j] * (t[k][j] * ratio);
+        red[2 * k] += wt;
+        red[2 * k + 1] += wt * d.r[j];
+      }
+      const double e = oldpro[j] - pro[j];
+      red[2 * KM] += d.w[j] * e * e;
+    }
+    wave_sum_n<2 * KM + 1>(red);
+#pragma unroll
+    for (int k = 0; k < KM; ++k) {
+      if (k < K) {
+        Np[k] = red[2 * k];
+        Sp[k] = 3.0 * red[2 * k] / red[2 * k + 1];
+      }
+    }
+    const double change = have_old ? sqrt(red[2 * KM]) : 1e100;
+    if (change < thr) {
+      conv = true;
+      break;
+    }
+#pragma unroll
+    for (int j = 0; j < PPL; ++j) oldpro[j] = pro[j];
+    have_old = true;
+  }
+  // change of the outer loop: new vs old pro-atom on the radial grid
+  double msd = 0.0;
+#pragma unroll
+  for (int j = 0; j < PPL; ++j) {
+    double y1 = 0.0, y2 = 0.0;
+#pragma unroll
+    for (int k = 0; k < KM; ++k) {
+      if (k < K) {
+        const double N1 = Np[k], S1 = Sp[k];
+        const double N2 = old_par[2 * (f0 + k)], S2 = old_par[2 * (f0 + k) + 1];
+        y1 += N1 * (S1 * S1 * S1) * exp(-S1 * d.r[j]) / (8.0 * HP_PI);
+        y2 += N2 * (S2 * S2 * S2) * exp(-S2 * d.r[j]) / (8.0 * HP_PI);
+      }
+    }
+    const double e = y1 - y2;
+    msd += d.wr[j] * (4.0 * HP_PI * (d.r[j] * d.r[j])) * e * e;
+  }
+  msd = wave_sum(msd);
+  if (lane == 0) {
+    int st = 0;
+    double sumN = 0.0;
+#pragma unroll
+    for (int k = 0; k < KM; ++k)
+      if (k < K) sumN += Np[k];
+    if (conv) {
+      // np.isclose(pop, sum, atol=1e-4): |pop - sum| <= 1e-4 + 1e-5 |sum|
+      if (!(fabs(pop - sumN) <= 1e-4 + 1e-5 * fabs(sumN))) st |= HP_ST_POP;
+    } else {
+      st |= HP_ST_NOCONV;
+    }
+#pragma unroll
+    for (int k = 0; k < KM; ++k) {
+      if (k < K) {
+        par[2 * (f0 + k)] = Np[k];
+        par[2 * (f0 + k) + 1] = Sp[k];
+      }
+    }
+    inner_niter[a] = conv ? it + 1 : maxit;
+    status[a] = st;
+    charges[a] = m.pseudo[a] - pop;
+    change_part[a] = msd;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// small dense solve in one lane (Gaussian elimination, partial pivoting)
+// M is n x (n+1) augmented, row stride ld, in LDS or private memory.
+// ---------------------------------------------------------------------------
+__device__ inline bool dense_solve(double *M, int n, int ld, double *x) {
+  for (int c = 0; c < n; ++c) {
+    int p = c;
+    double best = fabs(M[c * ld + c]);
+    for (int r = c + 1; r < n; ++r) {
+      const double v = fabs(M[r * ld + c]);
+      if (v > best) {
+        best = v;
+        p = r;
+      }
+    }
+    if (p != c) {
+      for (int j = 0; j <= n; ++j) {
+        const double tmp = M[c * ld + j];
+        M[c * ld + j] = M[p * ld + j];
+        M[p * ld + j] = tmp;
+      }
+    }
+    const double piv = M[c * ld + c];
+    if (piv == 0.0) return false;
+    for (int r = c + 1; r < n; ++r) {
+      const double f = M[r * ld + c] / piv;
+      if (f != 0.0) {
+        for (int j = c; j <= n; ++j) M[r * ld + j] -= f * M[c * ld + j];
+      }
+    }
+  }
+  for (int r = n - 1; r >= 0; --r) {
+    double s = 0.0;
+    for (int j = r + 1; j < n; ++j) s += M[r * ld + j] * x[j];
+    x[r] = (M[r * ld + n] - s) / M[r * ld + r];
+  }
+  return true;
+}
+
+// Primal active-set QP: min 1/2 x'Px + q'x, x >= 0, sum x = pop (same steps
+// as the oracle's qp_simplex_active_set).  Returns false if not converged.
+__device__ inline bool qp_simplex(const double *P, const double *q, double pop, int K, double *x, double *M) {
+  bool fixed[HP_KMAX];
+  int freeidx[HP_KMAX];
+  double xs[HP_KMAX + 1];
+  for (int k = 0; k < K; ++k) {
+    x[k] = pop / K;
+    fixed[k] = false;
+  }
+  for (int it = 0; it < 200; ++it) {
+    int mfree = 0;
+    for (int k = 0; k < K; ++k)
+      if (!fixed[k]) freeidx[mfree++] = k;
+    const int n = mfree + 1, ld = HP_KMAX + 2;
+    for (int i = 0; i < mfree; ++i) {
+      for (int j = 0; j < mfree; ++j) M[i * ld + j] = P[freeidx[i] * K + freeidx[j]];
+      M[i * ld + mfree] = -1.0;
+      M[i * ld + n] = -q[freeidx[i]];
+    }
+    for (int j = 0; j < mfree; ++j) M[mfree * ld + j] = 1.0;
+    M[mfree * ld + mfree] = 0.0;
+    M[mfree * ld + n] = pop;
+    if (!dense_solve(M, n, ld, xs)) return false;
+    bool feasible = true;
+    for (int i = 0; i < mfree; ++i)
+      if (!(xs[i] >= 0.0)) feasible = false;
+    if (feasible) {
+      const double nu = xs[mfree];
+      for (int k = 0; k < K; ++k) x[k] = 0.0;
+      for (int i = 0; i < mfree; ++i) x[freeidx[i]] = xs[i];
+      int jmin = -1;
+      double mumin = 0.0;
+      bool anyfixed = false;
+      for (int k = 0; k < K; ++k) {
+        double mu = 0.0;
+        if (fixed[k]) {
+          anyfixed = true;
+          double g = q[k];
+          for (int j = 0; j < K; ++j) g += P[k * K + j] * x[j];
+          mu = g - nu;
+        }
+        if (jmin < 0 || mu < mumin) {
+          mumin = mu;
+          jmin = k;
+        }
+      }
+      if (!anyfixed || mumin >= 0.0) return true;
+      fixed[jmin] = false;
+    } else {
+      double alpha = 1.0;
+      int block = -1;
+      for (int i = 0; i < mfree; ++i) {
+        const int k = freeidx[i];
+        if (xs[i] < 0.0) {
+          const double al = x[k] / (x[k] - xs[i]);
+          if (al < alpha) {
+            alpha = al;
+            block = k;
+          }
+        }
+      }
+      double xn[HP_KMAX];
+      for (int k = 0; k < K; ++k) xn[k] = 0.0;
+      for (int i = 0; i < mfree; ++i) {
+        const int k = freeidx[i];
+        xn[k] = x[k] + alpha * (xs[i] - x[k]);
+      }
+      for (int k = 0; k < K; ++k) x[k] = xn[k];
+      if (block >= 0) {
+        x[block] = 0.0;
+        fixed[block] = true;
+      }
+    }
+  }
+  return false;
+}
+
+// Basis value n alpha^(3/n)/(4 pi Gamma(3/n)) exp(-alpha r^n) for a radial point.
+__device__ __forceinline__ double basis_radial(double norm, double alpha, double order, double r) {
+  const double arg = (order == 1.0) ? r : ((order == 2.0) ? r * r : pow(r, order));
+  return norm * exp(-alpha * arg);
+}
+
+// ---------------------------------------------------------------------------
+// K4: GISA QP update, one wavefront per atom (gisa.py:281-318, 348-421)
+// ---------------------------------------------------------------------------
+template <int PPL>
+__global__ __launch_bounds__(256) void k_gisa_qp(DevMol m, const double *__restrict__ rhobar, double *__restrict__ par,
+                                                 const double *__restrict__ old_par, const int *__restrict__ fn_off,
+                                                 const double *__restrict__ fn_norm, const double *__restrict__ fn_alpha,
+                                                 const double *__restrict__ fn_order, const double *__restrict__ qpP,
+                                                 const int64_t *__restrict__ qp_off, double pop_cutoff, double neg_cutoff,
+                                                 int *__restrict__ inner_niter, int *__restrict__ status,
+                                                 double *__restrict__ charges, double *__restrict__ change_part) {
+  __shared__ double Mlds[4][(HP_KMAX + 1) * (HP_KMAX + 2)];
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const int a = blockIdx.x * (blockDim.x >> 6) + wv;
+  if (a >= m.natom) return;
+  const int s0 = m.atom_shell_off[a], R = m.atom_shell_off[a + 1] - s0;
+  const int f0 = fn_off[a], K = fn_off[a + 1] - f0;
+  RadialLane<PPL> d;
+  load_radial<PPL>(m, rhobar, s0, R, lane, d);
+  double red[HP_KMAX + 1];
+  for (int k = 0; k <= HP_KMAX; ++k) red[k] = 0.0;
+#pragma unroll
+  for (int j = 0; j < PPL; ++j) {
+    for (int k = 0; k < K; ++k) {
+      const double B = basis_radial(fn_norm[f0 + k], fn_alpha[f0 + k], fn_order[f0 + k], d.r[j]);
+      red[k] += d.w[j] * B * d.rho[j];
+    }
+    red[HP_KMAX] += d.w[j] * d.rho[j];
+  }
+  wave_sum_n<HP_KMAX + 1>(red);
+  const double pop = red[HP_KMAX];
+  double x[HP_KMAX];
+  for (int k = 0; k < HP_KMAX; ++k) x[k] = 0.0;
+  int st = 0;
+  if (lane == 0) {
+    double q[HP_KMAX];
+    for (int k = 0; k < K; ++k) q[k] = -2.0 * red[k];
+    const bool ok = qp_simplex(qpP + qp_off[a], q, pop, K, x, Mlds[wv]);
+    if (!ok) st |= HP_ST_QPFAIL;
+    double sx = 0.0;
+    for (int k = 0; k < K; ++k) {
+      if (x[k] < neg_cutoff) st |= HP_ST_NEGPARS;
+      sx += x[k];
+    }
+    if (fabs(sx - pop) > pop_cutoff) st |= HP_ST_POP;
+    for (int k = 0; k < K; ++k) par[f0 + k] = x[k];
+  }
+  for (int k = 0; k < HP_KMAX; ++k) x[k] = __shfl(k < K ? x[k] : 0.0, 0, 64);
+  // outer change on the radial grid
+  double msd = 0.0;
+#pragma unroll
+  for (int j = 0; j < PPL; ++j) {
+    double y1 = 0.0, y2 = 0.0;
+    for (int k = 0; k < K; ++k) {
+      const double B = basis_radial(fn_norm[f0 + k], fn_alpha[f0 + k], fn_order[f0 + k], d.r[j]);
+      y1 += B * x[k];
+      y2 += B * old_par[f0 + k];
+    }
+    const double e = y1 - y2;
+    msd += d.wr[j] * (4.0 * HP_PI * (d.r[j] * d.r[j])) * e * e;
+  }
+  msd = wave_sum(msd);
+  if (lane == 0) {
+    inner_niter[a] = 1;
+    status[a] = st;
+    charges[a] = m.pseudo[a] - pop;
+    change_part[a] = msd;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// K5: LISA solvers on the radial grid, one wavefront per atom
+// (alisa.py:193-291 solver_sc, 753-938 _solver_general_newton)
+// ---------------------------------------------------------------------------
+
+// For all points of the wave: 1 if any value v[j] (valid points only) fails
+// v[j] > cut, or, with mono, v[i] - v[i+1] > cut fails for a consecutive pair.
+template <int PPL>
+__device__ __forceinline__ bool wave_any_bad(const double *v, int R, int lane, double cut, bool mono) {
+  double bad = 0.0;
+#pragma unroll
+  for (int j = 0; j < PPL; ++j) {
+    const int i = lane + 64 * j;
+    if (i < R && !(v[j] > cut)) bad = 1.0;
+  }
+  if (mono) {
+#pragma unroll
+    for (int j = 0; j < PPL; ++j) {
+      const int i = lane + 64 * j;
+      double nxt = __shfl_down(v[j], 1, 64);
+      if (j + 1 < PPL) {
+        const double first_next = __shfl(v[j + 1 < PPL ? j + 1 : j], 0, 64);
+        if (lane == 63) nxt = first_next;
+      }
+      if (i + 1 < R && !(v[j] - nxt > cut)) bad = 1.0;
+    }
+  }
+  return wave_sum(bad) > 0.0;
+}
+
+template <int PPL, int KM>
+__global__ __launch_bounds__(256) void k_lisa_inner(DevMol m, const double *__restrict__ rhobar, double *__restrict__ par,
+                                                    const double *__restrict__ old_par, const int *__restrict__ fn_off,
+                                                    const double *__restrict__ fn_norm, const double *__restrict__ fn_alpha,
+                                                    const double *__restrict__ fn_order, int solver, double thr, int maxit,
+                                                    double dc, double neg_cutoff, double pop_cutoff, double tau,
+                                                    int nlin, int check_mono, int *__restrict__ inner_niter,
+                                                    int *__restrict__ status, double *__restrict__ charges,
+                                                    double *__restrict__ change_part) {
+  __shared__ double Mlds[4][(HP_KMAX + 1) * (HP_KMAX + 2)];
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const int a = blockIdx.x * (blockDim.x >> 6) + wv;
+  if (a >= m.natom) return;
+  const int s0 = m.atom_shell_off[a], R = m.atom_shell_off[a + 1] - s0;
+  const int f0 = fn_off[a], K = fn_off[a + 1] - f0;
+  RadialLane<PPL> d;
+  load_radial<PPL>(m, rhobar, s0, R, lane, d);
+  double B[KM][PPL];
+#pragma unroll
+  for (int k = 0; k < KM; ++k) {
+#pragma unroll
+    for (int j = 0; j < PPL; ++j)
+      B[k][j] = (k < K && lane + 64 * j < R) ? basis_radial(fn_norm[f0 + k], fn_alpha[f0 + k], fn_order[f0 + k], d.r[j])
+                                              : 0.0;
+  }
+  double pop = 0.0;
+#pragma unroll
+  for (int j = 0; j < PPL; ++j) pop += d.w[j] * d.rho[j];
+  pop = wave_sum(pop);
+  double c[KM];
+#pragma unroll
+  for (int k = 0; k < KM; ++k) c[k] = k < K ? par[f0 + k] : 0.0;
+  double oldpro[PPL];
+#pragma unroll
+  for (int j = 0; j < PPL; ++j) oldpro[j] = 0.0;
+  bool have_old = false, conv = false;
+  int st = 0, it = 0;
+  double change = 1e100;
+  for (it = 0; it < maxit; ++it) {
+    double pro[PPL], ratio[PPL];
+    bool sick[PPL];
+#pragma unroll
+    for (int j = 0; j < PPL; ++j) {
+      double s = 0.0;
+#pragma unroll
+      for (int k = 0; k < KM; ++k)
+        if (k < K) s += B[k][j] * c[k];
+      pro[j] = s;
+      sick[j] = (d.rho[j] < dc) || (s < dc);
+      ratio[j] = sick[j] ? 0.0 : d.rho[j] / s;
+    }
+    if (solver == 2) {
+      // sc: c_k <- sum_p w_p (B_kp c_k) ratio_p
+      double red[KM + 1];
+#pragma unroll
+      for (int k = 0; k <= KM; ++k) red[k] = 0.0;
+#pragma unroll
+      for (int j = 0; j < PPL; ++j) {
+#pragma unroll
+        for (int k = 0; k < KM; ++k) red[k] += d.w[j] * ((B[k][j] * c[k]) * ratio[j]);
+        const double e = oldpro[j] - pro[j];
+        red[KM] += d.w[j] * e * e;
+      }
+      wave_sum_n<KM + 1>(red);
+#pragma unroll
+      for (int k = 0; k < KM; ++k)
+        if (k < K) c[k] = red[k];
+      change = have_old ? sqrt(red[KM]) : 1e100;
+      if (change < thr) {
+        conv = true;
+        break;
+      }
+    } else {
+      if (have_old) {
+        double e2 = 0.0;
+#pragma unroll
+        for (int j = 0; j < PPL; ++j) {
+          const double e = oldpro[j] - pro[j];
+          e2 += d.w[j] * e * e;
+        }
+        change = sqrt(wave_sum(e2));
+      }
+      if (change < thr) {
+        conv = true;
+        // check_pro_atom_parameters_neg_pars (alisa.py:895-904)
+        if (wave_any_bad<PPL>(pro, R, lane, neg_cutoff, false)) {
+          double mn = 1e300;
+#pragma unroll
+          for (int j = 0; j < PPL; ++j)
+            if (lane + 64 * j < R) mn = fmin(mn, pro[j]);
+          if (wave_min(mn) < neg_cutoff) st |= HP_ST_NEGDENS;
+        }
+        if (check_mono) {
+          // monotonic decay: pro[i] - pro[i+1] >= neg_cutoff (utils.py:546)
+          double bad = 0.0;
+#pragma unroll
+          for (int j = 0; j < PPL; ++j) {
+            const int i = lane + 64 * j;
+            double nxt = __shfl_down(pro[j], 1, 64);
+            if (j + 1 < PPL) {
+              const double first_next = __shfl(pro[j + 1 < PPL ? j + 1 : j], 0, 64);
+              if (lane == 63) nxt = first_next;
+            }
+            if (i + 1 < R && (pro[j] - nxt < neg_cutoff)) bad = 1.0;
+          }
+          if (wave_sum(bad) > 0.0) st |= HP_ST_NOTMONO;
+        }
+        break;
+      }
+      // gradient and Hessian (alisa.py:907-924)
+      constexpr int NH = KM * (KM + 1) / 2;
+      double red[KM + NH];
+#pragma unroll
+      for (int q = 0; q < KM + NH; ++q) red[q] = 0.0;
+#pragma unroll
+      for (int j = 0; j < PPL; ++j) {
+#pragma unroll
+        for (int k = 0; k < KM; ++k) {
+          const double integ = B[k][j] * ratio[j];
+          red[k] += integ * d.w[j];
+          const double gk = sick[j] ? 0.0 : integ / pro[j];
+#pragma unroll
+          for (int l = 0; l <= k; ++l) red[KM + k * (k + 1) / 2 + l] += gk * B[l][j] * d.w[j];
+        }
+      }
+      wave_sum_n<KM + NH>(red);
+      bool finite = true;
+#pragma unroll
+      for (int q = 0; q < KM + NH; ++q) finite = finite && isfinite(red[q]);
+      if (!finite) {
+        // scipy.linalg.solve(check_finite=True) raises ValueError
+        st |= HP_ST_NONFINITE;
+        break;
+      }
+      double delta[KM];
+      int solved = 1;
+      if (lane == 0) {
+        double *M = Mlds[wv];
+        const int ld = HP_KMAX + 2;
+#pragma unroll
+        for (int k = 0; k < KM; ++k) {
+#pragma unroll
+          for (int l = 0; l <= k; ++l) {
+            if (k < K) {
+              M[k * ld + l] = red[KM + k * (k + 1) / 2 + l];
+              M[l * ld + k] = red[KM + k * (k + 1) / 2 + l];
+            }
+          }
+          if (k < K) M[k * ld + K] = -1.0 - (-red[k]);
+        }
+        double x[HP_KMAX];
+        for (int k = 0; k < HP_KMAX; ++k) x[k] = 0.0;
+        solved = dense_solve(M, K, ld, x) ? 1 : 0;
+#pragma unroll
+        for (int k = 0; k < KM; ++k) delta[k] = k < K ? x[k] : 0.0;
+      }
+      if (!__shfl(solved, 0, 64)) {
+        // exactly singular Hessian: scipy.linalg.solve raises LinAlgError
+        st |= HP_ST_SINGULAR;
+        break;
+      }
+#pragma unroll
+      for (int k = 0; k < KM; ++k) delta[k] = __shfl(lane == 0 ? delta[k] : 0.0, 0, 64);
+      if (solver == 3) {
+#pragma unroll
+        for (int k = 0; k < KM; ++k) c[k] = c[k] + delta[k];
+      } else {
+        // modified: first x in linspace(tau, 0, nlin) keeping pro > neg_cutoff
+        const double step = (0.0 - tau) / (double)(nlin - 1);
+        bool found = false;
+        for (int t = 0; t < nlin && !found; ++t) {
+          const double xs = (t == nlin - 1) ? 0.0 : (double)t * step + tau;
+          double cn[KM];
+#pragma unroll
+          for (int k = 0; k < KM; ++k) cn[k] = c[k] + xs * delta[k];
+          double np_[PPL];
+#pragma unroll
+          for (int j = 0; j < PPL; ++j) {
+            double s = 0.0;
+#pragma unroll
+            for (int k = 0; k < KM; ++k)
+              if (k < K) s += B[k][j] * cn[k];
+            np_[j] = s;
+          }
+          if (!wave_any_bad<PPL>(np_, R, lane, neg_cutoff, check_mono != 0)) {
+            found = true;
+#pragma unroll
+            for (int k = 0; k < KM; ++k) c[k] = cn[k];
+          }
+        }
+        if (!found) {
+          st |= HP_ST_LINESEARCH;
+          break;
+        }
+      }
+    }
+#pragma unroll
+    for (int j = 0; j < PPL; ++j) oldpro[j] = pro[j];
+    have_old = true;
+  }
+  if (!conv) {
+    if (solver == 2)
+      st |= HP_ST_NOCONV;
+    else if (!(st & (HP_ST_LINESEARCH | HP_ST_SINGULAR | HP_ST_NONFINITE)))
+      st |= HP_ST_NEWTONMAX;
+  } else {
+    double sx = 0.0;
+#pragma unroll
+    for (int k = 0; k < KM; ++k)
+      if (k < K) sx += c[k];
+    if (fabs(sx - pop) > pop_cutoff) st |= HP_ST_POP;
+  }
+  double msd = 0.0;
+#pragma unroll
+  for (int j = 0; j < PPL; ++j) {
+    double y1 = 0.0, y2 = 0.0;
+#pragma unroll
+    for (int k = 0; k < KM; ++k) {
+      if (k < K) {
+        y1 += B[k][j] * c[k];
+        y2 += B[k][j] * old_par[f0 + k];
+      }
+    }
+    const double e = y1 - y2;
+    msd += d.wr[j] * (4.0 * HP_PI * (d.r[j] * d.r[j])) * e * e;
+  }
+  msd = wave_sum(msd);
+  if (lane == 0) {
+#pragma unroll
+    for (int k = 0; k < KM; ++k)
+      if (k < K) par[f0 + k] = c[k];
+    inner_niter[a] = conv ? it + 1 : it;
+    status[a] = st;
+    charges[a] = m.pseudo[a] - pop;
+    change_part[a] = msd;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// K3: per-iteration finalisation (single workgroup)
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_finalize(int nshell, const double *__restrict__ ent_part, int natom,
+                                                  const double *__restrict__ change_part, const int *__restrict__ status,
+                                                  int *__restrict__ status_all, const double *__restrict__ par, int npar,
+                                                  const double *__restrict__ charges, int iter, double *__restrict__ hpar,
+                                                  double *__restrict__ hch, double *__restrict__ hent,
+                                                  double *__restrict__ hchg, double *__restrict__ out) {
+  __shared__ double lds[32];
+  double e = 0.0, dummy = 0.0;
+  for (int s = threadIdx.x; s < nshell; s += blockDim.x) e += ent_part[s];
+  block_sum2(e, dummy, lds);
+  for (int i = threadIdx.x; i < npar; i += blockDim.x) hpar[(int64_t)iter * npar + i] = par[i];
+  for (int i = threadIdx.x; i < natom; i += blockDim.x) {
+    hch[(int64_t)iter * natom + i] = charges[i];
+    status_all[i] |= status[i];
+  }
+  if (threadIdx.x == 0) {
+    double msd = 0.0;
+    int err = 0, any = 0;
+    for (int a = 0; a < natom; ++a) {
+      msd += change_part[a];
+      err |= status[a] & (HP_ST_NEGDENS | HP_ST_LINESEARCH | HP_ST_NEWTONMAX | HP_ST_QPFAIL | HP_ST_NOTMONO |
+                          HP_ST_SINGULAR | HP_ST_NONFINITE);
+      any |= status[a];
+    }
+    const double ch = sqrt(msd);
+    hent[iter] = e;
+    hchg[iter] = ch;
+    out[0] = ch;
+    out[1] = e;
+    out[2] = (double)err;
+    out[3] = (double)any;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// moments (lmax <= 3): one workgroup per radial shell, 40 partial sums
+// ---------------------------------------------------------------------------
+#define HP_NMOM 40
+template <int FK>
+__global__ __launch_bounds__(256) void k_moments(DevMol m, DevFn fn, double *__restrict__ part) {
+  __shared__ double lds[HP_NMOM * 4];
+  const int s = blockIdx.x;
+  const int a = m.shell_atom[s];
+  const int64_t p0 = m.shell_off[s], p1 = m.shell_off[s + 1];
+  double acc[HP_NMOM];
+  for (int i = 0; i < HP_NMOM; ++i) acc[i] = 0.0;
+  DevFn f2 = fn;
+  f2.nb_list = nullptr;
+  for (int64_t p = p0 + threadIdx.x; p < p1; p += blockDim.x) {
+    double promol, own;
+    promol_at<double, FK>(m, f2, a, m.natom, nullptr, m.px[p], m.py[p], m.pz[p], promol, own);
+    double w = own / promol;
+    w = fmin(fmax(w, 0.0), 1.0);
+    const double aim = m.rho[p] * w;
+    const double wf = m.wat[p] * aim;
+    const double x = m.px[p] - m.cx[a], y = m.py[p] - m.cy[a], z = m.pz[p] - m.cz[a];
+    const double xx = x * x, yy = y * y, zz = z * z, r2 = xx + yy + zz;
+    // cartesian, HORTON order
+    const double c[20] = {1.0,        x,          y,          z,          xx,         x * y,      x * z,
+                          yy,         y * z,      zz,         xx * x,     xx * y,     xx * z,     x * yy,
+                          x * y * z,  x * zz,     yy * y,     yy * z,     y * zz,     zz * z};
+    for (int i = 0; i < 20; ++i) acc[i] += wf * c[i];
+    // pure: real regular solid harmonics (Racah), c_l0, c_l1, s_l1, ...
+    const double s3 = 1.7320508075688772, s15 = 3.872983346207417;
+    const double s38 = 0.6123724356957945, s58 = 0.7905694150420949;
+    const double pu[16] = {1.0,
+                           z,
+                           x,
+                           y,
+                           0.5 * (3.0 * zz - r2),
+                           s3 * x * z,
+                           s3 * y * z,
+                           0.5 * s3 * (xx - yy),
+                           s3 * x * y,
+                           0.5 * z * (5.0 * zz - 3.0 * r2),
+                           s38 * x * (5.0 * zz - r2),
+                           s38 * y * (5.0 * zz - r2),
+                           0.5 * s15 * z * (xx - yy),
+                           s15 * x * y * z,
+                           s58 * x * (xx - 3.0 * yy),
+                           s58 * y * (3.0 * xx - yy)};
+    for (int i = 0; i < 16; ++i) acc[20 + i] += wf * pu[i];
+    const double r = sqrt(r2);
+    acc[36] += wf;
+    acc[37] += wf * r;
+    acc[38] += wf * r2;
+    acc[39] += wf * r2 * r;
+  }
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6, nw = blockDim.x >> 6;
+  for (int i = 0; i < HP_NMOM; ++i) {
+    double v = acc[i];
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_down(v, o, 64);
+    if (lane == 0) lds[i * 4 + wv] = v;
+  }
+  __syncthreads();
+  if (threadIdx.x < HP_NMOM) {
+    double v = 0.0;
+    for (int w = 0; w < nw; ++w) v += lds[threadIdx.x * 4 + w];
+    part[(int64_t)s * HP_NMOM + threadIdx.x] = v;
+  }
+}
+
+__global__ void k_moments_reduce(DevMol m, const double *__restrict__ part, double *__restrict__ out) {
+  const int a = blockIdx.x;
+  const int i = threadIdx.x;
+  if (i >= HP_NMOM) return;
+  double v = 0.0;
+  for (int s = m.atom_shell_off[a]; s < m.atom_shell_off[a + 1]; ++s) v += part[(int64_t)s * HP_NMOM + i];
+  out[(int64_t)a * HP_NMOM + i] = v;
+}
+
+// ---------------------------------------------------------------------------
+// grid utilities
+// ---------------------------------------------------------------------------
+
+// Becke fuzzy-cell weight of each point for the atom whose grid holds it
+// (A. D. Becke, J. Chem. Phys. 88 (1988) 2547; size-adjusted, order-3 step).
+// adj[i*A+j] = a_ij, invr[i*A+j] = 1/|R_i - R_j|.
+__global__ void k_becke(int natom, const double *__restrict__ cx, const double *__restrict__ cy,
+                        const double *__restrict__ cz, const double *__restrict__ adj, const double *__restrict__ invr,
+                        int64_t npoint, const double *__restrict__ px, const double *__restrict__ py,
+                        const double *__restrict__ pz, const int64_t *__restrict__ atom_off, int order,
+                        double *__restrict__ out) {
+  const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= npoint) return;
+  int lo = 0, hi = natom;  // owner: atom_off[lo] <= p < atom_off[lo+1]
+  while (hi - lo > 1) {
+    const int mid = (lo + hi) >> 1;
+    if (atom_off[mid] <= p)
+      lo = mid;
+    else
+      hi = mid;
+  }
+  const double x = px[p], y = py[p], z = pz[p];
+  double sum = 0.0, own = 0.0;
+  for (int i = 0; i < natom; ++i) {
+    const double dxi = x - cx[i], dyi = y - cy[i], dzi = z - cz[i];
+    const double di = sqrt(dxi * dxi + dyi * dyi + dzi * dzi);
+    double P = 1.0;
+    for (int j = 0; j < natom && P != 0.0; ++j) {
+      if (j == i) continue;
+      const double dxj = x - cx[j], dyj = y - cy[j], dzj = z - cz[j];
+      const double dj = sqrt(dxj * dxj + dyj * dyj + dzj * dzj);
+      const double mu = (di - dj) * invr[(int64_t)i * natom + j];
+      double f = mu + adj[(int64_t)i * natom + j] * (1.0 - mu * mu);
+      for (int k = 0; k < order; ++k) f = 1.5 * f - 0.5 * (f * f * f);
+      P *= 0.5 * (1.0 - f);
+    }
+    sum += P;
+    if (i == lo) own = P;
+  }
+  out[p] = natom == 1 ? 1.0 : own / sum;
+}
+
+// sum_a sum_k coef_ak exp(-alpha_ak |p - R_a|) at every point.
+__global__ void k_slater_sum(int natom, const double *__restrict__ cx, const double *__restrict__ cy,
+                             const double *__restrict__ cz, const int *__restrict__ fn_off,
+                             const double *__restrict__ coef, const double *__restrict__ alpha, int64_t npoint,
+                             const double *__restrict__ px, const double *__restrict__ py,
+                             const double *__restrict__ pz, double *__restrict__ out) {
+  const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= npoint) return;
+  const double x = px[p], y = py[p], z = pz[p];
+  double acc = 0.0;
+  for (int a = 0; a < natom; ++a) {
+    const double dx = x - cx[a], dy = y - cy[a], dz = z - cz[a];
+    const double r = sqrt(dx * dx + dy * dy + dz * dz);
+    for (int f = fn_off[a]; f < fn_off[a + 1]; ++f) acc += coef[f] * exp(-alpha[f] * r);
+  }
+  out[p] = acc;
+}

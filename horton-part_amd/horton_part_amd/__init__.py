@@ -1,0 +1,30 @@
+"""MI355X-native stockholder partitioning (MBIS / GISA / aLISA).
+
+Drop-in for the grid-based stockholder loop of horton-part: the scheme
+classes keep the reference's ``WPart`` API and run the loop in
+``libhortonpart.so`` (hand-written HIP kernels for gfx950).
+"""
+
+from .alisa import LinearISAWPart
+from .basis import ExpBasisFuncHelper, evaluate_function
+from .gisa import GaussianISAWPart
+from .grid import AtomGrid, BeckeWeights, ExpRTransform, MolGrid, OneDGrid, UniformInteger
+from .mbis import MBISWPart, get_initial_mbis_propars, get_nshell
+from .utils import wpart_schemes
+
+__all__ = [
+    "MBISWPart",
+    "GaussianISAWPart",
+    "LinearISAWPart",
+    "wpart_schemes",
+    "get_nshell",
+    "get_initial_mbis_propars",
+    "ExpBasisFuncHelper",
+    "evaluate_function",
+    "MolGrid",
+    "AtomGrid",
+    "BeckeWeights",
+    "ExpRTransform",
+    "OneDGrid",
+    "UniformInteger",
+]

@@ -1,0 +1,317 @@
+"""ctypes binding of ``libhortonpart.so`` (declared in ``include/hortonpart.h``).
+
+The library is the only compute path of this package: there is no CPU
+fallback.  ``load()`` raises ``RuntimeError`` when the shared library is
+missing or fails to load, and the scheme classes call it before any work.
+"""
+
+import ctypes
+import os
+from pathlib import Path
+
+import numpy as np
+
+__all__ = ["load", "lib_path", "HPError", "EXPORTED_SYMBOLS", "MoleculeDesc", "SchemeDesc"]
+
+HP_E_ARG = -1
+HP_E_HIP = -2
+HP_E_SOLVER = -3
+HP_E_UNSUPP = -4
+
+SCHEME = {"mbis": 0, "gisa": 1, "lisa": 2}
+SOLVER = {"mbis": 0, "qp": 1, "sc": 2, "newton": 3, "m-newton": 4}
+PREC = {"fp64": 0, "mixed": 1}
+
+ST_INNER_NOT_CONVERGED = 1
+ST_POP_MISMATCH = 2
+ST_NEG_DENSITY = 4
+ST_LINESEARCH = 8
+ST_NEWTON_MAXITER = 16
+ST_QP_FAILED = 32
+ST_NEG_PARS = 64
+ST_NOT_MONOTONIC = 128
+ST_SINGULAR = 256
+ST_NONFINITE = 512
+
+KERNEL_WEIGHTS = 0
+KERNEL_INNER = 1
+
+EXPORTED_SYMBOLS = [
+    "hp_abi_version",
+    "hp_last_error",
+    "hp_device_count",
+    "hp_part_create",
+    "hp_part_destroy",
+    "hp_part_set_propars",
+    "hp_part_run",
+    "hp_part_results",
+    "hp_part_history",
+    "hp_part_status",
+    "hp_part_spherical_averages",
+    "hp_part_at_weights",
+    "hp_part_promoldens",
+    "hp_part_moments",
+    "hp_part_kernel_time",
+    "hp_becke_weights",
+    "hp_slater_density",
+]
+
+_P = ctypes.POINTER
+_d = ctypes.c_double
+_pd = _P(ctypes.c_double)
+_pi32 = _P(ctypes.c_int32)
+_pi64 = _P(ctypes.c_int64)
+
+
+class MoleculeDesc(ctypes.Structure):
+    _fields_ = [
+        ("natom", ctypes.c_int),
+        ("coordinates", _pd),
+        ("numbers", _pi64),
+        ("pseudo_numbers", _pd),
+        ("npoint", ctypes.c_int64),
+        ("points", _pd),
+        ("density", _pd),
+        ("mol_weights", _pd),
+        ("atom_weights", _pd),
+        ("sph_weights", _pd),
+        ("nshell", ctypes.c_int),
+        ("shell_atom", _pi32),
+        ("shell_offsets", _pi64),
+        ("shell_radius", _pd),
+        ("shell_rweight", _pd),
+    ]
+
+
+class SchemeDesc(ctypes.Structure):
+    _fields_ = [
+        ("scheme", ctypes.c_int),
+        ("solver", ctypes.c_int),
+        ("grid_type", ctypes.c_int),
+        ("fn_offsets", _pi32),
+        ("fn_alpha", _pd),
+        ("fn_order", _pd),
+        ("fn_norm", _pd),
+        ("qp_matrix", _pd),
+        ("threshold", _d),
+        ("inner_threshold", _d),
+        ("maxiter", ctypes.c_int),
+        ("inner_maxiter", ctypes.c_int),
+        ("density_cutoff", _d),
+        ("negative_cutoff", _d),
+        ("population_cutoff", _d),
+        ("cutoff_radius", _d),
+        ("newton_tau", _d),
+        ("newton_linspace", ctypes.c_int),
+        ("check_mono", ctypes.c_int),
+    ]
+
+
+class HPError(RuntimeError):
+    """Failure reported by the device library (message from hp_last_error)."""
+
+    def __init__(self, code, message):
+        super().__init__(f"[hortonpart {code}] {message}")
+        self.code = code
+
+
+def lib_path():
+    env = os.environ.get("HORTONPART_LIB")
+    if env:
+        return Path(env)
+    return Path(__file__).resolve().parent / "lib" / "libhortonpart.so"
+
+
+_LIB = None
+
+
+def load():
+    """Load the device library once; raise ``RuntimeError`` if it is absent."""
+    global _LIB
+    if _LIB is not None:
+        return _LIB
+    path = lib_path()
+    if not path.exists():
+        raise RuntimeError(
+            f"libhortonpart.so not found at {path}; build it with `python -c 'import __graft_entry__ as g; g.build()'`"
+        )
+    try:
+        lib = ctypes.CDLL(str(path))
+    except OSError as exc:  # pragma: no cover - depends on the ROCm runtime
+        raise RuntimeError(f"cannot load {path}: {exc}") from exc
+    lib.hp_abi_version.restype = ctypes.c_int
+    lib.hp_last_error.restype = ctypes.c_char_p
+    lib.hp_device_count.argtypes = [_pi32]
+    lib.hp_part_create.argtypes = [ctypes.c_int, _P(MoleculeDesc), _P(SchemeDesc), _P(ctypes.c_void_p)]
+    lib.hp_part_destroy.argtypes = [ctypes.c_void_p]
+    lib.hp_part_set_propars.argtypes = [ctypes.c_void_p, _pd, ctypes.c_int64]
+    lib.hp_part_run.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, _pi32]
+    lib.hp_part_results.argtypes = [ctypes.c_void_p, _pd, _pd, _pd, _pd]
+    lib.hp_part_history.argtypes = [ctypes.c_void_p, _pd, _pd, _pd, _pd]
+    lib.hp_part_status.argtypes = [ctypes.c_void_p, _pi32, _pi32, _pi32]
+    lib.hp_part_spherical_averages.argtypes = [ctypes.c_void_p, _pd]
+    lib.hp_part_at_weights.argtypes = [ctypes.c_void_p, ctypes.c_int, _pd, ctypes.c_int64]
+    lib.hp_part_promoldens.argtypes = [ctypes.c_void_p, _pd, ctypes.c_int64]
+    lib.hp_part_moments.argtypes = [ctypes.c_void_p, ctypes.c_int, _pd, _pd, _pd]
+    lib.hp_part_kernel_time.argtypes = [ctypes.c_void_p, ctypes.c_int, _pd, _pi64]
+    lib.hp_becke_weights.argtypes = [ctypes.c_int, ctypes.c_int, _pd, _pd, ctypes.c_int64, _pd, _pi64, _pd]
+    lib.hp_slater_density.argtypes = [ctypes.c_int, ctypes.c_int, _pd, _pi32, _pd, _pd, ctypes.c_int64, _pd, _pd]
+    for name in EXPORTED_SYMBOLS:
+        if name not in ("hp_abi_version", "hp_last_error"):
+            getattr(lib, name).restype = ctypes.c_int
+    if lib.hp_abi_version() != 1:
+        raise RuntimeError("libhortonpart ABI version mismatch")
+    _LIB = lib
+    return lib
+
+
+def check(rc):
+    if rc != 0:
+        msg = _LIB.hp_last_error().decode() if _LIB is not None else "library not loaded"
+        raise HPError(rc, msg)
+
+
+def ptr(a, ctype=ctypes.c_double):
+    """Pointer to a C-contiguous numpy array (None -> NULL)."""
+    if a is None:
+        return None
+    return a.ctypes.data_as(_P(ctype))
+
+
+def device_count():
+    lib = load()
+    n = ctypes.c_int32(0)
+    rc = lib.hp_device_count(ctypes.byref(n))
+    return int(n.value) if rc == 0 else 0
+
+
+def becke_weights(coordinates, radii, points, atom_offsets, device=0):
+    """Becke weights of every point for its owning atom, computed on the GPU."""
+    lib = load()
+    coordinates = np.ascontiguousarray(coordinates, dtype=float)
+    radii = np.ascontiguousarray(radii, dtype=float)
+    points = np.ascontiguousarray(points, dtype=float)
+    atom_offsets = np.ascontiguousarray(atom_offsets, dtype=np.int64)
+    out = np.empty(len(points))
+    check(lib.hp_becke_weights(int(device), len(coordinates), ptr(coordinates), ptr(radii), len(points), ptr(points), ptr(atom_offsets, ctypes.c_int64), ptr(out)))
+    return out
+
+
+def slater_density(coordinates, fn_offsets, coef, alpha, points, device=0):
+    """sum_a sum_k coef_ak exp(-alpha_ak r_a) at every point, on the GPU."""
+    lib = load()
+    coordinates = np.ascontiguousarray(coordinates, dtype=float)
+    fn_offsets = np.ascontiguousarray(fn_offsets, dtype=np.int32)
+    coef = np.ascontiguousarray(coef, dtype=float)
+    alpha = np.ascontiguousarray(alpha, dtype=float)
+    points = np.ascontiguousarray(points, dtype=float)
+    out = np.empty(len(points))
+    check(lib.hp_slater_density(int(device), len(coordinates), ptr(coordinates), ptr(fn_offsets, ctypes.c_int32), ptr(coef), ptr(alpha), len(points), ptr(points), ptr(out)))
+    return out
+
+
+class DevicePartition:
+    """Owner of one ``hp_part`` handle: a molecule and scheme resident on a GPU."""
+
+    def __init__(self, mol_arrays, scheme_arrays, device=0):
+        self.lib = load()
+        self._keep = []  # keep host arrays alive during create
+        md = MoleculeDesc()
+        sd = SchemeDesc()
+        self._fill(md, mol_arrays)
+        self._fill(sd, scheme_arrays)
+        handle = ctypes.c_void_p()
+        check(self.lib.hp_part_create(int(device), ctypes.byref(md), ctypes.byref(sd), ctypes.byref(handle)))
+        self._keep = []
+        self.handle = handle
+        self.natom = mol_arrays["natom"]
+        self.nshell = mol_arrays["nshell"]
+        self.npoint = mol_arrays["npoint"]
+        self.npar = scheme_arrays["npar"]
+        self.niter = 0
+
+    def _fill(self, desc, arrays):
+        for name, ctype in desc._fields_:
+            if name not in arrays:
+                continue
+            v = arrays[name]
+            if isinstance(v, np.ndarray):
+                v = np.ascontiguousarray(v)
+                self._keep.append(v)
+                base = ctype._type_
+                setattr(desc, name, v.ctypes.data_as(_P(base)))
+            else:
+                setattr(desc, name, v)
+
+    def close(self):
+        if getattr(self, "handle", None) is not None and self.handle.value:
+            self.lib.hp_part_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_propars(self, propars):
+        propars = np.ascontiguousarray(propars, dtype=float)
+        check(self.lib.hp_part_set_propars(self.handle, ptr(propars), propars.size))
+
+    def run(self, precision="fp64", n_polish=1):
+        n = ctypes.c_int32(0)
+        rc = self.lib.hp_part_run(self.handle, PREC[precision], int(n_polish), ctypes.byref(n))
+        self.niter = int(n.value)
+        return rc
+
+    def results(self):
+        propars = np.empty(self.npar)
+        charges = np.empty(self.natom)
+        change = ctypes.c_double()
+        entropy = ctypes.c_double()
+        check(self.lib.hp_part_results(self.handle, ptr(propars), ptr(charges), ctypes.byref(change), ctypes.byref(entropy)))
+        return propars, charges, change.value, entropy.value
+
+    def history(self):
+        n = self.niter
+        hp = np.empty((n, self.npar))
+        hc = np.empty((n, self.natom))
+        he = np.empty(n)
+        hg = np.empty(n)
+        check(self.lib.hp_part_history(self.handle, ptr(hp), ptr(hc), ptr(he), ptr(hg)))
+        return hp, hc, he, hg
+
+    def status(self):
+        it = np.zeros(self.natom, np.int32)
+        last = np.zeros(self.natom, np.int32)
+        allst = np.zeros(self.natom, np.int32)
+        check(self.lib.hp_part_status(self.handle, ptr(it, ctypes.c_int32), ptr(last, ctypes.c_int32), ptr(allst, ctypes.c_int32)))
+        return it, last, allst
+
+    def spherical_averages(self):
+        out = np.empty(self.nshell)
+        check(self.lib.hp_part_spherical_averages(self.handle, ptr(out)))
+        return out
+
+    def at_weights(self, atom, size):
+        out = np.empty(size)
+        check(self.lib.hp_part_at_weights(self.handle, int(atom), ptr(out), size))
+        return out
+
+    def promoldens(self):
+        out = np.empty(self.npoint)
+        check(self.lib.hp_part_promoldens(self.handle, ptr(out), self.npoint))
+        return out
+
+    def moments(self, lmax=3):
+        cart = np.empty((self.natom, 20))
+        pure = np.empty((self.natom, 16))
+        rad = np.empty((self.natom, 4))
+        check(self.lib.hp_part_moments(self.handle, int(lmax), ptr(cart), ptr(pure), ptr(rad)))
+        return cart, pure, rad
+
+    def kernel_time(self, kernel=KERNEL_WEIGHTS):
+        ms = ctypes.c_double()
+        n = ctypes.c_int64()
+        check(self.lib.hp_part_kernel_time(self.handle, int(kernel), ctypes.byref(ms), ctypes.byref(n)))
+        return ms.value, n.value

@@ -1,0 +1,474 @@
+"""Partitioning base classes with the reference's WPart API, on the GPU.
+
+``Part``/``WPart`` restate ``core/base.py:35-692`` (geometry checks, grid
+types, cache, ``do_populations``/``do_charges``/``do_moments``/``do_all``);
+``AbstractISAWPart`` restates the outer loop of ``core/iterstock.py:61-193``.
+The loop itself runs in ``libhortonpart.so`` (``hp_part_run``); this class
+prepares the device description of the molecule and scheme, uploads the
+initial parameters, and turns the device results, histories and status
+bits back into the reference's cache entries, warnings and exceptions.
+"""
+
+import logging
+import warnings
+
+import numpy as np
+
+from . import _native
+from .cache import Cache, JustOnceClass, just_once
+from .utils import DENSITY_CUTOFF, NEGATIVE_CUTOFF, POPULATION_CUTOFF, typecheck_geo
+
+__all__ = ["Part", "WPart", "AbstractISAWPart", "get_ncart_cumul", "get_npure_cumul"]
+
+
+def get_ncart_cumul(lmax):
+    """Number of Cartesian powers up to ``lmax`` (core/base.py:685-687)."""
+    return (lmax + 1) * (lmax + 2) * (lmax + 3) // 6
+
+
+def get_npure_cumul(lmax):
+    """Number of pure functions up to ``lmax`` (core/base.py:690-692)."""
+    return (lmax + 1) ** 2
+
+
+class Part(JustOnceClass):
+    """Geometry, grid and cache of a density partitioning (core/base.py:35-410)."""
+
+    name = None
+
+    def __init__(self, coordinates, numbers, pseudo_numbers, grid, moldens, spindens, local, lmax, logger, *args, **kwargs):
+        super().__init__()
+        natom, coordinates, numbers, pseudo_numbers = typecheck_geo(coordinates, numbers, pseudo_numbers)
+        self._natom = natom
+        self._coordinates = coordinates
+        self._numbers = numbers
+        self._pseudo_numbers = pseudo_numbers
+        self._grid = grid
+        self._moldens = moldens
+        self._spindens = spindens
+        self._local = local
+        self._lmax = lmax
+        self._cache = Cache()
+        self.logger = logger
+        if local:
+            self._subgrids = self._grid.atgrids
+
+    def __getitem__(self, key):
+        return self.cache.load(key)
+
+    def variables_stored_in_cache(self):
+        return list(self.cache.iterkeys())
+
+    @property
+    def natom(self):
+        return self._natom
+
+    @property
+    def coordinates(self):
+        return self._coordinates
+
+    @property
+    def numbers(self):
+        return self._numbers
+
+    @property
+    def pseudo_numbers(self):
+        return self._pseudo_numbers
+
+    @property
+    def nelec(self):
+        """Number of electrons: molecular-grid integral of the density."""
+        return self.grid.integrate(self._moldens)
+
+    @property
+    def grid(self):
+        return self.get_grid()
+
+    @property
+    def local(self):
+        return self._local
+
+    @property
+    def lmax(self):
+        return self._lmax
+
+    @property
+    def cache(self):
+        return self._cache
+
+    def clear(self):
+        JustOnceClass.clear(self)
+        self.cache.clear()
+
+    def get_grid(self, index=None):
+        if index is None or not self.local:
+            return self._grid
+        return self._subgrids[index]
+
+    def to_atomic_grid(self, index, data):
+        if index is None or not self.local:
+            return data
+        begin, end = self._grid.indices[index], self._grid.indices[index + 1]
+        return data[begin:end]
+
+    def get_moldens(self, index=None, output=None):
+        result = self._moldens if (index is None or not self.local) else self.to_atomic_grid(index, self._moldens)
+        if output is not None:
+            output[:] = result
+        return result
+
+    def get_spindens(self, index=None, output=None):
+        result = self._spindens if (index is None or not self.local) else self.to_atomic_grid(index, self._spindens)
+        if output is not None:
+            output[:] = result
+        return result
+
+    def get_wcor(self, index):
+        return 1.0
+
+    @just_once
+    def do_partitioning(self):
+        raise NotImplementedError
+
+    @just_once
+    def do_populations(self):
+        """Atomic populations (core/base.py:277-300): no-op after the ISA loop."""
+        populations, new = self.cache.load("populations", alloc=self.natom, tags="o")
+        if new:
+            self.do_partitioning()
+            pseudo = self.cache.load("pseudo_populations", alloc=self.natom, tags="o")[0]
+            for index in range(self.natom):
+                grid = self.get_grid(index)
+                at_weights = self.cache.load(f"at_weights_{index}")
+                pseudo[index] = grid.integrate(at_weights, self.get_moldens(index))
+            populations[:] = pseudo
+            populations += self.numbers - self.pseudo_numbers
+
+    @just_once
+    def do_charges(self):
+        """Atomic charges (core/base.py:302-311)."""
+        charges, new = self._cache.load("charges", alloc=self.natom, tags="o")
+        if new:
+            self.do_populations()
+            charges[:] = self.numbers - self._cache.load("populations")
+
+    @just_once
+    def do_spin_charges(self):
+        """Spin charges (core/base.py:313-327) from the device AIM weights."""
+        if self._spindens is not None:
+            spin_charges, _ = self._cache.load("spin_charges", alloc=self.natom, tags="o")
+            self.do_partitioning()
+            for index in range(self.natom):
+                grid = self.get_grid(index)
+                at_weights = self.cache.load(f"at_weights_{index}")
+                spin_charges[index] = grid.integrate(at_weights, self.get_spindens(index))
+
+    @just_once
+    def do_moments(self):
+        """AIM Cartesian, pure and radial moments (core/base.py:329-402)."""
+        self.do_partitioning()
+        cart, pure, rad = self._device_moments()
+        self.cache.dump("cartesian_multipoles", cart, tags="o")
+        self.cache.dump("pure_multipoles", pure, tags="o")
+        self.cache.dump("radial_moments", rad, tags="o")
+
+    def _device_moments(self):
+        raise NotImplementedError
+
+    def do_all(self):
+        """Run every ``do_*`` method; return the output keys (core/base.py:404-410)."""
+        for attr_name in dir(self):
+            attr = getattr(self, attr_name)
+            if callable(attr) and attr_name.startswith("do_") and attr_name != "do_all":
+                attr()
+        return list(self.cache.iterkeys(tags="o"))
+
+
+class WPart(Part):
+    """Grid-based partitioning with grid types 1/2/3 (core/base.py:413-635)."""
+
+    def __init__(
+        self,
+        coordinates,
+        numbers,
+        pseudo_numbers,
+        grid,
+        moldens,
+        spindens=None,
+        lmax=3,
+        logger=None,
+        grid_type=1,
+        density_cutoff=DENSITY_CUTOFF,
+        negative_cutoff=NEGATIVE_CUTOFF,
+        population_cutoff=POPULATION_CUTOFF,
+        **kwargs,
+    ):
+        self._grid_type = grid_type
+        assert grid_type in [1, 2, 3]
+        self._on_molgrid = grid_type in [2, 3]
+        self._only_use_molgrid = grid_type == 3
+        local = not self._only_use_molgrid
+        if local and grid.atgrids is None:
+            raise ValueError("Atomic grids are discarded from molecular grid object, but are needed for local integrations.")
+        if logger is None:
+            logger = logging.getLogger(self.__class__.__name__)
+        super().__init__(coordinates, numbers, pseudo_numbers, grid, moldens, spindens, local, lmax, logger)
+        self.history_propars = []
+        self.history_charges = []
+        self.history_entropies = []
+        self.history_changes = []
+        self.history_time_update_at_weights = []
+        self.history_time_update_propars = []
+        self._density_cutoff = density_cutoff
+        self._population_cutoff = population_cutoff
+        self._negative_cutoff = negative_cutoff
+
+    @property
+    def grid_type(self):
+        return self._grid_type
+
+    @property
+    def on_molgrid(self):
+        return self._on_molgrid
+
+    @property
+    def only_use_molgrid(self):
+        return self._only_use_molgrid
+
+    @property
+    def density_cutoff(self):
+        return self._density_cutoff
+
+    @property
+    def population_cutoff(self):
+        return self._population_cutoff
+
+    @property
+    def negative_cutoff(self):
+        return self._negative_cutoff
+
+    @property
+    def radial_distances(self):
+        """Distances of every molecular point to every atom (host arrays, on demand)."""
+        if not hasattr(self, "_radial_distances_host"):
+            self._radial_distances_host = [np.linalg.norm(self.grid.points - c, axis=1) for c in self.coordinates]
+        return self._radial_distances_host
+
+
+class AbstractISAWPart(WPart):
+    """Iterative stockholder outer loop on the GPU (core/iterstock.py:61-193).
+
+    Extra keyword arguments (defaults reproduce the reference):
+      precision: "fp64" (default) or "mixed" (FP32 grid arithmetic + FP64 polish)
+      n_polish: FP64 iterations after a mixed run (default 1)
+      cutoff_radius: pro-atom tails beyond this distance are zero (default None)
+      device: GPU ordinal (default 0)
+    """
+
+    _scheme_name = None
+    _solver_name = None
+
+    def __init__(
+        self,
+        coordinates,
+        numbers,
+        pseudo_numbers,
+        grid,
+        moldens,
+        spindens=None,
+        lmax=3,
+        logger=None,
+        threshold=1e-6,
+        maxiter=500,
+        inner_threshold=1e-8,
+        grid_type=1,
+        precision="fp64",
+        n_polish=1,
+        cutoff_radius=None,
+        device=0,
+        **kwargs,
+    ):
+        self._threshold = threshold
+        self._inner_threshold = inner_threshold if inner_threshold < threshold else threshold
+        self._maxiter = maxiter
+        if precision not in ("fp64", "mixed"):
+            raise ValueError("precision must be 'fp64' or 'mixed'")
+        self._precision = precision
+        self._n_polish = n_polish
+        self._cutoff_radius = cutoff_radius
+        self._device = device
+        self._dev = None
+        super().__init__(coordinates, numbers, pseudo_numbers, grid, moldens, spindens, lmax, logger, grid_type=grid_type, **kwargs)
+
+    # -- device description ---------------------------------------------------
+    def _molecule_arrays(self):
+        grid = self.grid
+        if grid.atgrids is None:
+            raise NotImplementedError("the device path needs the atom grids (grid_type 1)")
+        indices = np.asarray(grid.indices, dtype=np.int64)
+        shell_atom, shell_off, shell_r, shell_w, sph, atw = [], [0], [], [], [], []
+        for a, atgrid in enumerate(grid.atgrids):
+            if atgrid.size != indices[a + 1] - indices[a]:
+                raise ValueError("molecular grid is not the concatenation of its atom grids")
+            r = atgrid.rgrid.points
+            w = atgrid.rgrid.weights
+            aidx = np.asarray(atgrid.indices, dtype=np.int64)
+            for i in range(len(r)):
+                shell_atom.append(a)
+                shell_off.append(indices[a] + aidx[i + 1])
+            shell_r.append(r)
+            shell_w.append(w)
+            if hasattr(atgrid, "sph_weights"):
+                sph.append(np.asarray(atgrid.sph_weights, float))
+            else:
+                per_shell = 4 * np.pi * r**2 * w
+                sph.append(atgrid.weights / np.repeat(per_shell, np.diff(aidx)))
+            atw.append(np.asarray(atgrid.weights, float))
+        points = np.ascontiguousarray(grid.points, dtype=float)
+        return {
+            "natom": self.natom,
+            "coordinates": np.ascontiguousarray(self.coordinates, dtype=float),
+            "numbers": np.ascontiguousarray(self.numbers, dtype=np.int64),
+            "pseudo_numbers": np.ascontiguousarray(self.pseudo_numbers, dtype=float),
+            "npoint": int(points.shape[0]),
+            "points": points,
+            "density": np.ascontiguousarray(self._moldens, dtype=float),
+            "mol_weights": np.ascontiguousarray(grid.weights, dtype=float),
+            "atom_weights": np.ascontiguousarray(np.concatenate(atw)),
+            "sph_weights": np.ascontiguousarray(np.concatenate(sph)),
+            "nshell": len(shell_atom),
+            "shell_atom": np.asarray(shell_atom, dtype=np.int32),
+            "shell_offsets": np.asarray(shell_off, dtype=np.int64),
+            "shell_radius": np.ascontiguousarray(np.concatenate(shell_r)),
+            "shell_rweight": np.ascontiguousarray(np.concatenate(shell_w)),
+        }
+
+    def _scheme_arrays(self):
+        """Scheme-specific entries of ``hp_scheme_desc`` (subclasses)."""
+        raise NotImplementedError
+
+    def _base_scheme_arrays(self):
+        return {
+            "grid_type": self.grid_type,
+            "threshold": float(self._threshold),
+            "inner_threshold": float(self._inner_threshold),
+            "maxiter": int(self._maxiter),
+            "density_cutoff": float(self.density_cutoff),
+            "negative_cutoff": float(self.negative_cutoff),
+            "population_cutoff": float(self.population_cutoff),
+            "cutoff_radius": float(self._cutoff_radius) if self._cutoff_radius else 0.0,
+            "newton_tau": 1.0,
+            "newton_linspace": 20,
+            "check_mono": 0,
+        }
+
+    def _init_propars(self):
+        raise NotImplementedError
+
+    def _ensure_device(self):
+        if self._dev is None:
+            if self.grid_type != 1:
+                raise NotImplementedError("the device path implements grid_type 1")
+            self._dev = _native.DevicePartition(self._molecule_arrays(), self._scheme_arrays(), device=self._device)
+        return self._dev
+
+    def _raise_status(self, status_all, status_last, rc):
+        flags = int(np.bitwise_or.reduce(status_all)) if len(status_all) else 0
+        if flags & _native.ST_INNER_NOT_CONVERGED:
+            self.logger.warning("Inner iteration not converged (atoms %s)." % np.flatnonzero(status_all & 1).tolist())
+        if flags & _native.ST_POP_MISMATCH:
+            self.logger.warning("The sum of pro-atom parameters is not equal to the atomic population.")
+        if flags & _native.ST_NEG_PARS:
+            warnings.warn("WARNING: Not all pro-atom parameters are positive!")
+        if rc == _native.HP_E_SOLVER:
+            last = int(np.bitwise_or.reduce(status_last))
+            if last & _native.ST_SINGULAR:
+                raise np.linalg.LinAlgError("Matrix is singular.")
+            if last & _native.ST_NONFINITE:
+                raise ValueError("array must not contain infs or NaNs")
+            if last & _native.ST_NEG_DENSITY:
+                raise RuntimeError("Negative pro-atom density found!")
+            if last & _native.ST_LINESEARCH:
+                raise RuntimeError("Line search failed!")
+            if last & _native.ST_NEWTON_MAXITER:
+                raise RuntimeError("Inner loop: Newton does not converge!")
+            if last & _native.ST_NOT_MONOTONIC:
+                raise RuntimeError("Pro-atom density should be monotonically decreasing.")
+            if last & _native.ST_QP_FAILED:
+                raise RuntimeError("QP solver failed.")
+            raise RuntimeError("a per-atom solver failed on the device")
+        if rc != 0:
+            _native.check(rc)
+
+    @just_once
+    def do_partitioning(self):
+        """Outer fixed-point loop (core/iterstock.py:159-193) on the GPU."""
+        dev = self._ensure_device()
+        propars = self._init_propars()
+        dev.set_propars(propars)
+        self.logger.info("Iteration       Change      Entropy")
+        rc = dev.run(self._precision, self._n_polish)
+        inner, last, allst = dev.status()
+        self._inner_niter = inner
+        if dev.niter > 0:
+            hp, hc, he, hg = dev.history()
+            self.history_propars = list(hp)
+            self.history_charges = list(hc)
+            self.history_entropies = list(he)
+            self.history_changes = list(hg)
+            for i in range(dev.niter):
+                self.logger.info("%9i   %10.5e   %10.5e" % (i + 1, hg[i], he[i]))
+        self._raise_status(allst, last, rc)
+        new_propars, charges, change, _ = dev.results()
+        propars[:] = new_propars
+        self.cache.dump("propars", propars, tags="o")
+        self.cache.dump("charges", charges, tags="o")
+        self.cache.dump("niter", dev.niter, tags="o")
+        self.cache.dump("change", change, tags="o")
+        self._register_lazy(dev)
+        self._finalize_propars()
+
+    def _register_lazy(self, dev):
+        grid = self.grid
+        for a in range(self.natom):
+            size = int(grid.indices[a + 1] - grid.indices[a])
+            self.cache.add_lazy(f"at_weights_{a}", (lambda a=a, size=size: dev.at_weights(a, size)))
+        self.cache.add_lazy("promoldens", dev.promoldens)
+        rhobar = dev.spherical_averages()
+        s = 0
+        for a, atgrid in enumerate(grid.atgrids):
+            nr = atgrid.rgrid.size
+            self.cache.dump(f"radial_points_{a}", atgrid.rgrid.points, tags="o")
+            self.cache.dump(f"spherical_average_{a}", rhobar[s : s + nr].copy(), tags="o")
+            self.cache.dump(f"radial_weights_{a}", atgrid.rgrid.weights, tags="o")
+            s += nr
+
+    def _finalize_propars(self):
+        """finalize_propars (core/iterstock.py:48-58)."""
+        charges = self.cache.load("charges")
+        self.cache.dump("history_propars", np.array(self.history_propars), tags="o")
+        self.cache.dump("history_charges", np.array(self.history_charges), tags="o")
+        self.cache.dump("history_entropies", np.array(self.history_entropies), tags="o")
+        self.cache.dump("history_changes", np.array(self.history_changes), tags="o")
+        self.cache.dump("populations", self.numbers - charges, tags="o")
+        self.cache.dump("pseudo_populations", self.pseudo_numbers - charges, tags="o")
+        dev = self._dev
+        t_w, _ = dev.kernel_time(_native.KERNEL_WEIGHTS)
+        t_i, _ = dev.kernel_time(_native.KERNEL_INNER)
+        self.cache.dump("time_update_at_weights", t_w * 1e-3, tags="o")
+        self.cache.dump("time_update_propars", t_i * 1e-3, tags="o")
+
+    def _device_moments(self):
+        if self.lmax != 3:
+            raise NotImplementedError("device moments are implemented for lmax = 3")
+        return self._dev.moments(self.lmax)
+
+    @property
+    def inner_niter(self):
+        """Per-atom inner iteration counts of the last outer iteration."""
+        return getattr(self, "_inner_niter", None)
+
+    def close(self):
+        """Release the device memory held by this partitioning."""
+        if self._dev is not None:
+            self._dev.close()

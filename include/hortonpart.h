@@ -1,0 +1,199 @@
+/*
+ * hortonpart.h -- C ABI of the MI355X stockholder-partitioning library.
+ *
+ * The reference (horton-part 1.1.7) is a pure-Python package with no FFI; its
+ * boundary for this path is the WPart scheme-class API.  The entry points
+ * below are what that API's host classes (horton_part_amd.mbis / .gisa /
+ * .alisa) bind through ctypes.  Each one names the reference routine whose
+ * work it takes over (paths relative to the reference's src/horton_part).
+ *
+ * Conventions
+ *  - plain C types only: pointers to host memory, sizes, doubles, ints;
+ *  - every array is C-contiguous; the library copies inputs to the GPU;
+ *  - the caller owns every output buffer;
+ *  - every function returns 0 on success and a negative HP_E_* code on
+ *    failure; hp_last_error() then describes the failure (thread-local);
+ *  - one hp_part object per host thread at a time (not re-entrant).
+ */
+#ifndef HORTONPART_H
+#define HORTONPART_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define HP_ABI_VERSION 1
+
+/* error codes (negative returns) */
+#define HP_E_ARG -1      /* invalid argument or shape */
+#define HP_E_HIP -2      /* HIP runtime error (no device, launch failure...) */
+#define HP_E_SOLVER -3   /* a per-atom solver raised (see hp_part_status) */
+#define HP_E_UNSUPP -4   /* combination not implemented on the device */
+
+/* partitioning schemes */
+#define HP_SCHEME_MBIS 0 /* mbis.py:206-362 */
+#define HP_SCHEME_GISA 1 /* gisa.py:109-345 */
+#define HP_SCHEME_LISA 2 /* alisa.py:1155-1335 */
+
+/* inner solvers */
+#define HP_SOLVER_MBIS 0     /* mbis.py:81-163 opt_mbis_propars */
+#define HP_SOLVER_QP 1       /* gisa.py:348-421 opt_propars_qp_interface */
+#define HP_SOLVER_SC 2       /* alisa.py:193-291 solver_sc */
+#define HP_SOLVER_NEWTON 3   /* alisa.py:543-610 solver_newton (exact) */
+#define HP_SOLVER_MNEWTON 4  /* alisa.py:613-680 solver_m_newton (line search) */
+
+/* arithmetic of the grid passes */
+#define HP_PREC_FP64 0  /* everything in FP64 */
+#define HP_PREC_MIXED 1 /* FP32 exp/promolecule/ratio, FP64 sums and solves */
+
+/* per-atom status bits (hp_part_status) */
+#define HP_ST_INNER_NOT_CONVERGED 1 /* mbis.py:162 / alisa.py:289 warning */
+#define HP_ST_POP_MISMATCH 2        /* mbis.py:157 / utils.py:442-456 warning */
+#define HP_ST_NEG_DENSITY 4         /* utils.py:512-520 RuntimeError */
+#define HP_ST_LINESEARCH 8          /* alisa.py:877 RuntimeError */
+#define HP_ST_NEWTON_MAXITER 16     /* alisa.py:938 RuntimeError */
+#define HP_ST_QP_FAILED 32          /* quadprog failure (gisa.py:409) */
+#define HP_ST_NEG_PARS 64           /* utils.py:482-489 warning */
+#define HP_ST_NOT_MONOTONIC 128     /* utils.py:546-553 RuntimeError */
+#define HP_ST_SINGULAR 256          /* alisa.py:924 scipy LinAlgError (singular Hessian) */
+#define HP_ST_NONFINITE 512         /* alisa.py:924 scipy ValueError (inf/NaN Hessian) */
+
+/* A molecule on a grid of atom-centred product grids.  The molecular grid is
+ * the concatenation of the atom grids (atom-major, radial-shell-major), as
+ * qc-grid's MolGrid builds it (core/base.py:490-499, 619-628). */
+typedef struct {
+  int natom;
+  const double *coordinates;    /* natom*3, bohr */
+  const int64_t *numbers;       /* natom */
+  const double *pseudo_numbers; /* natom */
+  int64_t npoint;
+  const double *points;      /* npoint*3 */
+  const double *density;     /* npoint: molecular density rho */
+  const double *mol_weights; /* npoint: atom-grid weight x Becke weight */
+  const double *atom_weights;/* npoint: atom-grid weight (moments) */
+  const double *sph_weights; /* npoint: angular weight within its shell (sum 1) */
+  int nshell;                /* total number of radial shells */
+  const int32_t *shell_atom; /* nshell: owning atom */
+  const int64_t *shell_offsets; /* nshell+1: point offsets */
+  const double *shell_radius;   /* nshell: r_i */
+  const double *shell_rweight;  /* nshell: radial quadrature weight w_i */
+} hp_molecule_desc;
+
+/* The scheme: per-atom pro-atom functions and solver options. */
+typedef struct {
+  int scheme;    /* HP_SCHEME_* */
+  int solver;    /* HP_SOLVER_* */
+  int grid_type; /* 1: atom grids (default); 2: molecular grid */
+  const int32_t *fn_offsets; /* natom+1: functions of atom a are [off[a], off[a+1]) */
+  const double *fn_alpha;    /* GISA/LISA: exponent of each function */
+  const double *fn_order;    /* GISA/LISA: power n of r in exp(-alpha r^n) */
+  const double *fn_norm;     /* GISA/LISA: n alpha^(3/n) / (4 pi Gamma(3/n)) */
+  const double *qp_matrix;   /* GISA: concatenated K_a x K_a blocks of P */
+  double threshold;          /* outer threshold (iterstock.py:99) */
+  double inner_threshold;    /* already min(inner, threshold) (iterstock.py:101) */
+  int maxiter;               /* outer cap (iterstock.py:187) */
+  int inner_maxiter;         /* 2000 (MBIS), 1e5 (sc), 10000 (newton) */
+  double density_cutoff;     /* sick-point cutoff (constants.yaml:27) */
+  double negative_cutoff;    /* constants.yaml:28 */
+  double population_cutoff;  /* constants.yaml:29 */
+  double cutoff_radius;      /* <= 0: every pro-atom on every point */
+  double newton_tau;         /* m-newton line-search start (alisa.py:772) */
+  int newton_linspace;       /* m-newton line-search size (alisa.py:773) */
+  int check_mono;            /* alisa.py:774 */
+} hp_scheme_desc;
+
+typedef struct hp_part hp_part;
+
+/* Library ABI version (HP_ABI_VERSION). */
+int hp_abi_version(void);
+
+/* Text of the last error on this thread ("" if none). */
+const char *hp_last_error(void);
+
+/* Number of visible GPUs. */
+int hp_device_count(int *count);
+
+/* Upload a molecule and its scheme and allocate the device state.
+ * Replaces the construction work of WPart.__init__ (core/base.py:416-488),
+ * calc_radial_distances (core/base.py:630-635; distances are recomputed on
+ * the fly instead of stored) and the basis set-up of evaluate_basis_functions
+ * (gisa.py:91-106). */
+int hp_part_create(int device, const hp_molecule_desc *mol, const hp_scheme_desc *scheme, hp_part **out);
+
+/* Release all device memory of a partitioning. */
+int hp_part_destroy(hp_part *part);
+
+/* Set the current pro-atom parameters (initial values: mbis.py:291-305,
+ * gisa.py:68-88).  Length: 2*sum(K_a) for MBIS, sum(K_a) otherwise. */
+int hp_part_set_propars(hp_part *part, const double *propars, int64_t n);
+
+/* Run the outer fixed-point loop of AbstractISAWPart.do_partitioning
+ * (core/iterstock.py:159-193): per iteration update_at_weights
+ * (core/stockholder.py:352-384), the per-atom updates (mbis.py:167-203,
+ * gisa.py:251-318, alisa.py:1304-1335), the entropy (stockholder.py:145-151)
+ * and compute_change (iterstock.py:32-45).  precision: HP_PREC_*.  With
+ * HP_PREC_MIXED the loop runs in mixed precision to convergence and then
+ * n_polish FP64 iterations follow (at least one).  Returns the number of
+ * outer iterations through *niter.  HP_E_SOLVER if a per-atom solver raised;
+ * the loop stops at that iteration. */
+int hp_part_run(hp_part *part, int precision, int n_polish, int *niter);
+
+/* Copy results to host buffers (any pointer may be NULL):
+ * propars (P), charges (natom), last change and entropy. */
+int hp_part_results(hp_part *part, double *propars, double *charges, double *change, double *entropy);
+
+/* Histories (iterstock.py:51-54): niter rows each.  history_propars niter*P,
+ * history_charges niter*natom, history_entropies niter, history_changes niter. */
+int hp_part_history(hp_part *part, double *hpropars, double *hcharges, double *hentropies, double *hchanges);
+
+/* Per-atom inner-iteration counts and HP_ST_* status bits of the last outer
+ * iteration, plus the OR of the status bits over all iterations. */
+int hp_part_status(hp_part *part, int32_t *inner_niter, int32_t *status_last, int32_t *status_all);
+
+/* Spherical averages of the AIM densities of the last update (gt1 only):
+ * the "spherical_average_{a}" cache entries of mbis.py:190-198 (nshell). */
+int hp_part_spherical_averages(hp_part *part, double *rhobar);
+
+/* AIM weights of atom `atom` for the pro-atoms that produced the last update
+ * ("at_weights_{a}" of stockholder.py:364-377): its own atom-grid points for
+ * grid_type 1, all points for grid_type 2. */
+int hp_part_at_weights(hp_part *part, int atom, double *out, int64_t n);
+
+/* Promolecule density of the last update ("promoldens", stockholder.py:355). */
+int hp_part_promoldens(hp_part *part, double *out, int64_t n);
+
+/* AIM multipole moments (core/base.py:329-402) for lmax <= 3 in HORTON order:
+ * cartesian natom*20, pure natom*16, radial natom*4 (pseudo_numbers added to
+ * the monopoles, electron sign applied as the reference does). */
+int hp_part_moments(hp_part *part, int lmax, double *cartesian, double *pure, double *radial);
+
+/* Timing of the pro-atom/weights kernel over the last hp_part_run, measured
+ * with HIP events on the library's stream: total milliseconds and launches. */
+int hp_part_kernel_time(hp_part *part, int kernel, double *total_ms, int64_t *launches);
+
+#define HP_KERNEL_WEIGHTS 0 /* fused pro-atom/promolecule/weights/sphavg pass */
+#define HP_KERNEL_INNER 1   /* per-atom inner solvers */
+#define HP_KERNEL_LISA_GRID 2 /* molecular-grid Newton pass (grid_type 2) */
+
+/* ---- grid utilities (input preparation, not the iteration) ---- */
+
+/* Becke fuzzy-cell weights (qc-grid BeckeWeights, used by MolGrid in
+ * tests/test_wpart.py:50-51): for every point, the weight of the atom whose
+ * atom grid holds it (atom_offsets: natom+1 point offsets).  radii in bohr. */
+int hp_becke_weights(int device, int natom, const double *coordinates, const double *radii, int64_t npoint,
+                     const double *points, const int64_t *atom_offsets, double *out);
+
+/* sum_a sum_k coef_ak exp(-alpha_ak |p - R_a|) at every point: the
+ * Slater-shell densities of the synthetic workloads (no reference
+ * counterpart; the reference reads densities from wavefunction files,
+ * scripts/generate_density.py:119-204). */
+int hp_slater_density(int device, int natom, const double *coordinates, const int32_t *fn_offsets,
+                      const double *coef, const double *alpha, int64_t npoint, const double *points, double *out);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* HORTONPART_H */

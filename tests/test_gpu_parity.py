@@ -1,0 +1,175 @@
+"""GPU parity: the HIP path (through libhortonpart.so) against the CPU oracle.
+
+Bar for the plain FP64 path: charges and populations within 1e-10 e, pro-atom
+parameters within 1e-9 relative, identical outer iteration counts and
+identical per-atom inner iteration counts of the last outer iteration.
+The mixed path is held to max |dq| <= 1e-6 e against the FP64 oracle.
+"""
+
+import sys
+import warnings
+
+import numpy as np
+import pytest
+
+from conftest import require_gpu
+from horton_part_amd import ExpBasisFuncHelper, GaussianISAWPart, LinearISAWPart, MBISWPart, workloads
+from oracle.horton_oracle import OracleBasis, OracleGISA, OracleLISA, OracleMBIS
+
+from common import WATER_MBIS_CHARGES, load_water_fixture
+
+pytestmark = pytest.mark.gpu
+
+warnings.simplefilter("ignore")
+
+
+def _say(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+def _compare(part, cache, oracle, qtol=1e-10, ptol=1e-9, check_inner=True):
+    dq = np.max(np.abs(part["charges"] - cache["charges"]))
+    dpop = np.max(np.abs(part["populations"] - cache["populations"]))
+    p1, p2 = part["propars"], cache["propars"]
+    rel = np.max(np.abs(p1 - p2) / np.maximum(np.abs(p2), 1e-300))
+    _say(f"  niter gpu={part['niter']} oracle={cache['niter']} max|dq|={dq:.2e} rel dpar={rel:.2e}")
+    assert part["niter"] == cache["niter"]
+    assert dq <= qtol
+    assert dpop <= qtol
+    assert rel <= ptol
+    if check_inner:
+        inner_oracle = [s.get("niter", 1) for s in oracle.inner_stats[-1]]
+        assert list(part.inner_niter) == inner_oracle
+    np.testing.assert_allclose(part["history_changes"], cache["history_changes"], rtol=1e-6, atol=1e-14)
+    np.testing.assert_allclose(part["history_entropies"], cache["history_entropies"], rtol=1e-9, atol=1e-12)
+
+
+def test_mbis_water_fixture_kat():
+    """Reference KAT (tests/test_wpart.py:95-101) through the device path."""
+    require_gpu()
+    _say("case: MBIS reference water fixture, 120x110, threshold 1e-6")
+    coords, nums, pseudo, grid, dens = load_water_fixture()
+    part = MBISWPart(coords, nums, pseudo, grid, dens)
+    part.do_all()
+    oracle = OracleMBIS(coords, nums, pseudo, grid, dens)
+    cache = oracle.do_partitioning()
+    _compare(part, cache, oracle)
+    assert np.abs(part["charges"] - WATER_MBIS_CHARGES).max() < 2e-3
+    assert part["charges"] == pytest.approx(part["valence_charges"] + part["core_charges"])
+    assert (part["core_charges"] > 0).all()
+    assert (part["valence_charges"] < 0).all()
+    assert (part["valence_widths"] > 0).all()
+    # moments (core/base.py:329-402) against the oracle, and the monopole pin
+    cart, pure, rad = oracle.do_moments()
+    np.testing.assert_allclose(part["cartesian_multipoles"], cart, rtol=1e-9, atol=1e-11)
+    np.testing.assert_allclose(part["pure_multipoles"], pure, rtol=1e-9, atol=1e-11)
+    np.testing.assert_allclose(part["radial_moments"], rad, rtol=1e-9, atol=1e-11)
+    assert np.abs(part["charges"] - part["cartesian_multipoles"][:, 0]).max() < 1e-3
+    assert np.abs(part["charges"] - part["pure_multipoles"][:, 0]).max() < 1e-3
+    # per-point weights, promolecule and spherical averages
+    for a in range(3):
+        np.testing.assert_allclose(part[f"at_weights_{a}"], oracle.cache[f"at_weights_{a}"], rtol=1e-12, atol=1e-15)
+        np.testing.assert_allclose(part[f"spherical_average_{a}"], oracle.cache[f"spherical_average_{a}"], rtol=1e-11, atol=1e-14)
+    np.testing.assert_allclose(part["promoldens"], oracle.cache["promoldens"], rtol=1e-12, atol=1e-300)
+
+
+def test_mbis_synthetic_water_degenerate_shells():
+    """Water-like workload: nearly equal O shell exponents drive the inner
+    solver to its 2000-iteration cap (mbis.py:162 warning path)."""
+    require_gpu()
+    _say("case: MBIS synthetic water seed 1000, 60x110, threshold 1e-8, maxiter 40")
+    w = workloads.water(nrad=60, nang=110)
+    kw = dict(threshold=1e-8, maxiter=40)
+    part = MBISWPart(w.coordinates, w.numbers, w.pseudo_numbers, w.grid, w.density, **kw)
+    part.do_partitioning()
+    oracle = OracleMBIS(w.coordinates, w.numbers, w.pseudo_numbers, w.grid, w.density, **kw)
+    cache = oracle.do_partitioning()
+    _compare(part, cache, oracle, qtol=1e-10, ptol=1e-8)
+
+
+def test_mbis_chain():
+    require_gpu()
+    _say("case: MBIS chain 6 atoms seed 7, 50x110")
+    w = workloads.chain(natom=6, seed=7, nrad=50, nang=110)
+    part = MBISWPart(w.coordinates, w.numbers, w.pseudo_numbers, w.grid, w.density, maxiter=80)
+    part.do_partitioning()
+    oracle = OracleMBIS(w.coordinates, w.numbers, w.pseudo_numbers, w.grid, w.density, maxiter=80)
+    cache = oracle.do_partitioning()
+    _compare(part, cache, oracle)
+
+
+def test_gisa_chain():
+    require_gpu()
+    _say("case: GISA chain 5 atoms seed 11, 50x110")
+    w = workloads.chain(natom=5, seed=11, nrad=50, nang=110)
+    part = GaussianISAWPart(w.coordinates, w.numbers, w.pseudo_numbers, w.grid, w.density, maxiter=60)
+    part.do_partitioning()
+    bs = part.bs_helper
+    basis = OracleBasis(bs.orders, bs.exponents, bs.initials)
+    oracle = OracleGISA(w.coordinates, w.numbers, w.pseudo_numbers, w.grid, w.density, maxiter=60, basis=basis)
+    cache = oracle.do_partitioning()
+    _compare(part, cache, oracle, check_inner=False)
+
+
+def _slater6(numbers):
+    alphas = 0.5 * 100.0 ** (np.arange(6) / 5.0)
+    return ExpBasisFuncHelper.uniform(numbers, alphas, order=1.0)
+
+
+def _lisa_pair(w, basis_name, solver, **kw):
+    helper = _slater6(w.numbers) if basis_name == "slater6" else ExpBasisFuncHelper.from_function_type("gauss")
+    part = LinearISAWPart(w.coordinates, w.numbers, w.pseudo_numbers, w.grid, w.density, solver=solver, basis_func=helper, **kw)
+    basis = OracleBasis(helper.orders, helper.exponents, helper.initials)
+    oracle = OracleLISA(w.coordinates, w.numbers, w.pseudo_numbers, w.grid, w.density, basis=basis, solver=solver, **kw)
+    return part, oracle
+
+
+@pytest.mark.parametrize("basis_name,solver", [("slater6", "sc"), ("gauss", "sc"), ("gauss", "m-newton"), ("slater6", "m-newton")])
+def test_lisa_chain(basis_name, solver):
+    require_gpu()
+    _say(f"case: LISA {solver} basis {basis_name} chain 4 atoms seed 5, 50x110")
+    w = workloads.chain(natom=4, seed=5, nrad=50, nang=110)
+    part, oracle = _lisa_pair(w, basis_name, solver, maxiter=100)
+    part.do_partitioning()
+    cache = oracle.do_partitioning()
+    _compare(part, cache, oracle, qtol=1e-10, ptol=1e-8)
+
+
+def test_lisa_exact_newton_raises_like_reference():
+    """Exact Newton diverges on these densities until the Hessian is exactly
+    singular; scipy.linalg.solve then raises LinAlgError (alisa.py:924) and
+    the device path must raise the same exception."""
+    require_gpu()
+    _say("case: LISA exact newton, gauss basis, chain 4 atoms seed 5 (expects LinAlgError)")
+    w = workloads.chain(natom=4, seed=5, nrad=50, nang=110)
+    part, oracle = _lisa_pair(w, "gauss", "newton", maxiter=100)
+    with pytest.raises(np.linalg.LinAlgError):
+        oracle.do_partitioning()
+    with pytest.raises(np.linalg.LinAlgError):
+        part.do_partitioning()
+
+
+def test_mbis_mixed_precision():
+    require_gpu()
+    _say("case: MBIS mixed precision + FP64 polish, chain 6 atoms seed 7")
+    w = workloads.chain(natom=6, seed=7, nrad=50, nang=110)
+    part = MBISWPart(w.coordinates, w.numbers, w.pseudo_numbers, w.grid, w.density, maxiter=80, precision="mixed")
+    part.do_partitioning()
+    oracle = OracleMBIS(w.coordinates, w.numbers, w.pseudo_numbers, w.grid, w.density, maxiter=80)
+    cache = oracle.do_partitioning()
+    dq = np.max(np.abs(part["charges"] - cache["charges"]))
+    _say(f"  mixed niter={part['niter']} oracle niter={cache['niter']} max|dq|={dq:.2e}")
+    assert dq <= 1e-6
+
+
+def test_mbis_cutoff_radius():
+    require_gpu()
+    _say("case: MBIS with 6-bohr pro-atom cutoff, cluster 12 atoms seed 3, 40x50")
+    w = workloads.cluster(natom=12, seed=3, nrad=40, nang=50)
+    kw = dict(maxiter=30, cutoff_radius=6.0)
+    part = MBISWPart(w.coordinates, w.numbers, w.pseudo_numbers, w.grid, w.density, **kw)
+    part.do_partitioning()
+    oracle = OracleMBIS(w.coordinates, w.numbers, w.pseudo_numbers, w.grid, w.density, **kw)
+    cache = oracle.do_partitioning()
+    _compare(part, cache, oracle)
