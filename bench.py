@@ -51,7 +51,7 @@ def build_workload(name, device):
     raise SystemExit(f"unknown workload {name}")
 
 
-def make_part(w, scheme, threshold, precision, maxiter=500):
+def make_part(w, scheme, threshold, precision, maxiter=1000):
     from horton_part_amd import MBISWPart
 
     part = MBISWPart(w.coordinates, w.numbers, w.pseudo_numbers, w.grid, w.density, threshold=threshold, maxiter=maxiter, precision=precision)

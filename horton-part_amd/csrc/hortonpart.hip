@@ -60,7 +60,7 @@ int dupload(T **p, const T *host, size_t n, hipStream_t s) {
 struct hp_part {
   int device = 0;
   hipStream_t stream = nullptr;
-  hipEvent_t ev[6] = {};
+  hipEvent_t ev[2] = {};
   int natom = 0, nshell = 0, nfn = 0, npar = 0;
   int64_t npoint = 0;
   int scheme = 0, solver = 0, grid_type = 1, fnkind = HP_FN_SLATER;
@@ -88,7 +88,11 @@ struct hp_part {
   int *inner_niter = nullptr, *status = nullptr, *status_all = nullptr;
   double *hpar = nullptr, *hch = nullptr, *hent = nullptr, *hchg = nullptr;
   int hist_cap = 0;
-  double *h_out = nullptr;  // pinned
+  double *h_out = nullptr;  // pinned: change, entropy, error bits, any bits, done, iterations
+  int *state = nullptr;     // device: done flag, iteration counter
+  static constexpr int kSlots = 32;
+  hipEvent_t slot_ev[kSlots][3] = {};
+  double mixed_switch = 1e-5;
   // host state
   int niter = 0;
   double last_change = 0.0, last_entropy = 0.0;
@@ -146,8 +150,12 @@ void free_part(hp_part *p) {
   for (void *b : bufs)
     if (b) (void)hipFree(b);
   if (p->h_out) (void)hipHostFree(p->h_out);
+  if (p->state) (void)hipFree(p->state);
   for (auto &e : p->ev)
     if (e) (void)hipEventDestroy(e);
+  for (auto &slot : p->slot_ev)
+    for (auto &e : slot)
+      if (e) (void)hipEventDestroy(e);
   if (p->stream) (void)hipStreamDestroy(p->stream);
   delete p;
 }
@@ -166,10 +174,10 @@ void launch_weights_fk(hp_part *p, bool gt2) {
   int bs = ((p->maxL + 63) / 64) * 64;
   bs = std::max(64, std::min(512, bs));
   if (gt2)
-    hipLaunchKernelGGL((k_weights<R, FK, true>), dim3(p->nshell), dim3(bs), 0, p->stream, p->mol(), p->fn(), p->rhobar,
-                       p->ent_part, p->promol, p->dc);
+    hipLaunchKernelGGL((k_weights<R, FK, true>), dim3(p->nshell), dim3(bs), 0, p->stream, p->state, p->mol(), p->fn(),
+                       p->rhobar, p->ent_part, p->promol, p->dc);
   else
-    hipLaunchKernelGGL((k_weights<R, FK, false>), dim3(p->nshell), dim3(bs), 0, p->stream, p->mol(), p->fn(),
+    hipLaunchKernelGGL((k_weights<R, FK, false>), dim3(p->nshell), dim3(bs), 0, p->stream, p->state, p->mol(), p->fn(),
                        p->rhobar, p->ent_part, p->promol, p->dc);
 }
 
@@ -188,20 +196,20 @@ int launch_inner_ppl(hp_part *p) {
   const dim3 grid((p->natom + 3) / 4), block(256);
   if (p->scheme == HP_SCHEME_MBIS) {
     if (p->maxK <= 3)
-      hipLaunchKernelGGL((k_mbis_inner<PPL, 3>), grid, block, 0, p->stream, p->mol(), p->rhobar, p->par, p->old_par,
+      hipLaunchKernelGGL((k_mbis_inner<PPL, 3>), grid, block, 0, p->stream, p->state, p->mol(), p->rhobar, p->par, p->old_par,
                          p->fn_off, p->inner_threshold, p->inner_maxiter, p->inner_niter, p->status, p->charges,
                          p->change_part);
     else
-      hipLaunchKernelGGL((k_mbis_inner<PPL, 7>), grid, block, 0, p->stream, p->mol(), p->rhobar, p->par, p->old_par,
+      hipLaunchKernelGGL((k_mbis_inner<PPL, 7>), grid, block, 0, p->stream, p->state, p->mol(), p->rhobar, p->par, p->old_par,
                          p->fn_off, p->inner_threshold, p->inner_maxiter, p->inner_niter, p->status, p->charges,
                          p->change_part);
   } else if (p->solver == HP_SOLVER_QP) {
-    hipLaunchKernelGGL((k_gisa_qp<PPL>), grid, block, 0, p->stream, p->mol(), p->rhobar, p->par, p->old_par, p->fn_off,
+    hipLaunchKernelGGL((k_gisa_qp<PPL>), grid, block, 0, p->stream, p->state, p->mol(), p->rhobar, p->par, p->old_par, p->fn_off,
                        p->fn_norm, p->fn_alpha, p->fn_order, p->qpP, p->qp_off, p->pop_cutoff, p->neg_cutoff,
                        p->inner_niter, p->status, p->charges, p->change_part);
   } else {
 #define HP_LISA(KMV)                                                                                             \
-  hipLaunchKernelGGL((k_lisa_inner<PPL, KMV>), grid, block, 0, p->stream, p->mol(), p->rhobar, p->par, p->old_par, \
+  hipLaunchKernelGGL((k_lisa_inner<PPL, KMV>), grid, block, 0, p->stream, p->state, p->mol(), p->rhobar, p->par, p->old_par, \
                      p->fn_off, p->fn_norm, p->fn_alpha, p->fn_order, p->solver, p->inner_threshold,             \
                      p->inner_maxiter, p->dc, p->neg_cutoff, p->pop_cutoff, p->tau, p->nlin, p->check_mono,       \
                      p->inner_niter, p->status, p->charges, p->change_part)
@@ -228,39 +236,69 @@ int launch_inner(hp_part *p) {
   return fail(HP_E_UNSUPP, "more than 512 radial points per atom");
 }
 
-// One outer iteration: update_at_weights + per-atom updates + entropy + change.
-int outer_step(hp_part *p, int precision, int iter) {
+// Enqueue one outer iteration (update_at_weights, per-atom updates, entropy,
+// change, stop test).  Every kernel returns at once when the device flag says
+// the loop has stopped, so iterations can be enqueued ahead of the decision.
+int enqueue_iteration(hp_part *p, int precision, double thr, int min_stop, int max_iter, int slot) {
   hipStream_t s = p->stream;
-  HP_HIP(hipMemcpyAsync(p->old_par, p->par, sizeof(double) * p->npar, hipMemcpyDeviceToDevice, s));
-  hipLaunchKernelGGL(k_build_fn, dim3((p->nfn + 255) / 256), dim3(256), 0, s, p->scheme == HP_SCHEME_MBIS ? 0 : 1,
-                     p->nfn, p->par, p->fn_norm, p->fn_alpha, p->coef, p->alpha);
+  const int n = std::max(p->npar, p->nfn);
+  hipLaunchKernelGGL(k_iter_begin, dim3((n + 255) / 256), dim3(256), 0, s, p->state, p->scheme == HP_SCHEME_MBIS ? 0 : 1,
+                     p->npar, p->nfn, p->par, p->old_par, p->fn_norm, p->fn_alpha, p->coef, p->alpha);
   HP_LAUNCHED();
-  HP_HIP(hipEventRecord(p->ev[0], s));
+  HP_HIP(hipEventRecord(p->slot_ev[slot][0], s));
   if (precision == HP_PREC_MIXED)
     launch_weights<float>(p, false);
   else
     launch_weights<double>(p, false);
   HP_LAUNCHED();
-  HP_HIP(hipEventRecord(p->ev[1], s));
+  HP_HIP(hipEventRecord(p->slot_ev[slot][1], s));
   int rc = launch_inner(p);
   if (rc) return rc;
   HP_LAUNCHED();
-  HP_HIP(hipEventRecord(p->ev[2], s));
-  hipLaunchKernelGGL(k_finalize, dim3(1), dim3(256), 0, s, p->nshell, p->ent_part, p->natom, p->change_part, p->status,
-                     p->status_all, p->par, p->npar, p->charges, iter, p->hpar, p->hch, p->hent, p->hchg, p->out);
+  HP_HIP(hipEventRecord(p->slot_ev[slot][2], s));
+  hipLaunchKernelGGL(k_finalize, dim3(1), dim3(256), 0, s, p->state, thr, min_stop, max_iter, p->nshell, p->ent_part,
+                     p->natom, p->change_part, p->status, p->status_all, p->par, p->npar, p->charges, p->hpar, p->hch,
+                     p->hent, p->hchg, p->out);
   HP_LAUNCHED();
-  HP_HIP(hipMemcpyAsync(p->h_out, p->out, 4 * sizeof(double), hipMemcpyDeviceToHost, s));
-  HP_HIP(hipStreamSynchronize(s));
-  float ms = 0.f;
-  HP_HIP(hipEventElapsedTime(&ms, p->ev[0], p->ev[1]));
-  p->k_ms[HP_KERNEL_WEIGHTS] += ms;
-  p->k_n[HP_KERNEL_WEIGHTS] += 1;
-  HP_HIP(hipEventElapsedTime(&ms, p->ev[1], p->ev[2]));
-  p->k_ms[HP_KERNEL_INNER] += ms;
-  p->k_n[HP_KERNEL_INNER] += 1;
-  p->last_change = p->h_out[0];
-  p->last_entropy = p->h_out[1];
-  if ((int)p->h_out[2] != 0) return fail(HP_E_SOLVER, "a per-atom solver raised; see hp_part_status");
+  return 0;
+}
+
+// Run iterations until the device stop test fires: batches of speculative
+// iterations, one host synchronisation per batch.
+int run_stage(hp_part *p, int precision, double thr, int min_stop, int max_iter) {
+  hipStream_t s = p->stream;
+  HP_HIP(hipMemsetAsync(p->state, 0, sizeof(int), s));
+  int batch = 2;
+  while (true) {
+    const int before = p->niter;
+    const int remaining = max_iter - before;
+    const int nb = std::max(1, std::min(batch, remaining));
+    for (int c = 0; c < nb; ++c) {
+      int rc = enqueue_iteration(p, precision, thr, min_stop, max_iter, c);
+      if (rc) return rc;
+    }
+    HP_HIP(hipMemcpyAsync(p->h_out, p->out, 4 * sizeof(double), hipMemcpyDeviceToHost, s));
+    HP_HIP(hipMemcpyAsync(p->h_out + 4, p->state, 2 * sizeof(int), hipMemcpyDeviceToHost, s));
+    HP_HIP(hipStreamSynchronize(s));
+    int st[2];
+    std::memcpy(st, p->h_out + 4, sizeof(st));
+    p->niter = st[1];
+    const int executed = p->niter - before;
+    for (int c = 0; c < executed && c < nb; ++c) {
+      float ms = 0.f;
+      HP_HIP(hipEventElapsedTime(&ms, p->slot_ev[c][0], p->slot_ev[c][1]));
+      p->k_ms[HP_KERNEL_WEIGHTS] += ms;
+      p->k_n[HP_KERNEL_WEIGHTS] += 1;
+      HP_HIP(hipEventElapsedTime(&ms, p->slot_ev[c][1], p->slot_ev[c][2]));
+      p->k_ms[HP_KERNEL_INNER] += ms;
+      p->k_n[HP_KERNEL_INNER] += 1;
+    }
+    p->last_change = p->h_out[0];
+    p->last_entropy = p->h_out[1];
+    if ((int)p->h_out[2] != 0) return fail(HP_E_SOLVER, "a per-atom solver raised; see hp_part_status");
+    if (st[0]) break;
+    batch = std::min(batch * 2, hp_part::kSlots);
+  }
   return 0;
 }
 
@@ -377,6 +415,7 @@ int hp_part_create(int device, const hp_molecule_desc *md, const hp_scheme_desc 
   p->tau = sd->newton_tau;
   p->nlin = sd->newton_linspace;
   p->check_mono = sd->check_mono;
+  p->mixed_switch = sd->mixed_switch > 0.0 ? sd->mixed_switch : 1e-5;
   p->maxiter = sd->maxiter;
   p->inner_maxiter = sd->inner_maxiter;
   p->h_shell_off.assign(md->shell_offsets, md->shell_offsets + S + 1);
@@ -394,7 +433,12 @@ int hp_part_create(int device, const hp_molecule_desc *md, const hp_scheme_desc 
     e = hipEventCreate(&ev);
     if (e != hipSuccess) return bail(fail(HP_E_HIP, "hipEventCreate failed"));
   }
-  e = hipHostMalloc((void **)&p->h_out, 4 * sizeof(double), hipHostMallocDefault);
+  for (auto &slot : p->slot_ev)
+    for (auto &ev : slot) {
+      e = hipEventCreate(&ev);
+      if (e != hipSuccess) return bail(fail(HP_E_HIP, "hipEventCreate failed"));
+    }
+  e = hipHostMalloc((void **)&p->h_out, 8 * sizeof(double), hipHostMallocDefault);
   if (e != hipSuccess) return bail(fail(HP_E_HIP, "hipHostMalloc failed"));
   hipStream_t s = p->stream;
 
@@ -475,7 +519,8 @@ int hp_part_create(int device, const hp_molecule_desc *md, const hp_scheme_desc 
       return bail(rc);
   }
   // state
-  p->hist_cap = p->maxiter + 64;
+  p->hist_cap = 2 * p->maxiter + 64;
+  if ((rc = dalloc(&p->state, 2))) return bail(rc);
   if ((rc = dalloc(&p->coef, nfn)) || (rc = dalloc(&p->alpha, nfn)) || (rc = dalloc(&p->par, p->npar)) ||
       (rc = dalloc(&p->old_par, p->npar)) || (rc = dalloc(&p->rhobar, S)) || (rc = dalloc(&p->ent_part, S)) ||
       (rc = dalloc(&p->charges, A)) || (rc = dalloc(&p->change_part, A)) || (rc = dalloc(&p->out, 4)) ||
@@ -483,7 +528,8 @@ int hp_part_create(int device, const hp_molecule_desc *md, const hp_scheme_desc 
       (rc = dalloc(&p->hpar, (size_t)p->hist_cap * p->npar)) || (rc = dalloc(&p->hch, (size_t)p->hist_cap * A)) ||
       (rc = dalloc(&p->hent, p->hist_cap)) || (rc = dalloc(&p->hchg, p->hist_cap)))
     return bail(rc);
-  e = hipMemsetAsync(p->status_all, 0, sizeof(int) * A, s);
+  e = hipMemsetAsync(p->state, 0, sizeof(int) * 2, s);
+  if (e == hipSuccess) e = hipMemsetAsync(p->status_all, 0, sizeof(int) * A, s);
   if (e == hipSuccess) e = hipMemsetAsync(p->status, 0, sizeof(int) * A, s);
   if (e == hipSuccess) e = hipMemsetAsync(p->inner_niter, 0, sizeof(int) * A, s);
   if (e == hipSuccess) e = hipMemsetAsync(p->charges, 0, sizeof(double) * A, s);
@@ -521,27 +567,24 @@ int hp_part_run(hp_part *p, int precision, int n_polish, int *niter) {
     p->k_n[k] = 0;
   }
   HP_HIP(hipMemsetAsync(p->status_all, 0, sizeof(int) * p->natom, p->stream));
-  int counter = 0;
+  HP_HIP(hipMemsetAsync(p->state, 0, 2 * sizeof(int), p->stream));
+  p->niter = 0;
   int rc = 0;
-  // main loop (iterstock.py:172-188)
-  while (true) {
-    if (counter >= p->hist_cap) return fail(HP_E_ARG, "history capacity exceeded");
-    rc = outer_step(p, precision, counter);
-    counter += 1;
-    p->niter = counter;
-    if (rc) break;
-    if (p->last_change < p->threshold || counter >= p->maxiter) break;
-  }
-  if (!rc && precision == HP_PREC_MIXED) {
-    const int np = std::max(1, n_polish);
-    for (int i = 0; i < np && !rc; ++i) {
-      if (counter >= p->hist_cap) return fail(HP_E_ARG, "history capacity exceeded");
-      rc = outer_step(p, HP_PREC_FP64, counter);
-      counter += 1;
-      p->niter = counter;
+  if (precision == HP_PREC_FP64) {
+    // iterstock.py:172-188: stop when change < threshold or counter >= maxiter
+    rc = run_stage(p, HP_PREC_FP64, p->threshold, 1, p->maxiter);
+  } else {
+    // mixed stage until the change is below max(threshold, switch level),
+    // then FP64 iterations (at least n_polish) until change < threshold
+    const double thr1 = std::max(p->threshold, p->mixed_switch);
+    rc = run_stage(p, HP_PREC_MIXED, thr1, 1, p->maxiter);
+    if (!rc) {
+      const int start = p->niter;
+      const int np = std::max(1, n_polish);
+      rc = run_stage(p, HP_PREC_FP64, p->threshold, start + np, start + std::max(np, p->maxiter));
     }
   }
-  if (niter) *niter = counter;
+  if (niter) *niter = p->niter;
   return rc;
 }
 

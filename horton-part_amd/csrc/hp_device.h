@@ -67,28 +67,65 @@ struct DevFn {
 // reductions (deterministic: fixed tree, result broadcast from lane 0)
 // ---------------------------------------------------------------------------
 
-__device__ __forceinline__ double wave_sum(double v) {
-#pragma unroll
-  for (int o = 32; o > 0; o >>= 1) v += __shfl_down(v, o, 64);
-  return __shfl(v, 0, 64);
+// One DPP step: the value of lane (pattern CTRL) for lanes of the rows in
+// ROWMASK, 0 elsewhere.  Doubles move as two 32-bit halves.
+template <int CTRL, int ROWMASK>
+__device__ __forceinline__ double dpp_d(double v) {
+  const int lo = __builtin_amdgcn_update_dpp(0, __double2loint(v), CTRL, ROWMASK, 0xF, false);
+  const int hi = __builtin_amdgcn_update_dpp(0, __double2hiint(v), CTRL, ROWMASK, 0xF, false);
+  return __hiloint2double(hi, lo);
 }
 
-__device__ __forceinline__ double wave_min(double v) {
-#pragma unroll
-  for (int o = 32; o > 0; o >>= 1) v = fmin(v, __shfl_down(v, o, 64));
-  return __shfl(v, 0, 64);
+__device__ __forceinline__ double lane63(double v) {
+  const int lo = __builtin_amdgcn_readlane(__double2loint(v), 63);
+  const int hi = __builtin_amdgcn_readlane(__double2hiint(v), 63);
+  return __hiloint2double(hi, lo);
 }
 
-// Sum of n values per lane over the wave, n known at compile time.
+// Wave-wide sums with DPP row operations: pairs, quads, half rows, rows, then
+// rows 0+1 and 0..3 by row broadcasts; lane 63 ends with the total, which is
+// read into a scalar register so that every lane sees the same value.  The
+// tree is fixed, so results are deterministic.  All 64 lanes must be active.
 template <int N>
 __device__ __forceinline__ void wave_sum_n(double *v) {
 #pragma unroll
-  for (int o = 32; o > 0; o >>= 1) {
+  for (int i = 0; i < N; ++i) v[i] += dpp_d<0xB1, 0xF>(v[i]);  // quad_perm [1,0,3,2]
 #pragma unroll
-    for (int i = 0; i < N; ++i) v[i] += __shfl_down(v[i], o, 64);
-  }
+  for (int i = 0; i < N; ++i) v[i] += dpp_d<0x4E, 0xF>(v[i]);  // quad_perm [2,3,0,1]
 #pragma unroll
-  for (int i = 0; i < N; ++i) v[i] = __shfl(v[i], 0, 64);
+  for (int i = 0; i < N; ++i) v[i] += dpp_d<0x141, 0xF>(v[i]);  // row_half_mirror
+#pragma unroll
+  for (int i = 0; i < N; ++i) v[i] += dpp_d<0x140, 0xF>(v[i]);  // row_mirror
+#pragma unroll
+  for (int i = 0; i < N; ++i) v[i] += dpp_d<0x142, 0xA>(v[i]);  // row_bcast:15 -> rows 1, 3
+#pragma unroll
+  for (int i = 0; i < N; ++i) v[i] += dpp_d<0x143, 0xC>(v[i]);  // row_bcast:31 -> rows 2, 3
+#pragma unroll
+  for (int i = 0; i < N; ++i) v[i] = lane63(v[i]);
+}
+
+__device__ __forceinline__ double wave_sum(double v) {
+  wave_sum_n<1>(&v);
+  return v;
+}
+
+// Wave-wide minimum (same DPP tree; masked rows contribute +inf).
+template <int CTRL, int ROWMASK>
+__device__ __forceinline__ double dpp_min_step(double v) {
+  const double inf = __builtin_inf();
+  const int lo = __builtin_amdgcn_update_dpp(__double2loint(inf), __double2loint(v), CTRL, ROWMASK, 0xF, false);
+  const int hi = __builtin_amdgcn_update_dpp(__double2hiint(inf), __double2hiint(v), CTRL, ROWMASK, 0xF, false);
+  return fmin(v, __hiloint2double(hi, lo));
+}
+
+__device__ __forceinline__ double wave_min(double v) {
+  v = dpp_min_step<0xB1, 0xF>(v);
+  v = dpp_min_step<0x4E, 0xF>(v);
+  v = dpp_min_step<0x141, 0xF>(v);
+  v = dpp_min_step<0x140, 0xF>(v);
+  v = dpp_min_step<0x142, 0xA>(v);
+  v = dpp_min_step<0x143, 0xC>(v);
+  return lane63(v);
 }
 
 // Block sum of two values (blockDim multiple of 64, <= 1024); thread 0 gets it.
@@ -129,6 +166,17 @@ __device__ __forceinline__ double hp_sqrt<double>(double x) { return sqrt(x); }
 template <>
 __device__ __forceinline__ float hp_sqrt<float>(float x) { return __fsqrt_rn(x); }
 
+// np.clip(w, 0, 1): NaN stays NaN, +-inf go to the bounds.
+__device__ __forceinline__ double clip01(double w) { return w < 0.0 ? 0.0 : (w > 1.0 ? 1.0 : w); }
+
+// AIM weight own/promol (stockholder.py:370-377).  In FP64 the promolecule
+// carries the 1e-100 floor, so the division is the reference's; the FP32
+// promolecule has no floor and an exact zero gives weight 0.
+__device__ __forceinline__ double aim_weight(double own, double promol) { return clip01(own / promol); }
+__device__ __forceinline__ double aim_weight(float own, float promol) {
+  return promol != 0.0f ? clip01((double)(own / promol)) : 0.0;
+}
+
 template <typename R>
 __device__ __forceinline__ R hp_floor_add();
 template <>
@@ -140,19 +188,26 @@ __device__ __forceinline__ float hp_floor_add<float>() { return 0.0f; }
 // function table
 // ---------------------------------------------------------------------------
 
+// Start of an outer iteration: keep the parameters that produce this
+// iteration's weights (old_par) and build the pro-atom function table.
 // MBIS: par[2f] = N, par[2f+1] = S -> coef = N S^3 / (8 pi), alpha = S.
 // Linear bases: coef = c_f * norm_f, alpha fixed.
-__global__ void k_build_fn(int scheme, int nfn, const double *__restrict__ par, const double *__restrict__ norm,
-                           const double *__restrict__ alpha_fixed, double *__restrict__ coef, double *__restrict__ alpha) {
-  const int f = blockIdx.x * blockDim.x + threadIdx.x;
-  if (f >= nfn) return;
-  if (scheme == 0) {
-    const double N = par[2 * f], S = par[2 * f + 1];
-    coef[f] = N * (S * S * S) / (8.0 * HP_PI);
-    alpha[f] = S;
-  } else {
-    coef[f] = par[f] * norm[f];
-    alpha[f] = alpha_fixed[f];
+__global__ void k_iter_begin(const int *__restrict__ done, int scheme, int npar, int nfn, const double *__restrict__ par,
+                             double *__restrict__ old_par, const double *__restrict__ norm,
+                             const double *__restrict__ alpha_fixed, double *__restrict__ coef,
+                             double *__restrict__ alpha) {
+  if (done && *done) return;
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < npar) old_par[i] = par[i];
+  if (i < nfn) {
+    if (scheme == 0) {
+      const double N = par[2 * i], S = par[2 * i + 1];
+      coef[i] = N * (S * S * S) / (8.0 * HP_PI);
+      alpha[i] = S;
+    } else {
+      coef[i] = par[i] * norm[i];
+      alpha[i] = alpha_fixed[i];
+    }
   }
 }
 
@@ -208,10 +263,11 @@ __device__ __forceinline__ void promol_at(const DevMol &m, const DevFn &fn, int 
 // K1: fused weights pass, one workgroup per radial shell
 // ---------------------------------------------------------------------------
 template <typename R, int FK, bool GT2>
-__global__ __launch_bounds__(512) void k_weights(DevMol m, DevFn fn, double *__restrict__ rhobar,
-                                                 double *__restrict__ ent_part, double *__restrict__ promol_out,
-                                                 double dc) {
+__global__ __launch_bounds__(512) void k_weights(const int *__restrict__ done, DevMol m, DevFn fn,
+                                                 double *__restrict__ rhobar, double *__restrict__ ent_part,
+                                                 double *__restrict__ promol_out, double dc) {
   __shared__ double lds[32];
+  if (done && *done) return;
   const int s = blockIdx.x;
   const int a = m.shell_atom[s];
   const int64_t p0 = m.shell_off[s], p1 = m.shell_off[s + 1];
@@ -228,8 +284,7 @@ __global__ __launch_bounds__(512) void k_weights(DevMol m, DevFn fn, double *__r
     const double pm = (double)promol;
     const double rho = m.rho[p];
     if (!GT2) {
-      double w = (promol > (R)0) ? (double)(own / promol) : 0.0;
-      w = fmin(fmax(w, 0.0), 1.0);
+      const double w = aim_weight(own, promol);
       const double f = w * rho;
       acc_avg += m.wsph[p] * f;
     } else {
@@ -259,8 +314,7 @@ __global__ void k_point_weights(DevMol m, DevFn fn, int atom, int64_t p0, int64_
   if (atom < 0) {
     out[i] = promol;
   } else {
-    double w = own / promol;
-    out[i] = fmin(fmax(w, 0.0), 1.0);
+    out[i] = aim_weight(own, promol);
   }
 }
 
@@ -299,11 +353,12 @@ __device__ __forceinline__ void load_radial(const DevMol &m, const double *rhoba
 // the wave-uniform K so that every per-lane array stays in registers).
 // ---------------------------------------------------------------------------
 template <int PPL, int KM>
-__global__ __launch_bounds__(256) void k_mbis_inner(DevMol m, const double *__restrict__ rhobar,
+__global__ __launch_bounds__(256) void k_mbis_inner(const int *__restrict__ done, DevMol m, const double *__restrict__ rhobar,
                                                     double *__restrict__ par, const double *__restrict__ old_par,
                                                     const int *__restrict__ fn_off, double thr, int maxit,
                                                     int *__restrict__ inner_niter, int *__restrict__ status,
                                                     double *__restrict__ charges, double *__restrict__ change_part) {
+  if (done && *done) return;
   const int lane = threadIdx.x & 63;
   const int a = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
   if (a >= m.natom) return;
@@ -335,11 +390,12 @@ __global__ __launch_bounds__(256) void k_mbis_inner(DevMol m, const double *__re
     for (int j = 0; j < PPL; ++j) pro[j] = 0.0;
 #pragma unroll
     for (int k = 0; k < KM; ++k) {
-      const double NS3 = Np[k] * (Sp[k] * Sp[k] * Sp[k]);
+      // N S^3 / (8 pi), once per shell instead of a division per point
+      const double coef = Np[k] * (Sp[k] * Sp[k] * Sp[k]) / (8.0 * HP_PI);
 #pragma unroll
       for (int j = 0; j < PPL; ++j) {
         if (k < K) {
-          t[k][j] = NS3 * exp(-Sp[k] * d.r[j]) / (8.0 * HP_PI);
+          t[k][j] = coef * exp(-Sp[k] * d.r[j]);
           pro[j] += t[k][j];
         } else {
           t[k][j] = 0.0;
@@ -389,8 +445,8 @@ __global__ __launch_bounds__(256) void k_mbis_inner(DevMol m, const double *__re
       if (k < K) {
         const double N1 = Np[k], S1 = Sp[k];
         const double N2 = old_par[2 * (f0 + k)], S2 = old_par[2 * (f0 + k) + 1];
-        y1 += N1 * (S1 * S1 * S1) * exp(-S1 * d.r[j]) / (8.0 * HP_PI);
-        y2 += N2 * (S2 * S2 * S2) * exp(-S2 * d.r[j]) / (8.0 * HP_PI);
+        y1 += N1 * (S1 * S1 * S1) / (8.0 * HP_PI) * exp(-S1 * d.r[j]);
+        y2 += N2 * (S2 * S2 * S2) / (8.0 * HP_PI) * exp(-S2 * d.r[j]);
       }
     }
     const double e = y1 - y2;
@@ -550,7 +606,7 @@ __device__ __forceinline__ double basis_radial(double norm, double alpha, double
 // K4: GISA QP update, one wavefront per atom (gisa.py:281-318, 348-421)
 // ---------------------------------------------------------------------------
 template <int PPL>
-__global__ __launch_bounds__(256) void k_gisa_qp(DevMol m, const double *__restrict__ rhobar, double *__restrict__ par,
+__global__ __launch_bounds__(256) void k_gisa_qp(const int *__restrict__ done, DevMol m, const double *__restrict__ rhobar, double *__restrict__ par,
                                                  const double *__restrict__ old_par, const int *__restrict__ fn_off,
                                                  const double *__restrict__ fn_norm, const double *__restrict__ fn_alpha,
                                                  const double *__restrict__ fn_order, const double *__restrict__ qpP,
@@ -558,6 +614,7 @@ __global__ __launch_bounds__(256) void k_gisa_qp(DevMol m, const double *__restr
                                                  int *__restrict__ inner_niter, int *__restrict__ status,
                                                  double *__restrict__ charges, double *__restrict__ change_part) {
   __shared__ double Mlds[4][(HP_KMAX + 1) * (HP_KMAX + 2)];
+  if (done && *done) return;
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
   const int a = blockIdx.x * (blockDim.x >> 6) + wv;
   if (a >= m.natom) return;
@@ -647,7 +704,7 @@ __device__ __forceinline__ bool wave_any_bad(const double *v, int R, int lane, d
 }
 
 template <int PPL, int KM>
-__global__ __launch_bounds__(256) void k_lisa_inner(DevMol m, const double *__restrict__ rhobar, double *__restrict__ par,
+__global__ __launch_bounds__(256) void k_lisa_inner(const int *__restrict__ done, DevMol m, const double *__restrict__ rhobar, double *__restrict__ par,
                                                     const double *__restrict__ old_par, const int *__restrict__ fn_off,
                                                     const double *__restrict__ fn_norm, const double *__restrict__ fn_alpha,
                                                     const double *__restrict__ fn_order, int solver, double thr, int maxit,
@@ -656,6 +713,7 @@ __global__ __launch_bounds__(256) void k_lisa_inner(DevMol m, const double *__re
                                                     int *__restrict__ status, double *__restrict__ charges,
                                                     double *__restrict__ change_part) {
   __shared__ double Mlds[4][(HP_KMAX + 1) * (HP_KMAX + 2)];
+  if (done && *done) return;
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
   const int a = blockIdx.x * (blockDim.x >> 6) + wv;
   if (a >= m.natom) return;
@@ -885,15 +943,21 @@ __global__ __launch_bounds__(256) void k_lisa_inner(DevMol m, const double *__re
 }
 
 // ---------------------------------------------------------------------------
-// K3: per-iteration finalisation (single workgroup)
+// K3: per-iteration finalisation (single workgroup): entropy and change sums,
+// histories, and the stop decision of iterstock.py:187 on the device:
+// stop when (iteration >= min_stop and change < thr) or iteration >= max_iter
+// or a per-atom solver raised.  state[0] = done, state[1] = iterations so far.
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(256) void k_finalize(int nshell, const double *__restrict__ ent_part, int natom,
+__global__ __launch_bounds__(256) void k_finalize(int *__restrict__ state, double thr, int min_stop, int max_iter,
+                                                  int nshell, const double *__restrict__ ent_part, int natom,
                                                   const double *__restrict__ change_part, const int *__restrict__ status,
                                                   int *__restrict__ status_all, const double *__restrict__ par, int npar,
-                                                  const double *__restrict__ charges, int iter, double *__restrict__ hpar,
+                                                  const double *__restrict__ charges, double *__restrict__ hpar,
                                                   double *__restrict__ hch, double *__restrict__ hent,
                                                   double *__restrict__ hchg, double *__restrict__ out) {
   __shared__ double lds[32];
+  if (state[0]) return;
+  const int iter = state[1];
   double e = 0.0, dummy = 0.0;
   for (int s = threadIdx.x; s < nshell; s += blockDim.x) e += ent_part[s];
   block_sum2(e, dummy, lds);
@@ -902,6 +966,7 @@ __global__ __launch_bounds__(256) void k_finalize(int nshell, const double *__re
     hch[(int64_t)iter * natom + i] = charges[i];
     status_all[i] |= status[i];
   }
+  __syncthreads();
   if (threadIdx.x == 0) {
     double msd = 0.0;
     int err = 0, any = 0;
@@ -918,6 +983,9 @@ __global__ __launch_bounds__(256) void k_finalize(int nshell, const double *__re
     out[1] = e;
     out[2] = (double)err;
     out[3] = (double)any;
+    const int n = iter + 1;
+    state[1] = n;
+    if (err != 0 || n >= max_iter || (n >= min_stop && ch < thr)) state[0] = 1;
   }
 }
 
@@ -938,8 +1006,7 @@ __global__ __launch_bounds__(256) void k_moments(DevMol m, DevFn fn, double *__r
   for (int64_t p = p0 + threadIdx.x; p < p1; p += blockDim.x) {
     double promol, own;
     promol_at<double, FK>(m, f2, a, m.natom, nullptr, m.px[p], m.py[p], m.pz[p], promol, own);
-    double w = own / promol;
-    w = fmin(fmax(w, 0.0), 1.0);
+    const double w = aim_weight(own, promol);
     const double aim = m.rho[p] * w;
     const double wf = m.wat[p] * aim;
     const double x = m.px[p] - m.cx[a], y = m.py[p] - m.cy[a], z = m.pz[p] - m.cz[a];

@@ -104,6 +104,7 @@ class SchemeDesc(ctypes.Structure):
         ("newton_tau", _d),
         ("newton_linspace", ctypes.c_int),
         ("check_mono", ctypes.c_int),
+        ("mixed_switch", _d),
     ]
 
 

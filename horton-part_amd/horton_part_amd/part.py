@@ -259,8 +259,11 @@ class AbstractISAWPart(WPart):
     """Iterative stockholder outer loop on the GPU (core/iterstock.py:61-193).
 
     Extra keyword arguments (defaults reproduce the reference):
-      precision: "fp64" (default) or "mixed" (FP32 grid arithmetic + FP64 polish)
-      n_polish: FP64 iterations after a mixed run (default 1)
+      precision: "fp64" (default) or "mixed" (FP32 grid arithmetic, then FP64 polish)
+      n_polish: minimum FP64 iterations after the mixed stage (default 1)
+      mixed_switch: the mixed stage ends when the change is below
+        max(threshold, mixed_switch) (default 1e-5); FP64 iterations then run
+        until the change is below threshold
       cutoff_radius: pro-atom tails beyond this distance are zero (default None)
       device: GPU ordinal (default 0)
     """
@@ -284,6 +287,7 @@ class AbstractISAWPart(WPart):
         grid_type=1,
         precision="fp64",
         n_polish=1,
+        mixed_switch=1e-5,
         cutoff_radius=None,
         device=0,
         **kwargs,
@@ -295,6 +299,7 @@ class AbstractISAWPart(WPart):
             raise ValueError("precision must be 'fp64' or 'mixed'")
         self._precision = precision
         self._n_polish = n_polish
+        self._mixed_switch = mixed_switch
         self._cutoff_radius = cutoff_radius
         self._device = device
         self._dev = None
@@ -360,6 +365,7 @@ class AbstractISAWPart(WPart):
             "newton_tau": 1.0,
             "newton_linspace": 20,
             "check_mono": 0,
+            "mixed_switch": float(self._mixed_switch),
         }
 
     def _init_propars(self):

@@ -102,6 +102,9 @@ typedef struct {
   double newton_tau;         /* m-newton line-search start (alisa.py:772) */
   int newton_linspace;       /* m-newton line-search size (alisa.py:773) */
   int check_mono;            /* alisa.py:774 */
+  double mixed_switch;       /* HP_PREC_MIXED: leave the mixed stage once the
+                                change is below max(threshold, mixed_switch);
+                                <= 0 selects 1e-5 */
 } hp_scheme_desc;
 
 typedef struct hp_part hp_part;
@@ -134,8 +137,9 @@ int hp_part_set_propars(hp_part *part, const double *propars, int64_t n);
  * (core/stockholder.py:352-384), the per-atom updates (mbis.py:167-203,
  * gisa.py:251-318, alisa.py:1304-1335), the entropy (stockholder.py:145-151)
  * and compute_change (iterstock.py:32-45).  precision: HP_PREC_*.  With
- * HP_PREC_MIXED the loop runs in mixed precision to convergence and then
- * n_polish FP64 iterations follow (at least one).  Returns the number of
+ * HP_PREC_MIXED the loop runs in mixed precision until the change is below
+ * max(threshold, mixed_switch), then in FP64 (at least n_polish iterations)
+ * until the change is below threshold.  Returns the number of
  * outer iterations through *niter.  HP_E_SOLVER if a per-atom solver raised;
  * the loop stops at that iteration. */
 int hp_part_run(hp_part *part, int precision, int n_polish, int *niter);

@@ -34,6 +34,11 @@ def _compare(part, cache, oracle, qtol=1e-10, ptol=1e-9, check_inner=True):
     p1, p2 = part["propars"], cache["propars"]
     rel = np.max(np.abs(p1 - p2) / np.maximum(np.abs(p2), 1e-300))
     _say(f"  niter gpu={part['niter']} oracle={cache['niter']} max|dq|={dq:.2e} rel dpar={rel:.2e}")
+    if part["niter"] == cache["niter"]:
+        hq = np.abs(part["history_charges"] - cache["history_charges"]).max(axis=1)
+        first = int(np.argmax(hq > 1e-12)) if (hq > 1e-12).any() else -1
+        _say(f"  history max|dq| first>1e-12 at iteration {first}; last values {hq[-3:]}")
+        _say(f"  inner niter gpu={list(part.inner_niter)} oracle={[s.get('niter', 1) for s in oracle.inner_stats[-1]]}")
     assert part["niter"] == cache["niter"]
     assert dq <= qtol
     assert dpop <= qtol
@@ -133,6 +138,22 @@ def test_lisa_chain(basis_name, solver):
     part, oracle = _lisa_pair(w, basis_name, solver, maxiter=100)
     part.do_partitioning()
     cache = oracle.do_partitioning()
+    _compare(part, cache, oracle, qtol=1e-10, ptol=1e-8)
+
+
+@pytest.mark.parametrize("basis_name,solver", [("gauss", "m-newton"), ("gauss", "sc")])
+def test_lisa_one_iteration(basis_name, solver):
+    """One outer iteration from identical parameters: isolates the weights
+    pass (spherical averages) from a single per-atom inner solve."""
+    require_gpu()
+    _say(f"case: LISA {solver} {basis_name} one outer iteration, chain 4 atoms seed 5, 50x110")
+    w = workloads.chain(natom=4, seed=5, nrad=50, nang=110)
+    part, oracle = _lisa_pair(w, basis_name, solver, maxiter=1)
+    part.do_partitioning()
+    cache = oracle.do_partitioning()
+    for a in range(4):
+        g, o = part[f"spherical_average_{a}"], oracle.cache[f"spherical_average_{a}"]
+        _say(f"  atom {a}: max rel d(rhobar) = {np.max(np.abs(g - o) / np.maximum(np.abs(o), 1e-300)):.2e}")
     _compare(part, cache, oracle, qtol=1e-10, ptol=1e-8)
 
 

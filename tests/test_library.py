@@ -1,0 +1,97 @@
+"""The C ABI: every function declared in include/hortonpart.h is exported by
+libhortonpart.so, and argument validation answers without a GPU."""
+
+import ctypes
+import os
+import re
+
+import numpy as np
+import pytest
+
+from horton_part_amd import _native
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def _declared():
+    text = open(os.path.join(ROOT, "include", "hortonpart.h")).read()
+    return sorted(set(re.findall(r"^\s*(?:int|const char \*)\s*(hp_\w+)\s*\(", text, re.M)))
+
+
+def test_header_symbols_exported():
+    lib = _native.load()
+    names = _declared()
+    assert len(names) >= 15
+    for name in names:
+        assert hasattr(lib, name), name
+    assert sorted(_native.EXPORTED_SYMBOLS) == names
+
+
+def test_abi_version_and_errors():
+    lib = _native.load()
+    assert lib.hp_abi_version() == 1
+    n = ctypes.c_int32(-1)
+    lib.hp_device_count(ctypes.byref(n))
+    assert n.value >= 0
+
+
+def _desc(**over):
+    natom, nshell, L = 2, 3, 4
+    npoint = nshell * L
+    arrays = {
+        "natom": natom,
+        "coordinates": np.zeros((natom, 3)),
+        "numbers": np.array([1, 1], np.int64),
+        "pseudo_numbers": np.ones(natom),
+        "npoint": npoint,
+        "points": np.zeros((npoint, 3)),
+        "density": np.ones(npoint),
+        "mol_weights": np.ones(npoint),
+        "atom_weights": np.ones(npoint),
+        "sph_weights": np.full(npoint, 0.25),
+        "nshell": nshell,
+        "shell_atom": np.array([0, 0, 1], np.int32),
+        "shell_offsets": np.arange(0, npoint + 1, L, dtype=np.int64),
+        "shell_radius": np.ones(nshell),
+        "shell_rweight": np.ones(nshell),
+    }
+    scheme = {
+        "scheme": 0,
+        "solver": 0,
+        "grid_type": 1,
+        "fn_offsets": np.array([0, 1, 2], np.int32),
+        "threshold": 1e-6,
+        "inner_threshold": 1e-8,
+        "maxiter": 10,
+        "inner_maxiter": 100,
+        "density_cutoff": 1e-15,
+        "negative_cutoff": -1e-12,
+        "population_cutoff": 1e-4,
+        "cutoff_radius": 0.0,
+        "newton_tau": 1.0,
+        "newton_linspace": 20,
+        "check_mono": 0,
+        "npar": 4,
+    }
+    for k, v in over.items():
+        (scheme if k in scheme else arrays)[k] = v
+    return arrays, scheme
+
+
+@pytest.mark.parametrize(
+    "over,code",
+    [
+        (dict(shell_atom=np.array([1, 0, 0], np.int32)), _native.HP_E_ARG),
+        (dict(shell_offsets=np.array([0, 4, 8, 11], np.int64)), _native.HP_E_ARG),
+        (dict(grid_type=2), _native.HP_E_UNSUPP),
+        (dict(solver=1), _native.HP_E_ARG),
+        (dict(fn_offsets=np.array([0, 8, 9], np.int32)), _native.HP_E_UNSUPP),
+        (dict(maxiter=0), _native.HP_E_ARG),
+    ],
+)
+def test_create_rejects_bad_descriptions(over, code):
+    """Validation happens before any device call, so it runs on the CPU."""
+    arrays, scheme = _desc(**over)
+    with pytest.raises(_native.HPError) as exc:
+        _native.DevicePartition(arrays, scheme)
+    assert exc.value.code == code
