@@ -169,16 +169,43 @@ int pick_ppl(int R) {
 
 // ---- kernel launch helpers -------------------------------------------------
 
+// Points per thread of the weights pass: 1 or 2, whichever idles fewer lanes
+// for the largest shell (ties go to 2, which halves the LDS reads per point).
+inline int weights_pt(int maxL) {
+  auto lanes = [](int n) { return ((n + 63) / 64) * 64; };
+  const double e1 = (double)maxL / lanes(maxL);
+  const double e2 = (double)maxL / (2.0 * lanes((maxL + 1) / 2));
+  return (maxL > 512 || e2 >= e1) ? 2 : 1;
+}
+
+template <typename R, int FK, bool GT2, int PT>
+void launch_weights_pt(hp_part *p, int bs, size_t lds, int tile, int kslot) {
+  hipLaunchKernelGGL((k_weights<R, FK, GT2, PT>), dim3(p->nshell), dim3(bs), lds, p->stream, p->state, p->mol(), p->fn(),
+                     p->rhobar, p->ent_part, p->promol, p->dc, tile, kslot);
+}
+
 template <typename R, int FK>
 void launch_weights_fk(hp_part *p, bool gt2) {
-  int bs = ((p->maxL + 63) / 64) * 64;
+  // one workgroup per radial shell, each thread one or two points
+  const int pt = weights_pt(p->maxL);
+  int bs = ((p->maxL + pt - 1) / pt + 63) / 64 * 64;
   bs = std::max(64, std::min(512, bs));
-  if (gt2)
-    hipLaunchKernelGGL((k_weights<R, FK, true>), dim3(p->nshell), dim3(bs), 0, p->stream, p->state, p->mol(), p->fn(),
-                       p->rhobar, p->ent_part, p->promol, p->dc);
-  else
-    hipLaunchKernelGGL((k_weights<R, FK, false>), dim3(p->nshell), dim3(bs), 0, p->stream, p->state, p->mol(), p->fn(),
-                       p->rhobar, p->ent_part, p->promol, p->dc);
+  const int kslot = std::max(1, p->maxK);
+  const size_t per_atom = weights_lds_per_atom<R>(kslot);
+  int tile = (int)std::min<size_t>(256, (40 * 1024) / per_atom);
+  tile = std::max(1, std::min(tile, p->natom));
+  const size_t lds = per_atom * tile + 16;
+  if (gt2) {
+    if (pt == 1)
+      launch_weights_pt<R, FK, true, 1>(p, bs, lds, tile, kslot);
+    else
+      launch_weights_pt<R, FK, true, 2>(p, bs, lds, tile, kslot);
+  } else {
+    if (pt == 1)
+      launch_weights_pt<R, FK, false, 1>(p, bs, lds, tile, kslot);
+    else
+      launch_weights_pt<R, FK, false, 2>(p, bs, lds, tile, kslot);
+  }
 }
 
 template <typename R>
@@ -195,14 +222,19 @@ template <int PPL>
 int launch_inner_ppl(hp_part *p) {
   const dim3 grid((p->natom + 3) / 4), block(256);
   if (p->scheme == HP_SCHEME_MBIS) {
-    if (p->maxK <= 3)
-      hipLaunchKernelGGL((k_mbis_inner<PPL, 3>), grid, block, 0, p->stream, p->state, p->mol(), p->rhobar, p->par, p->old_par,
-                         p->fn_off, p->inner_threshold, p->inner_maxiter, p->inner_niter, p->status, p->charges,
-                         p->change_part);
+    // one workgroup per atom, one radial point per lane
+    const dim3 g1(p->natom), b1(((p->maxR + 63) / 64) * 64);
+#define HP_MBIS(KMV)                                                                                                  \
+  hipLaunchKernelGGL((k_mbis_inner<KMV>), g1, b1, 0, p->stream, p->state, p->mol(), p->rhobar, p->par, p->old_par,    \
+                     p->fn_off, p->inner_threshold, p->inner_maxiter, p->inner_niter, p->status, p->charges,          \
+                     p->change_part)
+    if (p->maxK <= 2)
+      HP_MBIS(2);
+    else if (p->maxK <= 3)
+      HP_MBIS(3);
     else
-      hipLaunchKernelGGL((k_mbis_inner<PPL, 7>), grid, block, 0, p->stream, p->state, p->mol(), p->rhobar, p->par, p->old_par,
-                         p->fn_off, p->inner_threshold, p->inner_maxiter, p->inner_niter, p->status, p->charges,
-                         p->change_part);
+      HP_MBIS(7);
+#undef HP_MBIS
   } else if (p->solver == HP_SOLVER_QP) {
     hipLaunchKernelGGL((k_gisa_qp<PPL>), grid, block, 0, p->stream, p->state, p->mol(), p->rhobar, p->par, p->old_par, p->fn_off,
                        p->fn_norm, p->fn_alpha, p->fn_order, p->qpP, p->qp_off, p->pop_cutoff, p->neg_cutoff,
@@ -369,6 +401,7 @@ int hp_part_create(int device, const hp_molecule_desc *md, const hp_scheme_desc 
     if (L <= 0) return fail(HP_E_ARG, "empty radial shell");
     maxL = std::max<int64_t>(maxL, L);
   }
+  if (maxL > 1024) return fail(HP_E_UNSUPP, "more than 1024 points in one radial shell");
   for (int a = 0; a < A; ++a) maxR = std::max(maxR, atom_shell_off[a + 1] - atom_shell_off[a]);
   if (pick_ppl(maxR) == 0) return fail(HP_E_UNSUPP, "more than 512 radial points per atom");
   const int nfn = sd->fn_offsets[A];

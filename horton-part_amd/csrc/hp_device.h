@@ -260,14 +260,36 @@ __device__ __forceinline__ void promol_at(const DevMol &m, const DevFn &fn, int 
 }
 
 // ---------------------------------------------------------------------------
-// K1: fused weights pass, one workgroup per radial shell
+// K1: fused weights pass, one workgroup per radial shell.
+// The atoms that can reach the shell (all atoms, or the neighbour list when a
+// cut-off is set) are visited in tiles staged in LDS: centre, function count,
+// coefficients and exponents per atom, read by every lane as a broadcast.
+// FP64 path: distances from absolute coordinates.  FP32 path: coordinates
+// relative to the shell's atom (point offsets formed once in FP64), so the
+// per-pair arithmetic is all FP32.
+// LDS layout (dynamic): R cx[tile], cy[tile], cz[tile], coef[tile*kslot],
+// alpha[tile*kslot]; int idx[tile], nf[tile].
 // ---------------------------------------------------------------------------
-template <typename R, int FK, bool GT2>
+template <typename R>
+__host__ __device__ constexpr size_t weights_lds_per_atom(int kslot) {
+  return (3 + 2 * (size_t)kslot) * sizeof(R) + 2 * sizeof(int);
+}
+
+template <typename R, int FK, bool GT2, int PT>
 __global__ __launch_bounds__(512) void k_weights(const int *__restrict__ done, DevMol m, DevFn fn,
                                                  double *__restrict__ rhobar, double *__restrict__ ent_part,
-                                                 double *__restrict__ promol_out, double dc) {
+                                                 double *__restrict__ promol_out, double dc, int tile, int kslot) {
+  extern __shared__ double smem_d[];
   __shared__ double lds[32];
   if (done && *done) return;
+  R *s_cx = reinterpret_cast<R *>(smem_d);
+  R *s_cy = s_cx + tile;
+  R *s_cz = s_cy + tile;
+  R *s_coef = s_cz + tile;
+  R *s_alpha = s_coef + (size_t)tile * kslot;
+  int *s_idx = reinterpret_cast<int *>(s_alpha + (size_t)tile * kslot);
+  int *s_nf = s_idx + tile;
+
   const int s = blockIdx.x;
   const int a = m.shell_atom[s];
   const int64_t p0 = m.shell_off[s], p1 = m.shell_off[s + 1];
@@ -277,14 +299,103 @@ __global__ __launch_bounds__(512) void k_weights(const int *__restrict__ done, D
     nb = fn.nb_list + (int64_t)a * m.natom;
     nb_n = fn.shell_nb[s];
   }
+  const bool cull = fn.rc > 0.0;
+  const R rc = (R)fn.rc;
+  const double ax = m.cx[a], ay = m.cy[a], az = m.cz[a];
+  int64_t pidx[PT];
+  double px[PT], py[PT], pz[PT];
+  R rx[PT], ry[PT], rz[PT];
+  R promol[PT], own[PT];
+#pragma unroll
+  for (int j = 0; j < PT; ++j) {
+    pidx[j] = p0 + threadIdx.x + (int64_t)j * blockDim.x;
+    const bool ok = pidx[j] < p1;
+    px[j] = ok ? m.px[pidx[j]] : ax;
+    py[j] = ok ? m.py[pidx[j]] : ay;
+    pz[j] = ok ? m.pz[pidx[j]] : az;
+    rx[j] = (R)(px[j] - ax);
+    ry[j] = (R)(py[j] - ay);
+    rz[j] = (R)(pz[j] - az);
+    promol[j] = (R)0;
+    own[j] = (R)0;
+  }
+  const bool two = p0 + blockDim.x < p1;  // second point slot used by any thread
+  for (int t0 = 0; t0 < nb_n; t0 += tile) {
+    const int tn = min(tile, nb_n - t0);
+    __syncthreads();
+    for (int i = threadIdx.x; i < tn; i += blockDim.x) {
+      const int b = nb ? nb[t0 + i] : t0 + i;
+      s_idx[i] = b;
+      if (sizeof(R) == sizeof(double)) {
+        s_cx[i] = (R)m.cx[b];
+        s_cy[i] = (R)m.cy[b];
+        s_cz[i] = (R)m.cz[b];
+      } else {
+        s_cx[i] = (R)(m.cx[b] - ax);
+        s_cy[i] = (R)(m.cy[b] - ay);
+        s_cz[i] = (R)(m.cz[b] - az);
+      }
+      const int f0 = fn.fn_off[b], nf = fn.fn_off[b + 1] - f0;
+      s_nf[i] = nf;
+      for (int k = 0; k < nf; ++k) {
+        s_coef[i * kslot + k] = (R)fn.coef[f0 + k];
+        s_alpha[i * kslot + k] = (R)fn.alpha[f0 + k];
+      }
+    }
+    __syncthreads();
+    for (int i = 0; i < tn; ++i) {
+      const int b = s_idx[i];
+      const int nf = s_nf[i];
+      const R bx = s_cx[i], by = s_cy[i], bz = s_cz[i];
+#pragma unroll
+      for (int j = 0; j < PT; ++j) {
+        if (j > 0 && !two) break;
+        R dx, dy, dz;
+        if (sizeof(R) == sizeof(double)) {
+          dx = (R)(px[j] - (double)bx);
+          dy = (R)(py[j] - (double)by);
+          dz = (R)(pz[j] - (double)bz);
+        } else {
+          dx = rx[j] - bx;
+          dy = ry[j] - by;
+          dz = rz[j] - bz;
+        }
+        const R r2 = dx * dx + dy * dy + dz * dz;
+        const R r = (FK == HP_FN_GAUSS && !cull) ? (R)0 : hp_sqrt<R>(r2);
+        R pb = (R)0;
+        if (!cull || r <= rc) {
+          for (int k = 0; k < nf; ++k) {
+            const R al = s_alpha[i * kslot + k];
+            R arg;
+            if (FK == HP_FN_SLATER) {
+              arg = r;
+            } else if (FK == HP_FN_GAUSS) {
+              arg = r2;
+            } else {
+              const double n = fn.order[fn.fn_off[b] + k];
+              arg = (n == 1.0) ? r : ((n == 2.0) ? r2 : (R)pow((double)r, n));
+            }
+            pb += s_coef[i * kslot + k] * hp_exp<R>(-al * arg);
+          }
+        }
+        promol[j] += pb;
+        promol[j] += hp_floor_add<R>();
+        if (b == a) own[j] = pb;
+      }
+    }
+  }
   double acc_avg = 0.0, acc_ent = 0.0;
-  for (int64_t p = p0 + threadIdx.x; p < p1; p += blockDim.x) {
-    R promol, own;
-    promol_at<R, FK>(m, fn, a, nb_n, nb, m.px[p], m.py[p], m.pz[p], promol, own);
-    const double pm = (double)promol;
+#pragma unroll
+  for (int j = 0; j < PT; ++j) {
+    if (pidx[j] >= p1) continue;
+    const int64_t p = pidx[j];
+    R pm_r = promol[j];
+    // culled atoms still contribute their 1e-100 floor
+    if (nb) pm_r += (R)(m.natom - nb_n) * hp_floor_add<R>();
+    const double pm = (double)pm_r;
     const double rho = m.rho[p];
     if (!GT2) {
-      const double w = aim_weight(own, promol);
+      const double w = aim_weight(own[j], pm_r);
       const double f = w * rho;
       acc_avg += m.wsph[p] * f;
     } else {
@@ -348,28 +459,57 @@ __device__ __forceinline__ void load_radial(const DevMol &m, const double *rhoba
 }
 
 // ---------------------------------------------------------------------------
-// K2: MBIS inner fixed point, one wavefront per atom (mbis.py:81-163)
-// KM: compile-time bound on shells per atom (loops fully unrolled, guarded by
-// the wave-uniform K so that every per-lane array stays in registers).
+// K2: MBIS inner fixed point (mbis.py:81-163), one workgroup per atom with one
+// radial point per lane (ceil(R/64) waves).  Each iteration reduces 2K+1
+// values: a DPP tree inside every wave, then the wave totals through LDS in
+// wave order, so all lanes of all waves hold bitwise identical sums and take
+// the same convergence branch.  The LDS slots alternate with the iteration
+// parity, which needs one barrier per iteration.
+// KM: compile-time bound on shells per atom (loops unrolled, guarded by K).
 // ---------------------------------------------------------------------------
-template <int PPL, int KM>
-__global__ __launch_bounds__(256) void k_mbis_inner(const int *__restrict__ done, DevMol m, const double *__restrict__ rhobar,
-                                                    double *__restrict__ par, const double *__restrict__ old_par,
-                                                    const int *__restrict__ fn_off, double thr, int maxit,
-                                                    int *__restrict__ inner_niter, int *__restrict__ status,
-                                                    double *__restrict__ charges, double *__restrict__ change_part) {
+template <int NV>
+__device__ __forceinline__ void block_allsum(double *v, double *lds, int parity) {
+  // lds: 2 x 16 waves x NV doubles
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6, nw = blockDim.x >> 6;
+  wave_sum_n<NV>(v);
+  double *slot = lds + (size_t)parity * 16 * NV;
+  if (lane == 0) {
+#pragma unroll
+    for (int i = 0; i < NV; ++i) slot[wv * NV + i] = v[i];
+  }
+  __syncthreads();
+#pragma unroll
+  for (int i = 0; i < NV; ++i) {
+    double t = 0.0;
+    for (int w = 0; w < nw; ++w) t += slot[w * NV + i];
+    v[i] = t;
+  }
+}
+
+template <int KM>
+__global__ __launch_bounds__(512) void k_mbis_inner(const int *__restrict__ done, DevMol m,
+                                                    const double *__restrict__ rhobar, double *__restrict__ par,
+                                                    const double *__restrict__ old_par, const int *__restrict__ fn_off,
+                                                    double thr, int maxit, int *__restrict__ inner_niter,
+                                                    int *__restrict__ status, double *__restrict__ charges,
+                                                    double *__restrict__ change_part) {
+  constexpr int NV = 2 * KM + 1;
+  __shared__ double lds[2 * 16 * NV];
   if (done && *done) return;
-  const int lane = threadIdx.x & 63;
-  const int a = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
-  if (a >= m.natom) return;
+  const int a = blockIdx.x;
   const int s0 = m.atom_shell_off[a], R = m.atom_shell_off[a + 1] - s0;
   const int f0 = fn_off[a], K = fn_off[a + 1] - f0;
-  RadialLane<PPL> d;
-  load_radial<PPL>(m, rhobar, s0, R, lane, d);
-  double pop = 0.0;
-#pragma unroll
-  for (int j = 0; j < PPL; ++j) pop += d.w[j] * d.rho[j];
-  pop = wave_sum(pop);
+  const int i = threadIdx.x;
+  const bool valid = i < R;
+  const double r = valid ? m.shell_r[s0 + i] : 0.0;
+  const double wr = valid ? m.shell_w[s0 + i] : 0.0;
+  const double w = 4.0 * HP_PI * (r * r) * wr;
+  const double rho = valid ? rhobar[s0 + i] : 0.0;
+  int slot = 0;  // LDS slot of the next block reduction (alternates)
+  double pv = w * rho;
+  block_allsum<1>(&pv, lds, slot);
+  slot ^= 1;
+  const double pop = pv;
 
   double Np[KM], Sp[KM];
 #pragma unroll
@@ -377,48 +517,37 @@ __global__ __launch_bounds__(256) void k_mbis_inner(const int *__restrict__ done
     Np[k] = k < K ? par[2 * (f0 + k)] : 0.0;
     Sp[k] = k < K ? par[2 * (f0 + k) + 1] : 1.0;
   }
-  double oldpro[PPL];
-#pragma unroll
-  for (int j = 0; j < PPL; ++j) oldpro[j] = 0.0;
+  double oldpro = 0.0;
   bool have_old = false, conv = false;
   int it = 0;
   const double dc = 1e-15;  // opt_mbis_propars default density_cutoff (mbis.py:82)
   for (it = 0; it < maxit; ++it) {
-    double t[KM][PPL];
-    double pro[PPL];
-#pragma unroll
-    for (int j = 0; j < PPL; ++j) pro[j] = 0.0;
+    double t[KM];
+    double pro = 0.0;
 #pragma unroll
     for (int k = 0; k < KM; ++k) {
       // N S^3 / (8 pi), once per shell instead of a division per point
       const double coef = Np[k] * (Sp[k] * Sp[k] * Sp[k]) / (8.0 * HP_PI);
-#pragma unroll
-      for (int j = 0; j < PPL; ++j) {
-        if (k < K) {
-          t[k][j] = coef * exp(-Sp[k] * d.r[j]);
-          pro[j] += t[k][j];
-        } else {
-          t[k][j] = 0.0;
-        }
+      if (k < K) {
+        t[k] = coef * exp(-Sp[k] * r);
+        pro += t[k];
+      } else {
+        t[k] = 0.0;
       }
     }
-    double red[2 * KM + 1];
+    const bool sick = (rho < dc) || (pro < dc);
+    const double ratio = sick ? 0.0 : rho / pro;
+    double red[NV];
 #pragma unroll
-    for (int q = 0; q < 2 * KM + 1; ++q) red[q] = 0.0;
-#pragma unroll
-    for (int j = 0; j < PPL; ++j) {
-      const bool sick = (d.rho[j] < dc) || (pro[j] < dc);
-      const double ratio = sick ? 0.0 : d.rho[j] / pro[j];
-#pragma unroll
-      for (int k = 0; k < KM; ++k) {
-        const double wt = d.w[j] * (t[k][j] * ratio);
-        red[2 * k] += wt;
-        red[2 * k + 1] += wt * d.r[j];
-      }
-      const double e = oldpro[j] - pro[j];
-      red[2 * KM] += d.w[j] * e * e;
+    for (int k = 0; k < KM; ++k) {
+      const double wt = w * (t[k] * ratio);
+      red[2 * k] = wt;
+      red[2 * k + 1] = wt * r;
     }
-    wave_sum_n<2 * KM + 1>(red);
+    const double e = oldpro - pro;
+    red[2 * KM] = w * e * e;
+    block_allsum<NV>(red, lds, slot);
+    slot ^= 1;
 #pragma unroll
     for (int k = 0; k < KM; ++k) {
       if (k < K) {
@@ -431,29 +560,24 @@ __global__ __launch_bounds__(256) void k_mbis_inner(const int *__restrict__ done
       conv = true;
       break;
     }
-#pragma unroll
-    for (int j = 0; j < PPL; ++j) oldpro[j] = pro[j];
+    oldpro = pro;
     have_old = true;
   }
   // change of the outer loop: new vs old pro-atom on the radial grid
-  double msd = 0.0;
+  double y1 = 0.0, y2 = 0.0;
 #pragma unroll
-  for (int j = 0; j < PPL; ++j) {
-    double y1 = 0.0, y2 = 0.0;
-#pragma unroll
-    for (int k = 0; k < KM; ++k) {
-      if (k < K) {
-        const double N1 = Np[k], S1 = Sp[k];
-        const double N2 = old_par[2 * (f0 + k)], S2 = old_par[2 * (f0 + k) + 1];
-        y1 += N1 * (S1 * S1 * S1) / (8.0 * HP_PI) * exp(-S1 * d.r[j]);
-        y2 += N2 * (S2 * S2 * S2) / (8.0 * HP_PI) * exp(-S2 * d.r[j]);
-      }
+  for (int k = 0; k < KM; ++k) {
+    if (k < K) {
+      const double N1 = Np[k], S1 = Sp[k];
+      const double N2 = old_par[2 * (f0 + k)], S2 = old_par[2 * (f0 + k) + 1];
+      y1 += N1 * (S1 * S1 * S1) / (8.0 * HP_PI) * exp(-S1 * r);
+      y2 += N2 * (S2 * S2 * S2) / (8.0 * HP_PI) * exp(-S2 * r);
     }
-    const double e = y1 - y2;
-    msd += d.wr[j] * (4.0 * HP_PI * (d.r[j] * d.r[j])) * e * e;
   }
-  msd = wave_sum(msd);
-  if (lane == 0) {
+  const double dy = y1 - y2;
+  double msd = wr * (4.0 * HP_PI * (r * r)) * dy * dy;
+  block_allsum<1>(&msd, lds, slot);
+  if (threadIdx.x == 0) {
     int st = 0;
     double sumN = 0.0;
 #pragma unroll
