@@ -16,6 +16,7 @@
 
 #include "hortonpart.h"
 #include "hp_device.h"
+#include "hp_gt2.h"
 
 namespace {
 
@@ -86,6 +87,11 @@ struct hp_part {
          *charges = nullptr, *change_part = nullptr, *out = nullptr, *mom_part = nullptr, *mom_out = nullptr,
          *scratch_pts = nullptr;
   int *inner_niter = nullptr, *status = nullptr, *status_all = nullptr;
+  // grid_type 2: per-atom solver states, chunk partials, active-atom counter
+  Gt2Atom *gt2 = nullptr;
+  double *gt2_partial = nullptr;
+  int *gt2_active = nullptr;
+  int gt2_nchunk = 0;
   double *hpar = nullptr, *hch = nullptr, *hent = nullptr, *hchg = nullptr;
   int hist_cap = 0;
   double *h_out = nullptr;  // pinned: change, entropy, error bits, any bits, done, iterations
@@ -151,6 +157,9 @@ void free_part(hp_part *p) {
     if (b) (void)hipFree(b);
   if (p->h_out) (void)hipHostFree(p->h_out);
   if (p->state) (void)hipFree(p->state);
+  if (p->gt2) (void)hipFree(p->gt2);
+  if (p->gt2_partial) (void)hipFree(p->gt2_partial);
+  if (p->gt2_active) (void)hipFree(p->gt2_active);
   for (auto &e : p->ev)
     if (e) (void)hipEventDestroy(e);
   for (auto &slot : p->slot_ev)
@@ -336,6 +345,129 @@ int run_stage(hp_part *p, int precision, double thr, int min_stop, int max_iter)
 
 }  // namespace
 
+namespace {
+
+// One outer iteration on the molecular grid (grid_type 2): weights pass that
+// stores the promolecule, then rounds of per-atom passes until every atom's
+// solver has finished, then the finalisation.  Returns with the device state
+// read back (done flag, iteration count).
+int gt2_iteration(hp_part *p, int precision, double thr, int min_stop, int max_iter) {
+  hipStream_t s = p->stream;
+  const int n = std::max(p->npar, p->nfn);
+  hipLaunchKernelGGL(k_iter_begin, dim3((n + 255) / 256), dim3(256), 0, s, p->state, p->scheme == HP_SCHEME_MBIS ? 0 : 1,
+                     p->npar, p->nfn, p->par, p->old_par, p->fn_norm, p->fn_alpha, p->coef, p->alpha);
+  HP_LAUNCHED();
+  HP_HIP(hipEventRecord(p->ev[0], s));
+  if (precision == HP_PREC_MIXED)
+    launch_weights<float>(p, true);
+  else
+    launch_weights<double>(p, true);
+  HP_LAUNCHED();
+  HP_HIP(hipEventRecord(p->ev[1], s));
+  hipLaunchKernelGGL(k_gt2_init, dim3((p->natom + 127) / 128), dim3(128), 0, s, p->state, p->natom, p->gt2, p->fn_off,
+                     p->scheme == HP_SCHEME_MBIS ? 0 : 1, p->par);
+  HP_LAUNCHED();
+  const double dc = p->scheme == HP_SCHEME_MBIS ? 1e-15 : p->dc;  // mbis.py:82 default cutoff
+  const int scheme = p->scheme == HP_SCHEME_MBIS ? 0 : p->scheme;
+  const dim3 grid(p->gt2_nchunk, p->natom), block(256);
+  // every round advances each unfinished atom by one step; bound the rounds
+  const int max_rounds = 2 * p->inner_maxiter + 8;
+  int rounds = 0;
+  int *h_active = reinterpret_cast<int *>(p->h_out + 6);
+  bool finished = false;
+  while (!finished) {
+    const int nb = 2;
+    for (int c = 0; c < nb; ++c) {
+      hipLaunchKernelGGL(k_gt2_zero, dim3(1), dim3(1), 0, s, p->gt2_active);
+      HP_HIP(hipEventRecord(p->slot_ev[c][0], s));
+      const int kind = scheme == 0 ? 0 : ((p->solver == HP_SOLVER_NEWTON || p->solver == HP_SOLVER_MNEWTON) ? 2 : 1);
+#define HP_GT2_PASS(RT, KD)                                                                                      \
+  hipLaunchKernelGGL((k_gt2_pass<RT, KD>), grid, block, 0, s, p->state, p->mol(), p->promol, p->gt2, p->fn_off, \
+                     p->fn_norm, p->fn_alpha, p->fn_order, scheme, p->solver, dc, p->gt2_partial)
+      if (precision == HP_PREC_MIXED) {
+        if (kind == 0)
+          HP_GT2_PASS(float, 0);
+        else if (kind == 1)
+          HP_GT2_PASS(float, 1);
+        else
+          HP_GT2_PASS(float, 2);
+      } else {
+        if (kind == 0)
+          HP_GT2_PASS(double, 0);
+        else if (kind == 1)
+          HP_GT2_PASS(double, 1);
+        else
+          HP_GT2_PASS(double, 2);
+      }
+#undef HP_GT2_PASS
+      HP_LAUNCHED();
+      HP_HIP(hipEventRecord(p->slot_ev[c][1], s));
+      if (p->solver == HP_SOLVER_MNEWTON) {
+        if (precision == HP_PREC_MIXED)
+          hipLaunchKernelGGL(k_gt2_linesearch<float>, grid, block, 0, s, p->state, p->mol(), p->gt2, p->fn_off,
+                             p->fn_norm, p->fn_alpha, p->fn_order, p->tau, p->nlin, p->neg_cutoff, p->gt2_partial);
+        else
+          hipLaunchKernelGGL(k_gt2_linesearch<double>, grid, block, 0, s, p->state, p->mol(), p->gt2, p->fn_off,
+                             p->fn_norm, p->fn_alpha, p->fn_order, p->tau, p->nlin, p->neg_cutoff, p->gt2_partial);
+        HP_LAUNCHED();
+      }
+      hipLaunchKernelGGL(k_gt2_update, dim3(p->natom), dim3(128), 0, s, p->state, p->gt2_nchunk, p->gt2, p->fn_off,
+                         p->qpP, p->qp_off, p->gt2_partial, scheme, p->solver, p->inner_threshold, p->inner_maxiter,
+                         p->neg_cutoff, p->pop_cutoff, p->tau, p->nlin, p->gt2_active);
+      HP_LAUNCHED();
+      HP_HIP(hipEventRecord(p->slot_ev[c][2], s));
+    }
+    HP_HIP(hipMemcpyAsync(h_active, p->gt2_active, sizeof(int), hipMemcpyDeviceToHost, s));
+    HP_HIP(hipStreamSynchronize(s));
+    for (int c = 0; c < nb; ++c) {
+      float ms = 0.f;
+      HP_HIP(hipEventElapsedTime(&ms, p->slot_ev[c][0], p->slot_ev[c][1]));
+      p->k_ms[HP_KERNEL_LISA_GRID] += ms;
+      p->k_n[HP_KERNEL_LISA_GRID] += 1;
+      HP_HIP(hipEventElapsedTime(&ms, p->slot_ev[c][0], p->slot_ev[c][2]));
+      p->k_ms[HP_KERNEL_INNER] += ms;
+      p->k_n[HP_KERNEL_INNER] += 1;
+    }
+    rounds += nb;
+    finished = *h_active == 0;
+    if (!finished && rounds >= max_rounds) return fail(HP_E_SOLVER, "grid_type 2 inner solver did not finish");
+  }
+  float ms = 0.f;
+  HP_HIP(hipEventElapsedTime(&ms, p->ev[0], p->ev[1]));
+  p->k_ms[HP_KERNEL_WEIGHTS] += ms;
+  p->k_n[HP_KERNEL_WEIGHTS] += 1;
+  hipLaunchKernelGGL(k_gt2_finish, dim3((p->natom + 127) / 128), dim3(128), 0, s, p->state, p->mol(), p->natom, p->gt2,
+                     p->fn_off, p->scheme == HP_SCHEME_MBIS ? 0 : 1, p->par, p->inner_niter, p->status, p->charges,
+                     p->change_part);
+  HP_LAUNCHED();
+  hipLaunchKernelGGL(k_finalize, dim3(1), dim3(256), 0, s, p->state, thr, min_stop, max_iter, p->nshell, p->ent_part,
+                     p->natom, p->change_part, p->status, p->status_all, p->par, p->npar, p->charges, p->hpar, p->hch,
+                     p->hent, p->hchg, p->out);
+  HP_LAUNCHED();
+  HP_HIP(hipMemcpyAsync(p->h_out, p->out, 4 * sizeof(double), hipMemcpyDeviceToHost, s));
+  HP_HIP(hipMemcpyAsync(p->h_out + 4, p->state, 2 * sizeof(int), hipMemcpyDeviceToHost, s));
+  HP_HIP(hipStreamSynchronize(s));
+  return 0;
+}
+
+int run_stage_gt2(hp_part *p, int precision, double thr, int min_stop, int max_iter) {
+  HP_HIP(hipMemsetAsync(p->state, 0, sizeof(int), p->stream));
+  while (true) {
+    int rc = gt2_iteration(p, precision, thr, min_stop, max_iter);
+    if (rc) return rc;
+    int st[2];
+    std::memcpy(st, p->h_out + 4, sizeof(st));
+    p->niter = st[1];
+    p->last_change = p->h_out[0];
+    p->last_entropy = p->h_out[1];
+    if ((int)p->h_out[2] != 0) return fail(HP_E_SOLVER, "a per-atom solver raised; see hp_part_status");
+    if (st[0]) break;
+  }
+  return 0;
+}
+
+}  // namespace
+
 extern "C" {
 
 int hp_abi_version(void) { return HP_ABI_VERSION; }
@@ -366,7 +498,8 @@ int hp_part_create(int device, const hp_molecule_desc *md, const hp_scheme_desc 
       !md->shell_rweight)
     return fail(HP_E_ARG, "molecule description has a NULL array");
   if (sd->scheme < 0 || sd->scheme > 2) return fail(HP_E_ARG, "unknown scheme");
-  if (sd->grid_type != 1) return fail(HP_E_UNSUPP, "the device path implements grid_type 1");
+  if (sd->grid_type != 1 && sd->grid_type != 2) return fail(HP_E_UNSUPP, "the device path implements grid types 1 and 2");
+  if (sd->grid_type == 2 && sd->check_mono) return fail(HP_E_UNSUPP, "check_mono is defined on radial grids (grid_type 1)");
   if (!sd->fn_offsets) return fail(HP_E_ARG, "fn_offsets is NULL");
   const bool mbis = sd->scheme == HP_SCHEME_MBIS;
   if (mbis && sd->solver != HP_SOLVER_MBIS) return fail(HP_E_ARG, "MBIS uses HP_SOLVER_MBIS");
@@ -412,6 +545,9 @@ int hp_part_create(int device, const hp_molecule_desc *md, const hp_scheme_desc 
     maxK = std::max(maxK, K);
   }
   if (mbis && maxK > 7) return fail(HP_E_UNSUPP, "MBIS supports at most 7 shells per atom");
+  if (sd->grid_type == 2 && maxK > HP_GT2_KM) return fail(HP_E_UNSUPP, "grid_type 2 supports at most 8 functions per atom");
+  if (sd->grid_type == 2 && sd->solver == HP_SOLVER_MNEWTON && sd->newton_linspace > HP_GT2_NLIN)
+    return fail(HP_E_UNSUPP, "grid_type 2 line search supports at most 32 candidates");
   if (maxK > HP_KMAX) return fail(HP_E_UNSUPP, "too many functions per atom");
   int fnkind = HP_FN_SLATER;
   if (!mbis) {
@@ -561,6 +697,12 @@ int hp_part_create(int device, const hp_molecule_desc *md, const hp_scheme_desc 
       (rc = dalloc(&p->hpar, (size_t)p->hist_cap * p->npar)) || (rc = dalloc(&p->hch, (size_t)p->hist_cap * A)) ||
       (rc = dalloc(&p->hent, p->hist_cap)) || (rc = dalloc(&p->hchg, p->hist_cap)))
     return bail(rc);
+  if (p->grid_type == 2) {
+    p->gt2_nchunk = (int)((N + HP_GT2_CHUNK - 1) / HP_GT2_CHUNK);
+    if ((rc = dalloc(&p->promol, N)) || (rc = dalloc(&p->gt2, A)) ||
+        (rc = dalloc(&p->gt2_partial, (size_t)A * p->gt2_nchunk * HP_GT2_NV)) || (rc = dalloc(&p->gt2_active, 1)))
+      return bail(rc);
+  }
   e = hipMemsetAsync(p->state, 0, sizeof(int) * 2, s);
   if (e == hipSuccess) e = hipMemsetAsync(p->status_all, 0, sizeof(int) * A, s);
   if (e == hipSuccess) e = hipMemsetAsync(p->status, 0, sizeof(int) * A, s);
@@ -603,18 +745,22 @@ int hp_part_run(hp_part *p, int precision, int n_polish, int *niter) {
   HP_HIP(hipMemsetAsync(p->state, 0, 2 * sizeof(int), p->stream));
   p->niter = 0;
   int rc = 0;
+  auto stage = [&](int prec, double thr, int min_stop, int max_iter) {
+    return p->grid_type == 2 ? run_stage_gt2(p, prec, thr, min_stop, max_iter)
+                             : run_stage(p, prec, thr, min_stop, max_iter);
+  };
   if (precision == HP_PREC_FP64) {
     // iterstock.py:172-188: stop when change < threshold or counter >= maxiter
-    rc = run_stage(p, HP_PREC_FP64, p->threshold, 1, p->maxiter);
+    rc = stage(HP_PREC_FP64, p->threshold, 1, p->maxiter);
   } else {
     // mixed stage until the change is below max(threshold, switch level),
     // then FP64 iterations (at least n_polish) until change < threshold
     const double thr1 = std::max(p->threshold, p->mixed_switch);
-    rc = run_stage(p, HP_PREC_MIXED, thr1, 1, p->maxiter);
+    rc = stage(HP_PREC_MIXED, thr1, 1, p->maxiter);
     if (!rc) {
       const int start = p->niter;
       const int np = std::max(1, n_polish);
-      rc = run_stage(p, HP_PREC_FP64, p->threshold, start + np, start + std::max(np, p->maxiter));
+      rc = stage(HP_PREC_FP64, p->threshold, start + np, start + std::max(np, p->maxiter));
     }
   }
   if (niter) *niter = p->niter;
@@ -661,6 +807,7 @@ int hp_part_status(hp_part *p, int32_t *inner_niter, int32_t *status_last, int32
 
 int hp_part_spherical_averages(hp_part *p, double *rhobar) {
   if (!p || !rhobar) return fail(HP_E_ARG, "NULL argument");
+  if (p->grid_type != 1) return fail(HP_E_UNSUPP, "spherical averages exist for grid_type 1 only");
   HP_HIP(hipSetDevice(p->device));
   HP_HIP(hipMemcpyAsync(rhobar, p->rhobar, sizeof(double) * p->nshell, hipMemcpyDeviceToHost, p->stream));
   HP_HIP(hipStreamSynchronize(p->stream));
@@ -694,9 +841,13 @@ int hp_part_at_weights(hp_part *p, int atom, double *out, int64_t n) {
   if (!p || !out) return fail(HP_E_ARG, "NULL argument");
   if (atom < 0 || atom >= p->natom) return fail(HP_E_ARG, "atom index out of range");
   if (p->niter == 0) return fail(HP_E_ARG, "no weights yet: run the partitioning first");
-  const int64_t p0 = p->h_shell_off[p->h_atom_shell_off[atom]];
-  const int64_t p1 = p->h_shell_off[p->h_atom_shell_off[atom + 1]];
-  if (n != p1 - p0) return fail(HP_E_ARG, "output length must equal the atom-grid size");
+  int64_t p0 = p->h_shell_off[p->h_atom_shell_off[atom]];
+  int64_t p1 = p->h_shell_off[p->h_atom_shell_off[atom + 1]];
+  if (p->grid_type == 2) {  // weights on the whole molecular grid
+    p0 = 0;
+    p1 = p->npoint;
+  }
+  if (n != p1 - p0) return fail(HP_E_ARG, "output length must equal the weight-array size");
   return point_weights(p, atom, p0, n, out);
 }
 

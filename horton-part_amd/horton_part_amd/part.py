@@ -373,8 +373,8 @@ class AbstractISAWPart(WPart):
 
     def _ensure_device(self):
         if self._dev is None:
-            if self.grid_type != 1:
-                raise NotImplementedError("the device path implements grid_type 1")
+            if self.grid_type not in (1, 2):
+                raise NotImplementedError("the device path implements grid types 1 and 2")
             self._dev = _native.DevicePartition(self._molecule_arrays(), self._scheme_arrays(), device=self._device)
         return self._dev
 
@@ -437,9 +437,11 @@ class AbstractISAWPart(WPart):
     def _register_lazy(self, dev):
         grid = self.grid
         for a in range(self.natom):
-            size = int(grid.indices[a + 1] - grid.indices[a])
+            size = int(grid.indices[a + 1] - grid.indices[a]) if not self.on_molgrid else int(grid.size)
             self.cache.add_lazy(f"at_weights_{a}", (lambda a=a, size=size: dev.at_weights(a, size)))
         self.cache.add_lazy("promoldens", dev.promoldens)
+        if self.on_molgrid:
+            return  # spherical averages exist on atom grids only (mbis.py:176-198)
         rhobar = dev.spherical_averages()
         s = 0
         for a, atgrid in enumerate(grid.atgrids):

@@ -162,7 +162,8 @@ int hp_part_spherical_averages(hp_part *part, double *rhobar);
 
 /* AIM weights of atom `atom` for the pro-atoms that produced the last update
  * ("at_weights_{a}" of stockholder.py:364-377): its own atom-grid points for
- * grid_type 1, all points for grid_type 2. */
+ * grid_type 1 (n = its atom-grid size), all points for grid_type 2 (n =
+ * npoint). */
 int hp_part_at_weights(hp_part *part, int atom, double *out, int64_t n);
 
 /* Promolecule density of the last update ("promoldens", stockholder.py:355). */

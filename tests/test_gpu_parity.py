@@ -122,9 +122,11 @@ def _slater6(numbers):
     return ExpBasisFuncHelper.uniform(numbers, alphas, order=1.0)
 
 
-def _lisa_pair(w, basis_name, solver, **kw):
+def _lisa_pair(w, basis_name, solver, precision="fp64", **kw):
     helper = _slater6(w.numbers) if basis_name == "slater6" else ExpBasisFuncHelper.from_function_type("gauss")
-    part = LinearISAWPart(w.coordinates, w.numbers, w.pseudo_numbers, w.grid, w.density, solver=solver, basis_func=helper, **kw)
+    part = LinearISAWPart(
+        w.coordinates, w.numbers, w.pseudo_numbers, w.grid, w.density, solver=solver, basis_func=helper, precision=precision, **kw
+    )
     basis = OracleBasis(helper.orders, helper.exponents, helper.initials)
     oracle = OracleLISA(w.coordinates, w.numbers, w.pseudo_numbers, w.grid, w.density, basis=basis, solver=solver, **kw)
     return part, oracle
@@ -194,3 +196,77 @@ def test_mbis_cutoff_radius():
     oracle = OracleMBIS(w.coordinates, w.numbers, w.pseudo_numbers, w.grid, w.density, **kw)
     cache = oracle.do_partitioning()
     _compare(part, cache, oracle)
+
+
+# --- grid_type 2: per-atom solvers over the molecular grid ---------------------
+
+
+def _run_pair(part, oracle):
+    """Run both; if the oracle raises, the device must raise the same type."""
+    try:
+        cache = oracle.do_partitioning()
+    except Exception as exc:  # noqa: BLE001 - compare exception types
+        with pytest.raises(type(exc)):
+            part.do_partitioning()
+        _say(f"  both raised {type(exc).__name__}")
+        return None
+    part.do_partitioning()
+    return cache
+
+
+@pytest.mark.parametrize(
+    "basis_name,solver",
+    [("gauss", "sc"), ("slater6", "sc"), ("gauss", "m-newton"), ("slater6", "m-newton"), ("gauss", "newton")],
+)
+def test_gt2_lisa(basis_name, solver):
+    """grid_type 2: LISA solves over the molecular grid; the Newton Hessian
+    B diag(w rho_a/pro^2) B^T accumulates on the matrix cores (FP64 MFMA)."""
+    require_gpu()
+    _say(f"case: grid_type 2 LISA {solver} {basis_name}, chain 3 atoms seed 5, 30x26")
+    w = workloads.chain(natom=3, seed=5, nrad=30, nang=26)
+    part, oracle = _lisa_pair(w, basis_name, solver, maxiter=60, grid_type=2)
+    cache = _run_pair(part, oracle)
+    if cache is not None:
+        _compare(part, cache, oracle, qtol=1e-10, ptol=1e-8)
+        for a in range(3):
+            np.testing.assert_allclose(part[f"at_weights_{a}"], oracle.cache[f"at_weights_{a}"], rtol=1e-11, atol=1e-14)
+
+
+def test_gt2_gisa():
+    require_gpu()
+    _say("case: grid_type 2 GISA, chain 3 atoms seed 11, 30x26")
+    w = workloads.chain(natom=3, seed=11, nrad=30, nang=26)
+    part = GaussianISAWPart(w.coordinates, w.numbers, w.pseudo_numbers, w.grid, w.density, maxiter=60, grid_type=2)
+    bs = part.bs_helper
+    basis = OracleBasis(bs.orders, bs.exponents, bs.initials)
+    oracle = OracleGISA(w.coordinates, w.numbers, w.pseudo_numbers, w.grid, w.density, maxiter=60, basis=basis, grid_type=2)
+    cache = _run_pair(part, oracle)
+    if cache is not None:
+        _compare(part, cache, oracle, check_inner=False)
+
+
+def test_gt2_mbis():
+    require_gpu()
+    _say("case: grid_type 2 MBIS, chain 3 atoms seed 4, 30x26")
+    w = workloads.chain(natom=3, seed=4, nrad=30, nang=26)
+    kw = dict(maxiter=25, grid_type=2)
+    part = MBISWPart(w.coordinates, w.numbers, w.pseudo_numbers, w.grid, w.density, **kw)
+    oracle = OracleMBIS(w.coordinates, w.numbers, w.pseudo_numbers, w.grid, w.density, **kw)
+    cache = _run_pair(part, oracle)
+    if cache is not None:
+        _compare(part, cache, oracle, qtol=1e-10, ptol=1e-8)
+
+
+def test_gt2_lisa_mixed_mfma_fp32():
+    """Mixed path on grid_type 2: FP32 pro-atoms and FP32 MFMA Hessian, FP64
+    solves, then FP64 polish; charges within 1e-6 e of the FP64 oracle."""
+    require_gpu()
+    _say("case: grid_type 2 LISA m-newton gauss mixed, chain 3 atoms seed 5, 30x26")
+    w = workloads.chain(natom=3, seed=5, nrad=30, nang=26)
+    part, oracle = _lisa_pair(w, "gauss", "m-newton", maxiter=60, grid_type=2, precision="mixed")
+    part.do_partitioning()
+    _, oracle = _lisa_pair(w, "gauss", "m-newton", maxiter=60, grid_type=2)
+    cache = oracle.do_partitioning()
+    dq = np.max(np.abs(part["charges"] - cache["charges"]))
+    _say(f"  mixed niter={part['niter']} oracle niter={cache['niter']} max|dq|={dq:.2e}")
+    assert dq <= 1e-6
