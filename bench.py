@@ -1,16 +1,21 @@
 """Benchmark of the stockholder partitioning loop on one MI355X.
 
-Default workload: MBIS on the 20-atom C/H/N/O chain (seed 1001) with
-150 radial x 302 Lebedev points per atom (906,000 points), threshold 1e-6.
-One step = one partition: reset the pro-atom parameters to their initial
-values, run the outer loop to convergence on the GPU (inputs already resident
-in HBM) and read back charges and parameters.  ``value`` is the mixed path's
-partitions/s (FP32 pro-atoms/promolecule/ratios, FP64 sums and solves, one
-FP64 polish iteration); the plain FP64 path is timed beside it, and
-max |dq| between the two is reported (the FP64 path equals the CPU oracle
-to 1e-10 e in the parity tests).
+Default workload (``--workload serving``): the serving stream -- MBIS/GISA
+requests of synthetic chain molecules with log-uniform atom counts in
+[3, 300], 100 radial x 194 Lebedev points per atom, threshold 1e-6, Poisson
+arrivals at 200 requests/s (seed 1004).  The first ``--steps`` requests
+(after ``--warmup`` untimed ones) are served by the continuous-batching
+engine (mixed precision with an FP64 polish) as they arrive; ``value`` is
+completed requests / (last completion - first arrival), with p50/p99
+latency (completion - arrival).  The same requests are then run one at a
+time on the plain FP64 GPU path, and max |dq| between the two is reported
+(the FP64 path equals the CPU oracle to 1e-10 e in the parity tests).
 
-Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload chain20]
+Single-molecule workloads (``--workload chain20|water|cluster200``): one
+step = one partition with inputs resident in HBM; ``value`` is the mixed
+path's partitions/s with the plain FP64 path timed beside it.
+
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload serving]
 """
 
 import argparse
@@ -110,12 +115,263 @@ def cpu_baseline(w, threshold, gpu_niter, budget_s=20.0):
     }
 
 
+# ---------------------------------------------------------------------------
+# serving stream
+# ---------------------------------------------------------------------------
+SERVING_SEED = 1004
+SERVING_THRESHOLD = 1e-6
+SERVING_MAXITER = 1000
+SERVING_GRID = (100, 194)
+
+
+def _scheme_part(r, w):
+    from horton_part_amd import GaussianISAWPart, MBISWPart
+
+    cls = MBISWPart if r["scheme"] == "mbis" else GaussianISAWPart
+    return cls(w.coordinates, w.numbers, w.pseudo_numbers, w.grid, w.density, threshold=SERVING_THRESHOLD, maxiter=SERVING_MAXITER)
+
+
+def _serving_molecule(r, device):
+    from horton_part_amd import workloads
+
+    return workloads.chain(natom=r["natom"], seed=r["mol_seed"], nrad=SERVING_GRID[0], nang=SERVING_GRID[1], device=device)
+
+
+def build_requests(reqs, device):
+    """Molecule/scheme arrays of each request, bulk arrays in pinned memory."""
+    from horton_part_amd import _native
+    from horton_part_amd.serving import request_arrays
+
+    out = []
+    for r in reqs:
+        w = _serving_molecule(r, device)
+        mol, sch, p0 = request_arrays(_scheme_part(r, w))
+        for k in ("points", "density", "mol_weights", "sph_weights"):
+            mol[k] = _native.pinned_copy(mol[k])
+        out.append({"req": r, "mol": mol, "sch": sch, "p0": p0})
+    return out
+
+
+def serve(server, items, realtime=True):
+    """Submit each item at its arrival time (relative to the first) and step
+    the server until all are done.  Returns per-item (arrival, completion,
+    Completed) in seconds from the first arrival."""
+    t_first = items[0]["req"]["t"]
+    arrival = [it["req"]["t"] - t_first for it in items]
+    done = {}
+    i = 0
+    t0 = time.perf_counter()
+    while len(done) < len(items):
+        now = time.perf_counter() - t0
+        while i < len(items) and (not realtime or arrival[i] <= now):
+            it = items[i]
+            server.submit_arrays(it["mol"], it["sch"], it["p0"], precision="mixed", n_polish=1, tag=i)
+            i += 1
+        if server.pending == 0:
+            time.sleep(max(0.0, arrival[i] - (time.perf_counter() - t0)))
+            continue
+        server.step()
+        for d in server.poll():
+            done[d.tag] = (arrival[d.tag], time.perf_counter() - t0, d)
+    return [done[k] for k in range(len(items))]
+
+
+def fp64_one_at_a_time(items, device):
+    """The plain FP64 GPU path, one request after another (hp_part_run)."""
+    from horton_part_amd import _native
+
+    out = []
+    for it in items:
+        t0 = time.perf_counter()
+        dev = _native.DevicePartition(it["mol"], it["sch"], device=device)
+        dev.set_propars(it["p0"])
+        t1 = time.perf_counter()
+        rc = dev.run("fp64", 1)
+        propars, charges, change, _ = dev.results()
+        t2 = time.perf_counter()
+        out.append({"create_s": t1 - t0, "run_s": t2 - t1, "niter": dev.niter, "rc": rc, "charges": charges})
+        dev.close()
+    return out
+
+
+def serving_cpu_baseline(items, fp64, device, budget_s=15.0):
+    """The numpy oracle (reference structure, 1 thread) timed on one request
+    of the stream for a bounded number of outer iterations; the stream's CPU
+    time is extrapolated with a cost per point x atom x iteration."""
+    import warnings
+
+    from oracle.horton_oracle import OracleBasis, OracleGISA, OracleMBIS
+
+    warnings.simplefilter("ignore")
+    # a small request, so that a few iterations fit the budget
+    j = min(range(len(items)), key=lambda k: abs(items[k]["req"]["natom"] - 10))
+    r = items[j]["req"]
+    w = _serving_molecule(r, device)
+    kw = {}
+    if r["scheme"] == "gisa":
+        bs = _scheme_part(r, w).bs_helper
+        kw["basis"] = OracleBasis(bs.orders, bs.exponents, bs.initials)
+    ocls = OracleMBIS if r["scheme"] == "mbis" else OracleGISA
+    n, used = 1, 0.0
+    while True:
+        o = ocls(w.coordinates, w.numbers, w.pseudo_numbers, w.grid, w.density, threshold=SERVING_THRESHOLD, maxiter=n, **kw)
+        t0 = time.perf_counter()
+        o.do_partitioning()
+        dt = time.perf_counter() - t0
+        used += dt
+        if used + 2 * dt > budget_s or n >= fp64[j]["niter"]:
+            break
+        n *= 2
+    per_iter = dt / n
+    c = per_iter / (w.npoint * w.natom)
+    total = sum(c * it["mol"]["npoint"] * it["mol"]["natom"] * f["niter"] for it, f in zip(items, fp64))
+    return {
+        "value": len(items) / total,
+        "unit": "partitions/s",
+        "cores": 1,
+        "kind": "port",
+        "sample": f"{n} outer iteration(s) of the numpy oracle on request {j} ({r['scheme']}, {r['natom']} atoms, "
+        f"{w.npoint} points): {per_iter:.3f} s/iteration = {c * 1e12:.1f} ps per point x atom; extrapolated to the "
+        f"{len(items)} timed requests with their FP64 iteration counts (no queueing: partitions / summed CPU time)",
+        "host_cpus": len(os.sched_getaffinity(0)),
+        "cpu_model": platform.processor() or platform.machine(),
+    }
+
+
+def run_serving(args):
+    from horton_part_amd import PartitionServer, _native, workloads
+
+    nreq = args.warmup + args.steps
+    stream = workloads.serving_stream(seed=SERVING_SEED)[:nreq]
+    log(f"building {nreq} requests of the serving stream (seed {SERVING_SEED}) on the GPU")
+    t0 = time.perf_counter()
+    items = build_requests(stream, device=0)
+    timed = items[args.warmup :]
+    npts = [it["mol"]["npoint"] for it in items]
+    log(f"  {sum(npts)} points in all, largest {max(npts)}, built in {time.perf_counter() - t0:.1f} s")
+    cap_pts = int(sum(npts) + 1)
+    cap_atoms = int(sum(it["mol"]["natom"] + 1 for it in items))
+    cap_shells = int(sum(it["mol"]["nshell"] + 1 for it in items))
+    cap_fns = int(sum(int(it["sch"]["fn_offsets"][-1]) for it in items))
+    cap_qp = int(sum(it["sch"]["qp_matrix"].size for it in items if "qp_matrix" in it["sch"])) + 1
+    server = PartitionServer(
+        device=0,
+        point_capacity=cap_pts,
+        atom_capacity=cap_atoms,
+        shell_capacity=cap_shells,
+        function_capacity=cap_fns,
+        qp_capacity=cap_qp,
+        max_inflight=max(8, nreq),
+        max_molecule_points=max(npts),
+        ticks_per_step=args.ticks,
+        use_graphs=args.graphs,
+    )
+    if args.warmup:
+        log(f"warmup: {args.warmup} request(s) served together, untimed")
+        serve(server, items[: args.warmup], realtime=False)
+    st0 = server.stats()
+    log(f"serving {len(timed)} requests as they arrive (mixed precision + FP64 polish)")
+    res = serve(server, timed)
+    st1 = server.stats()
+    server.close()
+    arrival = np.array([a for a, _, _ in res])
+    complete = np.array([c for _, c, _ in res])
+    lat = complete - arrival
+    makespan = complete.max() - arrival.min()
+    value = len(timed) / makespan
+    log(f"  {value:.3f} partitions/s, makespan {makespan:.2f} s, p50 {np.percentile(lat, 50):.2f} s, p99 {np.percentile(lat, 99):.2f} s")
+    events = {}
+    for _, _, d in res:
+        events[d.event] = events.get(d.event, 0) + 1
+
+    log("plain FP64 path, one request at a time")
+    fp64 = fp64_one_at_a_time(timed, device=0)
+    run_s = np.array([f["run_s"] for f in fp64])
+    create_s = np.array([f["create_s"] for f in fp64])
+    fp64_value = len(timed) / run_s.sum()
+    fp64_e2e = len(timed) / (run_s.sum() + create_s.sum())
+    dq = max(float(np.max(np.abs(d.charges - f["charges"]))) for (_, _, d), f in zip(res, fp64))
+    log(f"  {fp64_value:.3f} partitions/s (run only), {fp64_e2e:.3f} with upload; max|dq| mixed vs FP64 {dq:.2e}")
+
+    d = {k: st1[k] - st0[k] for k in st1}
+    pts = d["points_fp32"] + d["points_fp64"]
+    t_w = d["weights_ms"] * 1e-3
+    ticks = max(d["ticks"], 1)
+    bytes_per_tick = 48.0 * pts / ticks
+    t_tick_w = t_w / ticks
+    achieved = bytes_per_tick / t_tick_w / 1e9 if t_tick_w > 0 else 0.0
+    out = {
+        "metric": METRIC,
+        "value": value,
+        "unit": "partitions/s",
+        "n_gpus": 1,
+        "steps": len(timed),
+        "warmup": args.warmup,
+        "ms_per_step": 1e3 / value,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32+f64",
+        "data": "synthetic",
+        "config": {
+            "workload": f"serving stream (seed {SERVING_SEED}): requests {args.warmup}..{nreq - 1} of 4096, MBIS/GISA "
+            f"chains with 3-300 atoms, {SERVING_GRID[0]}x{SERVING_GRID[1]} grids, Poisson arrivals at 200/s, "
+            f"threshold {SERVING_THRESHOLD:g}; continuous batching, mixed precision + 1 FP64 polish iteration",
+            "requests": len(timed),
+            "atoms": int(sum(it["req"]["natom"] for it in timed)),
+            "points": int(sum(it["mol"]["npoint"] for it in timed)),
+            "ticks_per_step": args.ticks,
+            "graphs": args.graphs,
+        },
+        "p50_ms": float(np.percentile(lat, 50)) * 1e3,
+        "p99_ms": float(np.percentile(lat, 99)) * 1e3,
+        "makespan_s": makespan,
+        "events": events,
+        "fp64_one_at_a_time": {
+            "value": fp64_value,
+            "value_with_upload": fp64_e2e,
+            "ms_per_request": float(run_s.mean()) * 1e3,
+            "iterations": int(sum(f["niter"] for f in fp64)),
+            "errors": int(sum(f["rc"] != 0 for f in fp64)),
+        },
+        "speedup_vs_fp64": value / fp64_value,
+        "max_dq_vs_fp64": dq,
+        "batch": {
+            "ticks": d["ticks"],
+            "molecule_iterations": d["iterations"],
+            "tick_ms_mean": d["tick_ms"] / ticks,
+            "weights_ms_mean": d["weights_ms"] / ticks,
+            "inner_ms_mean": d["inner_ms"] / ticks,
+            "gpu_busy_frac": d["tick_ms"] * 1e-3 / makespan,
+        },
+        "roofline": {
+            "kernel": "k_weights (fused pro-atom/promolecule/weights/spherical average), FP32 + FP64 launches of a tick",
+            "bound": "hbm",
+            "achieved": achieved,
+            "peak": HBM_PEAK_GBPS,
+            "unit": "GB/s",
+            "frac": achieved / HBM_PEAK_GBPS,
+            "traffic": None,
+            "bytes_per_launch": bytes_per_tick,
+            "launch_ms": t_tick_w * 1e3,
+            "evals_per_s": (d["evals_fp32"] + d["evals_fp64"]) / t_w if t_w > 0 else 0.0,
+            "fp32_point_frac": d["points_fp32"] / pts if pts else 0.0,
+        },
+    }
+    if not args.no_cpu:
+        log("CPU baseline (numpy oracle, 1 thread, bounded sample)")
+        out["cpu_baseline"] = serving_cpu_baseline(timed, fp64, device=0)
+    print(json.dumps(out))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=5)
-    ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--workload", default="chain20")
+    ap.add_argument("--steps", type=int, default=None, help="timed requests (serving) or partitions")
+    ap.add_argument("--warmup", type=int, default=None)
+    ap.add_argument("--workload", default="serving", choices=["serving", "chain20", "water", "cluster200"])
+    ap.add_argument("--ticks", type=int, default=4, help="serving: outer iterations per host step")
+    ap.add_argument("--graphs", action="store_true", help="serving: replay the ticks of a step as a hipGraph (times whole steps only)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -126,6 +382,12 @@ def main():
 
     if _native.device_count() < 1:
         raise SystemExit("no GPU visible")
+    if args.workload == "serving":
+        args.steps = 48 if args.steps is None else args.steps
+        args.warmup = 2 if args.warmup is None else args.warmup
+        return run_serving(args)
+    args.steps = 5 if args.steps is None else args.steps
+    args.warmup = 1 if args.warmup is None else args.warmup
     log(f"building workload {args.workload} on the GPU")
     t0 = time.perf_counter()
     w, scheme, threshold = build_workload(args.workload, device=0)

@@ -17,36 +17,15 @@
 #include "hortonpart.h"
 #include "hp_device.h"
 #include "hp_gt2.h"
+#include "hp_host.h"
 
 namespace {
 
-thread_local std::string g_err;
-
-int fail(int code, const std::string &msg) {
-  g_err = msg;
-  return code;
-}
-
-#define HP_HIP(expr)                                                                        \
-  do {                                                                                      \
-    hipError_t e_ = (expr);                                                                 \
-    if (e_ != hipSuccess) return fail(HP_E_HIP, std::string(#expr ": ") + hipGetErrorString(e_)); \
-  } while (0)
-
-#define HP_LAUNCHED()                                                                       \
-  do {                                                                                      \
-    hipError_t e_ = hipGetLastError();                                                      \
-    if (e_ != hipSuccess) return fail(HP_E_HIP, std::string("kernel launch: ") + hipGetErrorString(e_)); \
-  } while (0)
-
-template <typename T>
-int dalloc(T **p, size_t n) {
-  *p = nullptr;
-  if (n == 0) n = 1;
-  hipError_t e = hipMalloc((void **)p, n * sizeof(T));
-  if (e != hipSuccess) return fail(HP_E_HIP, std::string("hipMalloc: ") + hipGetErrorString(e));
-  return 0;
-}
+using hp::dalloc;
+using hp::fail;
+using hp::g_err;
+using hp::pick_ppl;
+using hp::weights_pt;
 
 template <typename T>
 int dupload(T **p, const T *host, size_t n, hipStream_t s) {
@@ -126,6 +105,8 @@ struct hp_part {
     m.shell_w = shell_w;
     m.atom_shell_off = atom_shell_off;
     m.pseudo = pseudo;
+    m.work = nullptr;
+    m.slots = nullptr;
     return m;
   }
   DevFn fn() const {
@@ -136,6 +117,7 @@ struct hp_part {
     f.order = fn_order;
     f.nb_list = rc > 0.0 ? nb_list : nullptr;
     f.shell_nb = shell_nb;
+    f.nb_off = nullptr;
     f.rc = rc;
     return f;
   }
@@ -169,23 +151,7 @@ void free_part(hp_part *p) {
   delete p;
 }
 
-int pick_ppl(int R) {
-  if (R <= 128) return 2;
-  if (R <= 256) return 4;
-  if (R <= 512) return 8;
-  return 0;
-}
-
 // ---- kernel launch helpers -------------------------------------------------
-
-// Points per thread of the weights pass: 1 or 2, whichever idles fewer lanes
-// for the largest shell (ties go to 2, which halves the LDS reads per point).
-inline int weights_pt(int maxL) {
-  auto lanes = [](int n) { return ((n + 63) / 64) * 64; };
-  const double e1 = (double)maxL / lanes(maxL);
-  const double e2 = (double)maxL / (2.0 * lanes((maxL + 1) / 2));
-  return (maxL > 512 || e2 >= e1) ? 2 : 1;
-}
 
 template <typename R, int FK, bool GT2, int PT>
 void launch_weights_pt(hp_part *p, int bs, size_t lds, int tile, int kslot) {
@@ -197,8 +163,7 @@ template <typename R, int FK>
 void launch_weights_fk(hp_part *p, bool gt2) {
   // one workgroup per radial shell, each thread one or two points
   const int pt = weights_pt(p->maxL);
-  int bs = ((p->maxL + pt - 1) / pt + 63) / 64 * 64;
-  bs = std::max(64, std::min(512, bs));
+  const int bs = hp::weights_block(p->maxL);
   const int kslot = std::max(1, p->maxK);
   const size_t per_atom = weights_lds_per_atom<R>(kslot);
   int tile = (int)std::min<size_t>(256, (40 * 1024) / per_atom);
@@ -736,6 +701,9 @@ int hp_part_set_propars(hp_part *p, const double *propars, int64_t n) {
 int hp_part_run(hp_part *p, int precision, int n_polish, int *niter) {
   if (!p) return fail(HP_E_ARG, "NULL part");
   if (precision != HP_PREC_FP64 && precision != HP_PREC_MIXED) return fail(HP_E_ARG, "unknown precision");
+  // the histories hold 2*maxiter + 64 iterations
+  if (precision == HP_PREC_MIXED && (n_polish < 0 || n_polish > p->maxiter + 64))
+    return fail(HP_E_ARG, "n_polish must be in [0, maxiter + 64]");
   HP_HIP(hipSetDevice(p->device));
   for (int k = 0; k < 3; ++k) {
     p->k_ms[k] = 0.0;
@@ -754,13 +722,14 @@ int hp_part_run(hp_part *p, int precision, int n_polish, int *niter) {
     rc = stage(HP_PREC_FP64, p->threshold, 1, p->maxiter);
   } else {
     // mixed stage until the change is below max(threshold, switch level),
-    // then FP64 iterations (at least n_polish) until change < threshold
+    // then FP64 iterations (at least n_polish) until change < threshold;
+    // maxiter caps the total unless the polish needs more
     const double thr1 = std::max(p->threshold, p->mixed_switch);
     rc = stage(HP_PREC_MIXED, thr1, 1, p->maxiter);
     if (!rc) {
       const int start = p->niter;
       const int np = std::max(1, n_polish);
-      rc = stage(HP_PREC_FP64, p->threshold, start + np, start + std::max(np, p->maxiter));
+      rc = stage(HP_PREC_FP64, p->threshold, start + np, std::max(p->maxiter, start + np));
     }
   }
   if (niter) *niter = p->niter;

@@ -1,7 +1,7 @@
 // Device code of the stockholder-partitioning library (gfx950, wave64).
 //
 // Kernels
-//   k_build_fn        pro-atom function table (coefficient, exponent) from the
+//   k_iter_begin      pro-atom function table (coefficient, exponent) from the
 //                     current parameters (mbis.py:284-289, basis.py:161-171)
 //   k_weights<R,F,G>  fused pass over one radial shell per workgroup:
 //                     pro-atoms of every (or every neighbouring) atom,
@@ -9,12 +9,14 @@
 //                     clipped to [0,1], spherical average of w*rho and the
 //                     entropy partial (stockholder.py:145-175, 352-384;
 //                     mbis.py:176-184).  R = float gives the mixed path.
-//   k_mbis_inner      one wavefront per atom: MBIS fixed point (mbis.py:81-163)
+//   k_mbis_inner      one workgroup per atom: MBIS fixed point (mbis.py:81-163)
 //   k_gisa_qp         one wavefront per atom: QP over the simplex (gisa.py:348-421)
 //   k_lisa_inner      one wavefront per atom: sc / newton / m-newton on the
 //                     radial grid (alisa.py:193-291, 753-938)
 //   k_finalize        entropy and change sums, histories (iterstock.py:32-58)
 //   k_moments*        AIM multipoles (core/base.py:329-402)
+//   k_batch_*         the outer-loop bookkeeping of many molecules at once
+//                     (one slot record per molecule in flight)
 #pragma once
 #include <hip/hip_runtime.h>
 #include <stdint.h>
@@ -33,6 +35,40 @@
 #define HP_ST_SINGULAR 256
 #define HP_ST_NONFINITE 512
 #define HP_PI 3.14159265358979323846
+// status bits that end the outer loop (the reference raises on them)
+#define HP_ST_FATAL                                                                                         \
+  (HP_ST_NEGDENS | HP_ST_LINESEARCH | HP_ST_NEWTONMAX | HP_ST_QPFAIL | HP_ST_NOTMONO | HP_ST_SINGULAR | \
+   HP_ST_NONFINITE)
+
+// Stage of a molecule in flight (HpSlot::run)
+#define HP_RUN_IDLE 0  // finished, failed, or waiting for the host
+#define HP_RUN_FP32 1  // mixed precision: FP32 grid arithmetic
+#define HP_RUN_FP64 2  // plain FP64 (or the FP64 polish of the mixed path)
+
+// Why a molecule stopped (HpSlot::event)
+#define HP_EV_RUNNING 0
+#define HP_EV_CONVERGED 1  // change < threshold
+#define HP_EV_MAXITER 2    // iteration cap reached first
+#define HP_EV_ERROR 3      // a per-atom solver raised
+#define HP_EV_SWITCH 4     // FP32 stage done: the host moves it to FP64
+
+// Outer-loop state of one molecule in a batch, owned by the device between
+// host polls.  The host fills it at admission and flips run from IDLE to
+// FP64 after an HP_EV_SWITCH.
+struct HpSlot {
+  int run, event;
+  int scheme;          // 0: MBIS (N, S pairs), 1: linear coefficients
+  int a0, na;          // atoms [a0, a0 + na) of the batch arrays
+  int s0, ns;          // radial shells
+  int f0, nf;          // pro-atom functions
+  int niter;           // outer iterations done
+  int min_stop, max_iter;  // bounds of the current stage
+  int n_polish, maxiter;   // the request's settings
+  int err, any;        // status bits: fatal ones of the last iteration, OR of all
+  double thr;          // threshold of the current stage
+  double threshold;    // the request's threshold
+  double change, entropy;
+};
 
 struct DevMol {
   int natom;
@@ -46,6 +82,11 @@ struct DevMol {
   const double *shell_r, *shell_w;
   const int *atom_shell_off;         // natom+1 radial-shell offsets per atom
   const double *pseudo;
+  // Batched form (many molecules in one launch; both null for one molecule):
+  // work[i] = (shell or atom index, slot of its molecule) for block/wave i,
+  // and natom then counts the entries of an atom work list.
+  const int2 *work;
+  const HpSlot *slots;
 };
 
 struct DevFn {
@@ -55,6 +96,7 @@ struct DevFn {
   const double *order;   // per function: power of r
   const int *nb_list;    // natom*natom neighbour lists sorted by distance (culling)
   const int *shell_nb;   // nshell: candidate count of each shell (culling)
+  const int64_t *nb_off; // per atom: start of its list (null: atom * natom)
   double rc;             // cutoff radius; <= 0 disables culling
 };
 
@@ -192,7 +234,7 @@ __device__ __forceinline__ float hp_floor_add<float>() { return 0.0f; }
 // iteration's weights (old_par) and build the pro-atom function table.
 // MBIS: par[2f] = N, par[2f+1] = S -> coef = N S^3 / (8 pi), alpha = S.
 // Linear bases: coef = c_f * norm_f, alpha fixed.
-__global__ void k_iter_begin(const int *__restrict__ done, int scheme, int npar, int nfn, const double *__restrict__ par,
+static __global__ void k_iter_begin(const int *__restrict__ done, int scheme, int npar, int nfn, const double *__restrict__ par,
                              double *__restrict__ old_par, const double *__restrict__ norm,
                              const double *__restrict__ alpha_fixed, double *__restrict__ coef,
                              double *__restrict__ alpha) {
@@ -290,13 +332,23 @@ __global__ __launch_bounds__(512) void k_weights(const int *__restrict__ done, D
   int *s_idx = reinterpret_cast<int *>(s_alpha + (size_t)tile * kslot);
   int *s_nf = s_idx + tile;
 
-  const int s = blockIdx.x;
+  int s = blockIdx.x;
+  int a_begin = 0, a_end = m.natom;  // atoms of this shell's molecule
+  if (m.work) {
+    const int2 wk = m.work[blockIdx.x];
+    const HpSlot &sl = m.slots[wk.y];
+    // a molecule takes part only in the launches of its current stage
+    if (sl.run != (sizeof(R) == sizeof(double) ? HP_RUN_FP64 : HP_RUN_FP32)) return;
+    s = wk.x;
+    a_begin = sl.a0;
+    a_end = sl.a0 + sl.na;
+  }
   const int a = m.shell_atom[s];
   const int64_t p0 = m.shell_off[s], p1 = m.shell_off[s + 1];
-  int nb_n = m.natom;
+  int nb_n = a_end - a_begin;
   const int *nb = nullptr;
   if (fn.rc > 0.0) {
-    nb = fn.nb_list + (int64_t)a * m.natom;
+    nb = fn.nb_list + (fn.nb_off ? fn.nb_off[a] : (int64_t)a * m.natom);
     nb_n = fn.shell_nb[s];
   }
   const bool cull = fn.rc > 0.0;
@@ -324,7 +376,7 @@ __global__ __launch_bounds__(512) void k_weights(const int *__restrict__ done, D
     const int tn = min(tile, nb_n - t0);
     __syncthreads();
     for (int i = threadIdx.x; i < tn; i += blockDim.x) {
-      const int b = nb ? nb[t0 + i] : t0 + i;
+      const int b = nb ? nb[t0 + i] : a_begin + t0 + i;
       s_idx[i] = b;
       if (sizeof(R) == sizeof(double)) {
         s_cx[i] = (R)m.cx[b];
@@ -391,7 +443,7 @@ __global__ __launch_bounds__(512) void k_weights(const int *__restrict__ done, D
     const int64_t p = pidx[j];
     R pm_r = promol[j];
     // culled atoms still contribute their 1e-100 floor
-    if (nb) pm_r += (R)(m.natom - nb_n) * hp_floor_add<R>();
+    if (nb) pm_r += (R)(a_end - a_begin - nb_n) * hp_floor_add<R>();
     const double pm = (double)pm_r;
     const double rho = m.rho[p];
     if (!GT2) {
@@ -496,7 +548,12 @@ __global__ __launch_bounds__(512) void k_mbis_inner(const int *__restrict__ done
   constexpr int NV = 2 * KM + 1;
   __shared__ double lds[2 * 16 * NV];
   if (done && *done) return;
-  const int a = blockIdx.x;
+  int a = blockIdx.x;
+  if (m.work) {
+    const int2 wk = m.work[blockIdx.x];
+    if (m.slots[wk.y].run == HP_RUN_IDLE) return;
+    a = wk.x;
+  }
   const int s0 = m.atom_shell_off[a], R = m.atom_shell_off[a + 1] - s0;
   const int f0 = fn_off[a], K = fn_off[a + 1] - f0;
   const int i = threadIdx.x;
@@ -740,8 +797,14 @@ __global__ __launch_bounds__(256) void k_gisa_qp(const int *__restrict__ done, D
   __shared__ double Mlds[4][(HP_KMAX + 1) * (HP_KMAX + 2)];
   if (done && *done) return;
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-  const int a = blockIdx.x * (blockDim.x >> 6) + wv;
-  if (a >= m.natom) return;
+  const int ai = blockIdx.x * (blockDim.x >> 6) + wv;
+  if (ai >= m.natom) return;
+  int a = ai;
+  if (m.work) {
+    const int2 wk = m.work[ai];
+    if (m.slots[wk.y].run == HP_RUN_IDLE) return;
+    a = wk.x;
+  }
   const int s0 = m.atom_shell_off[a], R = m.atom_shell_off[a + 1] - s0;
   const int f0 = fn_off[a], K = fn_off[a + 1] - f0;
   RadialLane<PPL> d;
@@ -839,8 +902,14 @@ __global__ __launch_bounds__(256) void k_lisa_inner(const int *__restrict__ done
   __shared__ double Mlds[4][(HP_KMAX + 1) * (HP_KMAX + 2)];
   if (done && *done) return;
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-  const int a = blockIdx.x * (blockDim.x >> 6) + wv;
-  if (a >= m.natom) return;
+  const int ai = blockIdx.x * (blockDim.x >> 6) + wv;
+  if (ai >= m.natom) return;
+  int a = ai;
+  if (m.work) {
+    const int2 wk = m.work[ai];
+    if (m.slots[wk.y].run == HP_RUN_IDLE) return;
+    a = wk.x;
+  }
   const int s0 = m.atom_shell_off[a], R = m.atom_shell_off[a + 1] - s0;
   const int f0 = fn_off[a], K = fn_off[a + 1] - f0;
   RadialLane<PPL> d;
@@ -1072,7 +1141,7 @@ __global__ __launch_bounds__(256) void k_lisa_inner(const int *__restrict__ done
 // stop when (iteration >= min_stop and change < thr) or iteration >= max_iter
 // or a per-atom solver raised.  state[0] = done, state[1] = iterations so far.
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(256) void k_finalize(int *__restrict__ state, double thr, int min_stop, int max_iter,
+static __global__ __launch_bounds__(256) void k_finalize(int *__restrict__ state, double thr, int min_stop, int max_iter,
                                                   int nshell, const double *__restrict__ ent_part, int natom,
                                                   const double *__restrict__ change_part, const int *__restrict__ status,
                                                   int *__restrict__ status_all, const double *__restrict__ par, int npar,
@@ -1096,8 +1165,7 @@ __global__ __launch_bounds__(256) void k_finalize(int *__restrict__ state, doubl
     int err = 0, any = 0;
     for (int a = 0; a < natom; ++a) {
       msd += change_part[a];
-      err |= status[a] & (HP_ST_NEGDENS | HP_ST_LINESEARCH | HP_ST_NEWTONMAX | HP_ST_QPFAIL | HP_ST_NOTMONO |
-                          HP_ST_SINGULAR | HP_ST_NONFINITE);
+      err |= status[a] & HP_ST_FATAL;
       any |= status[a];
     }
     const double ch = sqrt(msd);
@@ -1180,7 +1248,7 @@ __global__ __launch_bounds__(256) void k_moments(DevMol m, DevFn fn, double *__r
   }
 }
 
-__global__ void k_moments_reduce(DevMol m, const double *__restrict__ part, double *__restrict__ out) {
+static __global__ void k_moments_reduce(DevMol m, const double *__restrict__ part, double *__restrict__ out) {
   const int a = blockIdx.x;
   const int i = threadIdx.x;
   if (i >= HP_NMOM) return;
@@ -1196,7 +1264,7 @@ __global__ void k_moments_reduce(DevMol m, const double *__restrict__ part, doub
 // Becke fuzzy-cell weight of each point for the atom whose grid holds it
 // (A. D. Becke, J. Chem. Phys. 88 (1988) 2547; size-adjusted, order-3 step).
 // adj[i*A+j] = a_ij, invr[i*A+j] = 1/|R_i - R_j|.
-__global__ void k_becke(int natom, const double *__restrict__ cx, const double *__restrict__ cy,
+static __global__ void k_becke(int natom, const double *__restrict__ cx, const double *__restrict__ cy,
                         const double *__restrict__ cz, const double *__restrict__ adj, const double *__restrict__ invr,
                         int64_t npoint, const double *__restrict__ px, const double *__restrict__ py,
                         const double *__restrict__ pz, const int64_t *__restrict__ atom_off, int order,
@@ -1233,7 +1301,7 @@ __global__ void k_becke(int natom, const double *__restrict__ cx, const double *
 }
 
 // sum_a sum_k coef_ak exp(-alpha_ak |p - R_a|) at every point.
-__global__ void k_slater_sum(int natom, const double *__restrict__ cx, const double *__restrict__ cy,
+static __global__ void k_slater_sum(int natom, const double *__restrict__ cx, const double *__restrict__ cy,
                              const double *__restrict__ cz, const int *__restrict__ fn_off,
                              const double *__restrict__ coef, const double *__restrict__ alpha, int64_t npoint,
                              const double *__restrict__ px, const double *__restrict__ py,
@@ -1248,4 +1316,162 @@ __global__ void k_slater_sum(int natom, const double *__restrict__ cx, const dou
     for (int f = fn_off[a]; f < fn_off[a + 1]; ++f) acc += coef[f] * exp(-alpha[f] * r);
   }
   out[p] = acc;
+}
+
+// ---------------------------------------------------------------------------
+// Batched outer loop: many molecules resident at once, each with its own
+// HpSlot.  One tick = k_batch_begin, the weights launches, the per-atom
+// launches and k_batch_finalize over the slots in flight; a molecule whose
+// run is HP_RUN_IDLE is skipped by every kernel.
+// ---------------------------------------------------------------------------
+
+// k_iter_begin for every molecule in flight: one workgroup per slot.
+static __global__ __launch_bounds__(256) void k_batch_begin(const HpSlot *__restrict__ slots, const int *__restrict__ slot_list,
+                                                     const double *__restrict__ par_m, double *__restrict__ old_m,
+                                                     const double *__restrict__ par_l, double *__restrict__ old_l,
+                                                     const double *__restrict__ norm,
+                                                     const double *__restrict__ alpha_fixed, double *__restrict__ coef,
+                                                     double *__restrict__ alpha) {
+  const HpSlot &sl = slots[slot_list[blockIdx.x]];
+  if (sl.run == HP_RUN_IDLE) return;
+  for (int i = threadIdx.x; i < sl.nf; i += blockDim.x) {
+    const int f = sl.f0 + i;
+    if (sl.scheme == 0) {
+      const double N = par_m[2 * f], S = par_m[2 * f + 1];
+      old_m[2 * f] = N;
+      old_m[2 * f + 1] = S;
+      coef[f] = N * (S * S * S) / (8.0 * HP_PI);
+      alpha[f] = S;
+    } else {
+      old_l[f] = par_l[f];
+      coef[f] = par_l[f] * norm[f];
+      alpha[f] = alpha_fixed[f];
+    }
+  }
+}
+
+// k_finalize for every molecule in flight, plus the stage machine of the
+// mixed path: FP32 until change < max(threshold, switch level), then FP64
+// (at least n_polish iterations) until change < threshold; maxiter caps the
+// total unless the polish needs more.
+static __global__ __launch_bounds__(256) void k_batch_finalize(HpSlot *__restrict__ slots, const int *__restrict__ slot_list,
+                                                        const double *__restrict__ ent_part,
+                                                        const double *__restrict__ change_part,
+                                                        const int *__restrict__ status, int *__restrict__ status_all) {
+  __shared__ double lds[32];
+  HpSlot &sl = slots[slot_list[blockIdx.x]];
+  if (sl.run == HP_RUN_IDLE) return;
+  const int s0 = sl.s0, ns = sl.ns, a0 = sl.a0, na = sl.na;
+  double e = 0.0, dummy = 0.0;
+  for (int s = threadIdx.x; s < ns; s += blockDim.x) e += ent_part[s0 + s];
+  block_sum2(e, dummy, lds);
+  for (int i = threadIdx.x; i < na; i += blockDim.x) status_all[a0 + i] |= status[a0 + i];
+  if (threadIdx.x == 0) {
+    double msd = 0.0;
+    int err = 0, any = 0;
+    for (int a = 0; a < na; ++a) {
+      msd += change_part[a0 + a];
+      err |= status[a0 + a] & HP_ST_FATAL;
+      any |= status[a0 + a];
+    }
+    const double ch = sqrt(msd);
+    const int n = sl.niter + 1;
+    sl.niter = n;
+    sl.change = ch;
+    sl.entropy = e;
+    sl.err = err;
+    sl.any |= any;
+    if (err != 0) {
+      sl.event = HP_EV_ERROR;
+      sl.run = HP_RUN_IDLE;
+    } else if (n >= sl.max_iter || (n >= sl.min_stop && ch < sl.thr)) {
+      if (sl.run == HP_RUN_FP32) {
+        const int np = max(1, sl.n_polish);
+        sl.min_stop = n + np;
+        sl.max_iter = max(sl.maxiter, n + np);
+        sl.thr = sl.threshold;
+        sl.event = HP_EV_SWITCH;
+      } else {
+        sl.event = (n >= sl.min_stop && ch < sl.thr) ? HP_EV_CONVERGED : HP_EV_MAXITER;
+      }
+      sl.run = HP_RUN_IDLE;
+    }
+  }
+}
+
+// Host (set_run): the host moves molecules between stages by value.
+struct HpRunSet {
+  int n;
+  int run;
+  int slot[62];
+};
+static __global__ void k_batch_set_run(HpSlot *__restrict__ slots, HpRunSet rs) {
+  const int i = threadIdx.x;
+  if (i < rs.n) {
+    slots[rs.slot[i]].run = rs.run;
+    slots[rs.slot[i]].event = HP_EV_RUNNING;
+  }
+}
+
+// Admission: interleaved points (x, y, z) into the SoA arena columns.
+static __global__ void k_batch_points(const double *__restrict__ aos, int64_t n, double *__restrict__ px,
+                               double *__restrict__ py, double *__restrict__ pz) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) {
+    px[i] = aos[3 * i];
+    py[i] = aos[3 * i + 1];
+    pz[i] = aos[3 * i + 2];
+  }
+}
+
+// Admission: a molecule's local indices shifted to its arena ranges, per-atom
+// shell and QP offsets, and its per-atom state cleared.  One workgroup.
+struct HpAdmit {
+  int a0, na, s0, ns, f0;
+  int64_t n0, q0;
+  int has_qp;
+};
+static __global__ __launch_bounds__(256) void k_batch_admit(HpAdmit ad, const double *__restrict__ coords,
+                                                     const int *__restrict__ shell_atom_l,
+                                                     const int64_t *__restrict__ shell_off_l,
+                                                     const int *__restrict__ fn_off_l, double *__restrict__ cx,
+                                                     double *__restrict__ cy, double *__restrict__ cz,
+                                                     int *__restrict__ shell_atom, int64_t *__restrict__ shell_off,
+                                                     int *__restrict__ atom_shell_off, int *__restrict__ fn_off,
+                                                     int64_t *__restrict__ qp_off, int *__restrict__ inner_niter,
+                                                     int *__restrict__ status, int *__restrict__ status_all,
+                                                     double *__restrict__ charges, double *__restrict__ change_part) {
+  const int t = threadIdx.x;
+  for (int a = t; a < ad.na; a += blockDim.x) {
+    cx[ad.a0 + a] = coords[3 * a];
+    cy[ad.a0 + a] = coords[3 * a + 1];
+    cz[ad.a0 + a] = coords[3 * a + 2];
+    inner_niter[ad.a0 + a] = 0;
+    status[ad.a0 + a] = 0;
+    status_all[ad.a0 + a] = 0;
+    charges[ad.a0 + a] = 0.0;
+    change_part[ad.a0 + a] = 0.0;
+  }
+  for (int a = t; a <= ad.na; a += blockDim.x) fn_off[ad.a0 + a] = ad.f0 + fn_off_l[a];
+  for (int s = t; s < ad.ns; s += blockDim.x) {
+    const int a = shell_atom_l[s];
+    shell_atom[ad.s0 + s] = ad.a0 + a;
+    shell_off[ad.s0 + s] = ad.n0 + shell_off_l[s];
+    // first shell of its atom (shells are sorted by atom, each atom has one)
+    if (s == 0 || shell_atom_l[s - 1] != a) atom_shell_off[ad.a0 + a] = ad.s0 + s;
+  }
+  if (t == 0) {
+    shell_off[ad.s0 + ad.ns] = ad.n0 + shell_off_l[ad.ns];
+    atom_shell_off[ad.a0 + ad.na] = ad.s0 + ad.ns;
+    if (ad.has_qp) {
+      int64_t q = ad.q0;
+      for (int a = 0; a <= ad.na; ++a) {
+        qp_off[ad.a0 + a] = q;
+        if (a < ad.na) {
+          const int64_t K = fn_off_l[a + 1] - fn_off_l[a];
+          q += K * K;
+        }
+      }
+    }
+  }
 }

@@ -1,0 +1,66 @@
+// Host-side helpers shared by the translation units of the library: the
+// thread-local error text behind hp_last_error(), HIP error checks and the
+// launch-shape rules of the kernels.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <string>
+
+#include "hortonpart.h"
+
+namespace hp {
+
+inline thread_local std::string g_err;
+
+inline int fail(int code, const std::string &msg) {
+  g_err = msg;
+  return code;
+}
+
+template <typename T>
+int dalloc(T **p, size_t n) {
+  *p = nullptr;
+  if (n == 0) n = 1;
+  hipError_t e = hipMalloc((void **)p, n * sizeof(T));
+  if (e != hipSuccess) return fail(HP_E_HIP, std::string("hipMalloc: ") + hipGetErrorString(e));
+  return 0;
+}
+
+// Points per thread of the weights pass: 1 or 2, whichever idles fewer lanes
+// for the largest shell (ties go to 2, which halves the LDS reads per point).
+inline int weights_pt(int maxL) {
+  auto lanes = [](int n) { return ((n + 63) / 64) * 64; };
+  const double e1 = (double)maxL / lanes(maxL);
+  const double e2 = (double)maxL / (2.0 * lanes((maxL + 1) / 2));
+  return (maxL > 512 || e2 >= e1) ? 2 : 1;
+}
+
+// Workgroup size of the weights pass for a molecule whose largest radial
+// shell has maxL points.
+inline int weights_block(int maxL) {
+  const int pt = weights_pt(maxL);
+  int bs = ((maxL + pt - 1) / pt + 63) / 64 * 64;
+  return bs < 64 ? 64 : (bs > 512 ? 512 : bs);
+}
+
+// Radial points per lane of the one-wave-per-atom solvers.
+inline int pick_ppl(int R) {
+  if (R <= 128) return 2;
+  if (R <= 256) return 4;
+  if (R <= 512) return 8;
+  return 0;
+}
+
+}  // namespace hp
+
+#define HP_HIP(expr)                                                                                 \
+  do {                                                                                               \
+    hipError_t e_ = (expr);                                                                          \
+    if (e_ != hipSuccess) return hp::fail(HP_E_HIP, std::string(#expr ": ") + hipGetErrorString(e_)); \
+  } while (0)
+
+#define HP_LAUNCHED()                                                                                     \
+  do {                                                                                                    \
+    hipError_t e_ = hipGetLastError();                                                                    \
+    if (e_ != hipSuccess) return hp::fail(HP_E_HIP, std::string("kernel launch: ") + hipGetErrorString(e_)); \
+  } while (0)

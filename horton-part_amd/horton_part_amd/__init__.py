@@ -10,12 +10,14 @@ from .basis import ExpBasisFuncHelper, evaluate_function
 from .gisa import GaussianISAWPart
 from .grid import AtomGrid, BeckeWeights, ExpRTransform, MolGrid, OneDGrid, UniformInteger
 from .mbis import MBISWPart, get_initial_mbis_propars, get_nshell
+from .serving import PartitionServer
 from .utils import wpart_schemes
 
 __all__ = [
     "MBISWPart",
     "GaussianISAWPart",
     "LinearISAWPart",
+    "PartitionServer",
     "wpart_schemes",
     "get_nshell",
     "get_initial_mbis_propars",

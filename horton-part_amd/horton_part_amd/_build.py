@@ -36,7 +36,8 @@ def build(verbose=False):
     """Build the library; returns its path.  Raises on compiler errors."""
     LIB.parent.mkdir(parents=True, exist_ok=True)
     tmp = LIB.with_suffix(".so.tmp")
-    cmd = [hipcc(), *HIPCC_FLAGS, f"-I{INCLUDE}", f"-I{CSRC}", "-o", str(tmp), str(CSRC / "hortonpart.hip")]
+    sources = [str(CSRC / "hortonpart.hip"), str(CSRC / "hp_batch.hip")]
+    cmd = [hipcc(), *HIPCC_FLAGS, f"-I{INCLUDE}", f"-I{CSRC}", "-o", str(tmp), *sources]
     res = subprocess.run(cmd, capture_output=True, text=True)
     if res.returncode != 0:
         raise RuntimeError("hipcc failed:\n" + res.stderr[-4000:])

@@ -11,7 +11,7 @@ from pathlib import Path
 
 import numpy as np
 
-__all__ = ["load", "lib_path", "HPError", "EXPORTED_SYMBOLS", "MoleculeDesc", "SchemeDesc"]
+__all__ = ["load", "lib_path", "HPError", "EXPORTED_SYMBOLS", "MoleculeDesc", "SchemeDesc", "BatchConfig", "BatchResult"]
 
 HP_E_ARG = -1
 HP_E_HIP = -2
@@ -52,9 +52,21 @@ EXPORTED_SYMBOLS = [
     "hp_part_promoldens",
     "hp_part_moments",
     "hp_part_kernel_time",
+    "hp_batch_create",
+    "hp_batch_destroy",
+    "hp_batch_submit",
+    "hp_batch_step",
+    "hp_batch_poll",
+    "hp_batch_get_stats",
+    "hp_host_alloc",
+    "hp_host_free",
     "hp_becke_weights",
     "hp_slater_density",
 ]
+
+BATCH_CONVERGED = 1
+BATCH_MAXITER = 2
+BATCH_ERROR = 3
 
 _P = ctypes.POINTER
 _d = ctypes.c_double
@@ -108,6 +120,52 @@ class SchemeDesc(ctypes.Structure):
     ]
 
 
+class BatchConfig(ctypes.Structure):
+    _fields_ = [
+        ("point_capacity", ctypes.c_int64),
+        ("atom_capacity", ctypes.c_int32),
+        ("shell_capacity", ctypes.c_int32),
+        ("function_capacity", ctypes.c_int32),
+        ("max_inflight", ctypes.c_int32),
+        ("qp_capacity", ctypes.c_int64),
+        ("max_molecule_points", ctypes.c_int64),
+        ("ticks_per_step", ctypes.c_int32),
+        ("use_graphs", ctypes.c_int32),
+        ("inner_threshold", _d),
+        ("inner_maxiter", ctypes.c_int32),
+        ("density_cutoff", _d),
+        ("negative_cutoff", _d),
+        ("population_cutoff", _d),
+    ]
+
+
+class BatchResult(ctypes.Structure):
+    _fields_ = [
+        ("ticket", ctypes.c_int64),
+        ("event", ctypes.c_int32),
+        ("niter", ctypes.c_int32),
+        ("status", ctypes.c_int32),
+        ("status_last", ctypes.c_int32),
+        ("change", _d),
+        ("entropy", _d),
+    ]
+
+
+class BatchStats(ctypes.Structure):
+    _fields_ = [
+        ("ticks", ctypes.c_int64),
+        ("iterations", ctypes.c_int64),
+        ("completed", ctypes.c_int64),
+        ("tick_ms", _d),
+        ("weights_ms", _d),
+        ("inner_ms", _d),
+        ("points_fp32", _d),
+        ("points_fp64", _d),
+        ("evals_fp32", _d),
+        ("evals_fp64", _d),
+    ]
+
+
 class HPError(RuntimeError):
     """Failure reported by the device library (message from hp_last_error)."""
 
@@ -155,6 +213,14 @@ def load():
     lib.hp_part_promoldens.argtypes = [ctypes.c_void_p, _pd, ctypes.c_int64]
     lib.hp_part_moments.argtypes = [ctypes.c_void_p, ctypes.c_int, _pd, _pd, _pd]
     lib.hp_part_kernel_time.argtypes = [ctypes.c_void_p, ctypes.c_int, _pd, _pi64]
+    lib.hp_batch_create.argtypes = [ctypes.c_int, _P(BatchConfig), _P(ctypes.c_void_p)]
+    lib.hp_batch_destroy.argtypes = [ctypes.c_void_p]
+    lib.hp_batch_submit.argtypes = [ctypes.c_void_p, _P(MoleculeDesc), _P(SchemeDesc), _pd, ctypes.c_int, ctypes.c_int, _pd, _pd, _pi64]
+    lib.hp_batch_step.argtypes = [ctypes.c_void_p, _pi32, _pi32]
+    lib.hp_batch_poll.argtypes = [ctypes.c_void_p, _P(BatchResult), ctypes.c_int32, _pi32]
+    lib.hp_batch_get_stats.argtypes = [ctypes.c_void_p, _P(BatchStats)]
+    lib.hp_host_alloc.argtypes = [ctypes.c_int64, _P(ctypes.c_void_p)]
+    lib.hp_host_free.argtypes = [ctypes.c_void_p]
     lib.hp_becke_weights.argtypes = [ctypes.c_int, ctypes.c_int, _pd, _pd, ctypes.c_int64, _pd, _pi64, _pd]
     lib.hp_slater_density.argtypes = [ctypes.c_int, ctypes.c_int, _pd, _pi32, _pd, _pd, ctypes.c_int64, _pd, _pd]
     for name in EXPORTED_SYMBOLS:
@@ -316,3 +382,36 @@ class DevicePartition:
         n = ctypes.c_int64()
         check(self.lib.hp_part_kernel_time(self.handle, int(kernel), ctypes.byref(ms), ctypes.byref(n)))
         return ms.value, n.value
+
+
+class PinnedArray(np.ndarray):
+    """numpy array in page-locked host memory (freed with the array)."""
+
+    def __array_finalize__(self, obj):
+        if obj is not None and not hasattr(self, "_owner"):
+            self._owner = getattr(obj, "_owner", None)
+
+
+class _PinnedOwner:
+    def __init__(self, nbytes):
+        self.lib = load()
+        p = ctypes.c_void_p()
+        check(self.lib.hp_host_alloc(int(nbytes), ctypes.byref(p)))
+        self.ptr = p
+
+    def __del__(self):
+        try:
+            self.lib.hp_host_free(self.ptr)
+        except Exception:
+            pass
+
+
+def pinned_copy(a):
+    """A copy of ``a`` in page-locked host memory."""
+    a = np.ascontiguousarray(a)
+    owner = _PinnedOwner(max(a.nbytes, 1))
+    buf = (ctypes.c_char * max(a.nbytes, 1)).from_address(owner.ptr.value)
+    out = np.frombuffer(buf, dtype=a.dtype, count=a.size).reshape(a.shape).view(PinnedArray)
+    out._owner = owner
+    out[...] = a
+    return out

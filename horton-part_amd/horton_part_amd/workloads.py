@@ -9,6 +9,9 @@ Every workload is fixed by a ``numpy.random.default_rng`` seed:
   1-3 density shells per atom.
 * ``cluster`` N atoms placed uniformly at random in a cube sized for
   0.1 atoms/A^3 by random sequential addition (>= 0.9 A apart).
+* ``serving_stream`` a stream of partitioning requests: Poisson arrivals,
+  atom counts log-uniform in [3, 300], MBIS/GISA 50/50, each molecule a
+  ``chain`` with its own seed.
 
 The density is ``rho(p) = sum_a sum_k N_ak a_ak^3/(8 pi) exp(-a_ak |p-R_a|)
 * (1 + 0.02 u_p)`` with ``a_ak ~ U[1, 20]`` bohr^-1, ``N_ak > 0`` scaled so
@@ -22,7 +25,7 @@ import numpy as np
 from .grid import BeckeWeights, MolGrid, exp_radial_grid
 from .utils import ANGSTROM
 
-__all__ = ["Workload", "water", "chain", "cluster", "density_shells", "eval_density"]
+__all__ = ["Workload", "water", "chain", "cluster", "density_shells", "eval_density", "serving_stream"]
 
 Z_OF = {"H": 1, "C": 6, "N": 7, "O": 8, "S": 16}
 
@@ -144,3 +147,23 @@ def cluster(natom=200, seed=1002, nrad=100, nang=194, probs=None, becke=True, de
             pos.append(cand)
     coords = np.array(pos) * ANGSTROM
     return _build(f"cluster{natom}", coords, numbers, nrad, nang, rng, lambda z: int(rng.integers(1, 4)), becke=becke, device=device)
+
+
+def serving_stream(seed=1004, nreq=4096, rate=200.0, amin=3, amax=300):
+    """Requests of the serving stream, in arrival order.
+
+    Each entry: ``t`` arrival time in seconds (Poisson process of ``rate``
+    per second, the first at 0), ``natom`` (log-uniform in [amin, amax],
+    rounded), ``scheme`` ("mbis" or "gisa" with probability 1/2) and
+    ``mol_seed``, the seed of its ``chain`` molecule.
+    """
+    rng = np.random.default_rng(seed)
+    gaps = rng.exponential(1.0 / rate, size=nreq)
+    t = np.concatenate([[0.0], np.cumsum(gaps[1:])])
+    natom = np.clip(np.rint(np.exp(rng.uniform(np.log(amin), np.log(amax), size=nreq))), amin, amax).astype(int)
+    scheme = np.where(rng.random(nreq) < 0.5, "mbis", "gisa")
+    mol_seed = rng.integers(0, 2**31 - 1, size=nreq)
+    return [
+        {"t": float(t[i]), "natom": int(natom[i]), "scheme": str(scheme[i]), "mol_seed": int(mol_seed[i])}
+        for i in range(nreq)
+    ]

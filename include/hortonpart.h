@@ -139,7 +139,8 @@ int hp_part_set_propars(hp_part *part, const double *propars, int64_t n);
  * and compute_change (iterstock.py:32-45).  precision: HP_PREC_*.  With
  * HP_PREC_MIXED the loop runs in mixed precision until the change is below
  * max(threshold, mixed_switch), then in FP64 (at least n_polish iterations)
- * until the change is below threshold.  Returns the number of
+ * until the change is below threshold; maxiter caps the total number of
+ * iterations unless the polish needs more.  Returns the number of
  * outer iterations through *niter.  HP_E_SOLVER if a per-atom solver raised;
  * the loop stops at that iteration. */
 int hp_part_run(hp_part *part, int precision, int n_polish, int *niter);
@@ -181,6 +182,113 @@ int hp_part_kernel_time(hp_part *part, int kernel, double *total_ms, int64_t *la
 #define HP_KERNEL_WEIGHTS 0 /* fused pro-atom/promolecule/weights/sphavg pass */
 #define HP_KERNEL_INNER 1   /* per-atom inner solvers */
 #define HP_KERNEL_LISA_GRID 2 /* molecular-grid Newton pass (grid_type 2) */
+
+/* ---- continuous batching: many molecules in flight on one GPU ----
+ *
+ * The reference partitions one molecule per WPart object, one
+ * do_partitioning() call after another (the part-dens driver loops over its
+ * input files, scripts/program.py:120-131, one single_launch each,
+ * scripts/partition_density.py:172-263).  A batch
+ * keeps many such partitionings resident on the device and advances all of
+ * them with the same launches: every tick is one outer iteration
+ * (iterstock.py:159-193) of every molecule in flight.  Requests are admitted
+ * while the GPU works, as slots and arena space free up; each one stops on
+ * its own convergence test and leaves the batch.  The iteration of each
+ * molecule is the same as hp_part_run's with the same precision (the
+ * kernels see one molecule's data at a time, in the same order).
+ *
+ * Limits: grid_type 1, schemes MBIS (HP_SOLVER_MBIS) and GISA (HP_SOLVER_QP),
+ * no cut-off radius.  The settings the kernels take per launch are shared by
+ * every request (hp_batch_config): density_cutoff; inner_threshold and
+ * inner_maxiter of MBIS requests; negative_cutoff and population_cutoff of
+ * GISA requests.  Submissions with other values are rejected. */
+typedef struct {
+  int64_t point_capacity;    /* grid points resident at once */
+  int32_t atom_capacity;     /* atoms resident at once (+1 per molecule) */
+  int32_t shell_capacity;    /* radial shells resident at once (+1 per molecule) */
+  int32_t function_capacity; /* pro-atom functions resident at once */
+  int32_t max_inflight;      /* molecules resident at once */
+  int64_t qp_capacity;       /* doubles of GISA QP matrices resident at once */
+  int64_t max_molecule_points; /* largest molecule accepted (staging size) */
+  int32_t ticks_per_step;    /* outer iterations enqueued per hp_batch_step */
+  int32_t use_graphs;        /* replay the ticks of a step as a captured
+                                hipGraph while the molecules in flight do
+                                not change */
+  double inner_threshold;    /* shared inner-solver settings, as in */
+  int32_t inner_maxiter;     /* hp_scheme_desc */
+  double density_cutoff;
+  double negative_cutoff;
+  double population_cutoff;
+} hp_batch_config;
+
+/* Why a request left the batch (hp_batch_result.event) */
+#define HP_BATCH_CONVERGED 1 /* change < threshold */
+#define HP_BATCH_MAXITER 2   /* iteration cap reached (iterstock.py:187) */
+#define HP_BATCH_ERROR 3     /* a per-atom solver raised (status bits say which) */
+
+typedef struct {
+  int64_t ticket;  /* from hp_batch_submit */
+  int32_t event;   /* HP_BATCH_* */
+  int32_t niter;   /* outer iterations */
+  int32_t status;  /* OR of the per-atom HP_ST_* bits over all iterations */
+  int32_t status_last; /* fatal HP_ST_* bits of the last iteration */
+  double change;   /* last change */
+  double entropy;  /* last entropy */
+} hp_batch_result;
+
+typedef struct hp_batch hp_batch;
+
+/* Allocate the device arena of a batch (capacities in cfg). */
+int hp_batch_create(int device, const hp_batch_config *cfg, hp_batch **out);
+
+/* Wait for the GPU, then release the batch and its device memory. */
+int hp_batch_destroy(hp_batch *batch);
+
+/* Queue one partitioning request.  mol/scheme as for hp_part_create,
+ * propars0 the initial parameters (as hp_part_set_propars), precision and
+ * n_polish as for hp_part_run.  The arrays behind mol, scheme and propars0
+ * must stay valid and unchanged until the request's result is returned by
+ * hp_batch_poll; propars_out (P doubles) and charges_out (natom) receive
+ * the results before then.  *ticket identifies the request. */
+int hp_batch_submit(hp_batch *batch, const hp_molecule_desc *mol, const hp_scheme_desc *scheme,
+                    const double *propars0, int precision, int n_polish, double *propars_out, double *charges_out,
+                    int64_t *ticket);
+
+/* Advance the batch: wait for the previous step's iterations, retire the
+ * requests that stopped, admit queued requests that fit, and enqueue the
+ * next ticks_per_step iterations of everything in flight.  Returns the
+ * number of requests resident on the device and still queued. */
+int hp_batch_step(hp_batch *batch, int32_t *n_inflight, int32_t *n_queued);
+
+/* Take up to max finished requests (oldest first); *n receives how many. */
+int hp_batch_poll(hp_batch *batch, hp_batch_result *out, int32_t max, int32_t *n);
+
+/* Work and GPU time of a batch since it was created (HIP events on the
+ * batch's compute stream, read at each step for the ticks of the previous
+ * one).  The counts are algorithmic: one molecule-iteration of a molecule
+ * with N points and F pro-atom functions counts N points and N*F function
+ * evaluations, whatever the kernels skip. */
+typedef struct {
+  int64_t ticks;           /* ticks run */
+  int64_t iterations;      /* molecule-iterations done */
+  int64_t completed;       /* requests finished */
+  double tick_ms;          /* GPU time of the ticks */
+  double weights_ms;       /* ... of their weights launches (pro-atoms, promolecule,
+                              weights); 0 with use_graphs, which times whole steps */
+  double inner_ms;         /* ... of their per-atom solver launches (0 with use_graphs) */
+  double points_fp32;      /* grid points through the FP32 weights pass */
+  double points_fp64;      /* grid points through the FP64 weights pass */
+  double evals_fp32;       /* pro-atom function evaluations in FP32 */
+  double evals_fp64;       /* pro-atom function evaluations in FP64 */
+} hp_batch_stats;
+
+int hp_batch_get_stats(hp_batch *batch, hp_batch_stats *out);
+
+/* Page-locked host memory for request arrays: the batch's uploads from it
+ * run asynchronously, overlapping the ticks (no reference counterpart; a
+ * serving front end keeps its incoming grids and densities there). */
+int hp_host_alloc(int64_t bytes, void **ptr);
+int hp_host_free(void *ptr);
 
 /* ---- grid utilities (input preparation, not the iteration) ---- */
 

@@ -83,7 +83,8 @@ def _desc(**over):
     [
         (dict(shell_atom=np.array([1, 0, 0], np.int32)), _native.HP_E_ARG),
         (dict(shell_offsets=np.array([0, 4, 8, 11], np.int64)), _native.HP_E_ARG),
-        (dict(grid_type=2), _native.HP_E_UNSUPP),
+        (dict(grid_type=3), _native.HP_E_UNSUPP),
+        (dict(grid_type=2, check_mono=1), _native.HP_E_UNSUPP),
         (dict(solver=1), _native.HP_E_ARG),
         (dict(fn_offsets=np.array([0, 8, 9], np.int32)), _native.HP_E_UNSUPP),
         (dict(maxiter=0), _native.HP_E_ARG),
@@ -95,3 +96,29 @@ def test_create_rejects_bad_descriptions(over, code):
     with pytest.raises(_native.HPError) as exc:
         _native.DevicePartition(arrays, scheme)
     assert exc.value.code == code
+
+
+def test_batch_config_validated_without_gpu():
+    """hp_batch_create rejects empty capacities before touching a device."""
+    lib = _native.load()
+    cfg = _native.BatchConfig()
+    cfg.point_capacity = 1000
+    cfg.atom_capacity = 10
+    cfg.shell_capacity = 10
+    cfg.function_capacity = 10
+    cfg.max_inflight = 0  # no slots
+    cfg.qp_capacity = 0
+    cfg.max_molecule_points = 1000
+    cfg.ticks_per_step = 1
+    handle = ctypes.c_void_p()
+    rc = lib.hp_batch_create(0, ctypes.byref(cfg), ctypes.byref(handle))
+    assert rc == _native.HP_E_ARG
+    assert "capacities" in lib.hp_last_error().decode()
+    assert not handle.value
+    cfg.max_inflight = 8
+    cfg.ticks_per_step = 0
+    assert lib.hp_batch_create(0, ctypes.byref(cfg), ctypes.byref(handle)) == _native.HP_E_ARG
+    # NULL arguments
+    assert lib.hp_batch_step(None, None, None) == _native.HP_E_ARG
+    n = ctypes.c_int32(0)
+    assert lib.hp_batch_poll(None, None, 0, ctypes.byref(n)) == _native.HP_E_ARG
