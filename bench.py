@@ -358,10 +358,28 @@ def run_serving(args):
             "fp32_point_frac": d["points_fp32"] / pts if pts else 0.0,
         },
     }
+    peaks = measured_peaks()
+    out["measured"] = peaks
+    # exponential throughput against the probe's rate for the same exp
+    t_ideal = d["evals_fp32"] / peaks["exp_fp32_per_s"] + d["evals_fp64"] / peaks["exp_fp64_per_s"]
+    out["roofline"]["exp_frac"] = t_ideal / t_w if t_w > 0 else 0.0
+    out["roofline"]["copy_frac"] = achieved / peaks["copy_gbps"]
     if not args.no_cpu:
         log("CPU baseline (numpy oracle, 1 thread, bounded sample)")
         out["cpu_baseline"] = serving_cpu_baseline(timed, fp64, device=0)
     print(json.dumps(out))
+
+
+def measured_peaks():
+    """Device copy bandwidth and the exp throughput of both precisions."""
+    from horton_part_amd import _native
+
+    gbps, _ = _native.stream_copy(1 << 30, 10)
+    return {
+        "copy_gbps": gbps,
+        "exp_fp64_per_s": _native.exp_rate("fp64"),
+        "exp_fp32_per_s": _native.exp_rate("mixed"),
+    }
 
 
 def main():

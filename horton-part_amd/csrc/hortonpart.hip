@@ -153,10 +153,29 @@ void free_part(hp_part *p) {
 
 // ---- kernel launch helpers -------------------------------------------------
 
+template <typename R, int FK, bool GT2, int PT, int KM>
+void launch_weights_km(hp_part *p, int bs, int tile) {
+  const size_t lds = weights_lds_per_atom<R>(KM) * tile + 16;
+  hipLaunchKernelGGL((k_weights<R, FK, GT2, PT, KM>), dim3(p->nshell), dim3(bs), lds, p->stream, p->state, p->mol(),
+                     p->fn(), p->rhobar, p->ent_part, p->promol, p->dc, tile);
+}
+
 template <typename R, int FK, bool GT2, int PT>
-void launch_weights_pt(hp_part *p, int bs, size_t lds, int tile, int kslot) {
-  hipLaunchKernelGGL((k_weights<R, FK, GT2, PT>), dim3(p->nshell), dim3(bs), lds, p->stream, p->state, p->mol(), p->fn(),
-                     p->rhobar, p->ent_part, p->promol, p->dc, tile, kslot);
+void launch_weights_pt(hp_part *p, int bs) {
+  const int km = weights_km(p->maxK, FK);
+  const int tile = weights_tile_size<R>(km, p->natom);
+  if constexpr (FK == HP_FN_GENERAL) {
+    launch_weights_km<R, FK, GT2, PT, 16>(p, bs, tile);
+  } else {
+    if (km == 2)
+      launch_weights_km<R, FK, GT2, PT, 2>(p, bs, tile);
+    else if (km == 4)
+      launch_weights_km<R, FK, GT2, PT, 4>(p, bs, tile);
+    else if (km == 8)
+      launch_weights_km<R, FK, GT2, PT, 8>(p, bs, tile);
+    else
+      launch_weights_km<R, FK, GT2, PT, 16>(p, bs, tile);
+  }
 }
 
 template <typename R, int FK>
@@ -164,21 +183,16 @@ void launch_weights_fk(hp_part *p, bool gt2) {
   // one workgroup per radial shell, each thread one or two points
   const int pt = weights_pt(p->maxL);
   const int bs = hp::weights_block(p->maxL);
-  const int kslot = std::max(1, p->maxK);
-  const size_t per_atom = weights_lds_per_atom<R>(kslot);
-  int tile = (int)std::min<size_t>(256, (40 * 1024) / per_atom);
-  tile = std::max(1, std::min(tile, p->natom));
-  const size_t lds = per_atom * tile + 16;
   if (gt2) {
     if (pt == 1)
-      launch_weights_pt<R, FK, true, 1>(p, bs, lds, tile, kslot);
+      launch_weights_pt<R, FK, true, 1>(p, bs);
     else
-      launch_weights_pt<R, FK, true, 2>(p, bs, lds, tile, kslot);
+      launch_weights_pt<R, FK, true, 2>(p, bs);
   } else {
     if (pt == 1)
-      launch_weights_pt<R, FK, false, 1>(p, bs, lds, tile, kslot);
+      launch_weights_pt<R, FK, false, 1>(p, bs);
     else
-      launch_weights_pt<R, FK, false, 2>(p, bs, lds, tile, kslot);
+      launch_weights_pt<R, FK, false, 2>(p, bs);
   }
 }
 
@@ -985,6 +999,84 @@ int hp_slater_density(int device, int natom, const double *coordinates, const in
   if (doff) (void)hipFree(doff);
   if (dcoef) (void)hipFree(dcoef);
   if (dalpha) (void)hipFree(dalpha);
+  return rc;
+}
+
+int hp_stream_copy(int device, int64_t nbytes, int reps, double *gbps, double *ms_per_copy) {
+  g_err.clear();
+  if (nbytes < 16 || reps < 1 || !gbps || !ms_per_copy) return fail(HP_E_ARG, "bad argument");
+  HP_HIP(hipSetDevice(device));
+  const int64_t n2 = nbytes / 16;
+  double2 *src = nullptr, *dst = nullptr;
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  int rc = dalloc(&src, n2);
+  if (!rc) rc = dalloc(&dst, n2);
+  hipError_t e = hipSuccess;
+  if (!rc) {
+    int ncu = 256;
+    (void)hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, device);
+    const unsigned grid = (unsigned)std::min<int64_t>((n2 + 255) / 256, (int64_t)ncu * 16);
+    e = hipMemset(src, 0, n2 * sizeof(double2));
+    if (e == hipSuccess) e = hipEventCreate(&e0);
+    if (e == hipSuccess) e = hipEventCreate(&e1);
+    // one untimed copy, then reps timed ones
+    for (int r = 0; e == hipSuccess && r <= reps; ++r) {
+      if (r == 1) e = hipEventRecord(e0, nullptr);
+      hipLaunchKernelGGL(k_copy16, dim3(grid), dim3(256), 0, nullptr, src, dst, n2);
+      if (e == hipSuccess) e = hipGetLastError();
+    }
+    if (e == hipSuccess) e = hipEventRecord(e1, nullptr);
+    if (e == hipSuccess) e = hipEventSynchronize(e1);
+    float ms = 0.f;
+    if (e == hipSuccess) e = hipEventElapsedTime(&ms, e0, e1);
+    if (e != hipSuccess) {
+      rc = fail(HP_E_HIP, std::string("copy: ") + hipGetErrorString(e));
+    } else {
+      *ms_per_copy = ms / reps;
+      *gbps = 2.0 * (double)(n2 * 16) / (*ms_per_copy * 1e-3) / 1e9;
+    }
+  }
+  if (e0) (void)hipEventDestroy(e0);
+  if (e1) (void)hipEventDestroy(e1);
+  if (src) (void)hipFree(src);
+  if (dst) (void)hipFree(dst);
+  return rc;
+}
+
+int hp_exp_rate(int device, int precision, double *exps_per_s) {
+  g_err.clear();
+  if (!exps_per_s || (precision != HP_PREC_FP64 && precision != HP_PREC_MIXED)) return fail(HP_E_ARG, "bad argument");
+  HP_HIP(hipSetDevice(device));
+  int ncu = 256;
+  (void)hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, device);
+  const int block = 256, grid = ncu * 16, iters = 256;
+  double *out = nullptr;
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  int rc = dalloc(&out, (size_t)grid * block);
+  hipError_t e = hipSuccess;
+  if (!rc) {
+    e = hipEventCreate(&e0);
+    if (e == hipSuccess) e = hipEventCreate(&e1);
+    for (int r = 0; e == hipSuccess && r < 2; ++r) {  // the first launch is untimed
+      if (r == 1) e = hipEventRecord(e0, nullptr);
+      if (precision == HP_PREC_FP64)
+        hipLaunchKernelGGL(k_exp_rate<double>, dim3(grid), dim3(block), 0, nullptr, iters, out);
+      else
+        hipLaunchKernelGGL(k_exp_rate<float>, dim3(grid), dim3(block), 0, nullptr, iters, out);
+      if (e == hipSuccess) e = hipGetLastError();
+    }
+    if (e == hipSuccess) e = hipEventRecord(e1, nullptr);
+    if (e == hipSuccess) e = hipEventSynchronize(e1);
+    float ms = 0.f;
+    if (e == hipSuccess) e = hipEventElapsedTime(&ms, e0, e1);
+    if (e != hipSuccess)
+      rc = fail(HP_E_HIP, std::string("exp rate: ") + hipGetErrorString(e));
+    else
+      *exps_per_s = (double)grid * block * iters * HP_EXP_RATE_CHAINS / (ms * 1e-3);
+  }
+  if (e0) (void)hipEventDestroy(e0);
+  if (e1) (void)hipEventDestroy(e1);
+  if (out) (void)hipFree(out);
   return rc;
 }
 

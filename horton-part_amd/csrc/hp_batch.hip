@@ -79,6 +79,8 @@ class RangeAlloc {
 
 enum SlotState { SLOT_FREE = 0, SLOT_UPLOADING, SLOT_ACTIVE };
 
+constexpr int kGroups = 2;  // groups of molecules whose ticks interleave
+
 struct Request {
   int64_t ticket = 0;
   hp_molecule_desc md{};
@@ -101,7 +103,7 @@ struct Launch {
   int kind;  // 0 weights, 1 MBIS inner, 2 GISA QP
   int64_t off, len;
   // weights
-  int fp64, fk, bs, pt, kslot, tile;
+  int fp64, fk, bs, pt, tile;
   size_t lds;
   // per-atom kernels
   int km, block, ppl;
@@ -142,8 +144,11 @@ struct hp_batch {
   int *d_slot_list = nullptr, *h_slot_list = nullptr;
   int64_t work_cap = 0;
   int nslot_list = 0;
-  std::vector<Launch> plan;
-  bool plan_dirty = true;
+  // two groups of molecules, launched on cs and cs1
+  std::vector<Launch> plan[kGroups];
+  int nslot[kGroups] = {0, 0}, slot_off[kGroups] = {0, 0};
+  hipStream_t cs1 = nullptr;
+  hipEvent_t ev_fork = nullptr, ev_join = nullptr, ev_step[2] = {nullptr, nullptr};
   std::vector<int> pending_fp64;  // slots to flip to FP64 before the next ticks
   // graph of one step's ticks
   hipGraph_t graph = nullptr;
@@ -154,7 +159,7 @@ struct hp_batch {
   std::deque<hp_batch_result> done;
   int64_t next_ticket = 1;
   bool ticks_in_flight = false;
-  // timing: per tick of a step, events at its start, after k_batch_begin,
+  // per tick of a step and group: events at its start, after k_batch_begin,
   // after the weights launches, after the per-atom launches and at its end
   std::vector<hipEvent_t> tev;
   std::vector<int> last_niter;  // per slot: iterations already counted
@@ -223,6 +228,9 @@ void free_batch(hp_batch *b) {
     if (e) (void)hipEventDestroy(e);
   for (auto e : b->tev)
     if (e) (void)hipEventDestroy(e);
+  for (hipEvent_t e : {b->ev_fork, b->ev_join, b->ev_step[0], b->ev_step[1]})
+    if (e) (void)hipEventDestroy(e);
+  if (b->cs1) (void)hipStreamDestroy(b->cs1);
   for (auto *r : b->slot_req) delete r;
   for (auto *r : b->queue) delete r;
   if (b->cs) (void)hipStreamDestroy(b->cs);
@@ -443,15 +451,13 @@ int upload(hp_batch *b, Request *r) {
 
 // ---- work lists ---------------------------------------------------------------
 
-int build_plan(hp_batch *b) {
-  b->plan.clear();
-  int64_t w = 0;
-  int ns = 0;
-  std::vector<int> active;
-  for (int sl = 0; sl < b->cfg.max_inflight; ++sl)
-    if (b->slot_state[sl] == SLOT_ACTIVE) active.push_back(sl);
-  for (int sl : active) b->h_slot_list[ns++] = sl;
-  b->nslot_list = ns;
+// Weights-pass cost of one iteration of a molecule (the FP64 pass runs at
+// about 0.43x the FP32 rate).
+double weights_cost(const Request *r) { return (double)r->N * (double)r->F * (r->stage == HP_RUN_FP64 ? 2.3 : 1.0); }
+
+// Launches of one group of molecules; their work entries go to h_work[w...].
+void plan_group(hp_batch *b, const std::vector<int> &active, std::vector<Launch> &plan, int64_t &w) {
+  plan.clear();
   // weights launches: one per (precision, function kind, workgroup shape)
   std::map<std::tuple<int, int, int, int>, std::vector<int>> groups;
   for (int sl : active) {
@@ -476,12 +482,10 @@ int build_plan(hp_batch *b) {
       amax = std::max(amax, r->A);
     }
     L.len = w - L.off;
-    L.kslot = kmax;
-    const size_t per_atom = L.fp64 ? weights_lds_per_atom<double>(kmax) : weights_lds_per_atom<float>(kmax);
-    int tile = (int)std::min<size_t>(256, (40 * 1024) / per_atom);
-    L.tile = std::max(1, std::min(tile, amax));
-    L.lds = per_atom * L.tile + 16;
-    b->plan.push_back(L);
+    L.km = weights_km(kmax, L.fk);
+    L.tile = L.fp64 ? weights_tile_size<double>(L.km, amax) : weights_tile_size<float>(L.km, amax);
+    L.lds = (L.fp64 ? weights_lds_per_atom<double>(L.km) : weights_lds_per_atom<float>(L.km)) * L.tile + 16;
+    plan.push_back(L);
   }
   // MBIS inner solver: one launch; GISA QP: one launch per points-per-lane
   {
@@ -499,7 +503,7 @@ int build_plan(hp_batch *b) {
     L.len = w - L.off;
     L.km = kmax <= 2 ? 2 : (kmax <= 3 ? 3 : 7);
     L.block = ((rmax + 63) / 64) * 64;
-    if (L.len) b->plan.push_back(L);
+    if (L.len) plan.push_back(L);
   }
   for (int ppl : {2, 4, 8}) {
     Launch L{};
@@ -512,129 +516,183 @@ int build_plan(hp_batch *b) {
       for (int a = 0; a < r->A; ++a) b->h_work[w++] = make_int2(r->a0 + a, sl);
     }
     L.len = w - L.off;
-    if (L.len) b->plan.push_back(L);
+    if (L.len) plan.push_back(L);
   }
+}
+
+// Split the molecules in flight into two groups of about equal weights cost
+// (largest first, each to the lighter group) and build their launches.
+int build_plan(hp_batch *b) {
+  std::vector<int> active;
+  for (int sl = 0; sl < b->cfg.max_inflight; ++sl)
+    if (b->slot_state[sl] == SLOT_ACTIVE) active.push_back(sl);
+  std::sort(active.begin(), active.end(), [&](int x, int y) {
+    const double cx = weights_cost(b->slot_req[x]), cy = weights_cost(b->slot_req[y]);
+    return cx != cy ? cx > cy : x < y;
+  });
+  std::vector<int> grp[kGroups];
+  double load[kGroups] = {0.0, 0.0};
+  for (int sl : active) {
+    const int g = load[0] <= load[1] ? 0 : 1;
+    grp[g].push_back(sl);
+    load[g] += weights_cost(b->slot_req[sl]);
+  }
+  int64_t w = 0;
+  int ns = 0;
+  for (int g = 0; g < kGroups; ++g) {
+    std::sort(grp[g].begin(), grp[g].end());
+    b->slot_off[g] = ns;
+    b->nslot[g] = (int)grp[g].size();
+    for (int sl : grp[g]) b->h_slot_list[ns++] = sl;
+    plan_group(b, grp[g], b->plan[g], w);
+  }
+  b->nslot_list = ns;
   if (w > b->work_cap) return fail(HP_E_ARG, "work list overflow");
   if (w) HP_HIP(hipMemcpyAsync(b->d_work, b->h_work, w * sizeof(int2), hipMemcpyHostToDevice, b->cs));
   if (ns) HP_HIP(hipMemcpyAsync(b->d_slot_list, b->h_slot_list, ns * sizeof(int), hipMemcpyHostToDevice, b->cs));
   return 0;
 }
 
+template <typename R, int FK, int PT, int KM>
+void launch_w(hp_batch *b, const Launch &L, hipStream_t s) {
+  hipLaunchKernelGGL((k_weights<R, FK, false, PT, KM>), dim3((unsigned)L.len), dim3(L.bs), L.lds, s, nullptr,
+                     b->mol(L.off, 0), b->fn(), b->rhobar, b->ent_part, nullptr, b->cfg.density_cutoff, L.tile);
+}
+
 template <typename R, int FK, int PT>
-void launch_w(hp_batch *b, const Launch &L) {
-  hipLaunchKernelGGL((k_weights<R, FK, false, PT>), dim3((unsigned)L.len), dim3(L.bs), L.lds, b->cs, nullptr,
-                     b->mol(L.off, 0), b->fn(), b->rhobar, b->ent_part, nullptr, b->cfg.density_cutoff, L.tile,
-                     L.kslot);
+void launch_w_pt(hp_batch *b, const Launch &L, hipStream_t s) {
+  if (L.km == 2)
+    launch_w<R, FK, PT, 2>(b, L, s);
+  else if (L.km == 4)
+    launch_w<R, FK, PT, 4>(b, L, s);
+  else if (L.km == 8)
+    launch_w<R, FK, PT, 8>(b, L, s);
+  else
+    launch_w<R, FK, PT, 16>(b, L, s);
 }
 
 template <typename R>
-void launch_w_r(hp_batch *b, const Launch &L) {
+void launch_w_r(hp_batch *b, const Launch &L, hipStream_t s) {
   if (L.fk == HP_FN_SLATER) {
     if (L.pt == 1)
-      launch_w<R, HP_FN_SLATER, 1>(b, L);
+      launch_w_pt<R, HP_FN_SLATER, 1>(b, L, s);
     else
-      launch_w<R, HP_FN_SLATER, 2>(b, L);
+      launch_w_pt<R, HP_FN_SLATER, 2>(b, L, s);
   } else {
     if (L.pt == 1)
-      launch_w<R, HP_FN_GAUSS, 1>(b, L);
+      launch_w_pt<R, HP_FN_GAUSS, 1>(b, L, s);
     else
-      launch_w<R, HP_FN_GAUSS, 2>(b, L);
+      launch_w_pt<R, HP_FN_GAUSS, 2>(b, L, s);
   }
 }
 
-// One outer iteration of every molecule in flight (tick t of the step).
+// Event e of group g in tick t of a step: 0 start, 1 after k_batch_begin,
+// 2 after the weights launches, 3 after the per-atom launches, 4 end.
+inline hipEvent_t &tick_event(hp_batch *b, int t, int g, int e) { return b->tev[((size_t)t * kGroups + g) * 5 + e]; }
+
+// One outer iteration of every molecule in flight (tick t of the step).  The
+// weights launches of the two groups alternate (group 1 after group 0 of the
+// same tick, group 0 after group 1 of the previous tick), so that each
+// group's latency-bound per-atom solves run beside the other's weights pass.
 int enqueue_tick(hp_batch *b, int t) {
-  hipStream_t s = b->cs;
-  const int ns = b->nslot_list;
-  // per-kernel timing on direct launches only (events recorded inside a
-  // captured graph cannot be timed)
-  const bool timed = !b->cfg.use_graphs;
-  hipEvent_t *ev = &b->tev[5 * t];
-  if (timed) HP_HIP(hipEventRecord(ev[0], s));
-  hipLaunchKernelGGL(k_batch_begin, dim3(ns), dim3(256), 0, s, b->slots, b->d_slot_list, b->par_m, b->old_m, b->par_l,
-                     b->old_l, b->fn_norm, b->fn_alpha, b->coef, b->alpha);
-  HP_LAUNCHED();
-  if (timed) HP_HIP(hipEventRecord(ev[1], s));
-  bool inner_started = false;
-  for (const Launch &L : b->plan) {
-    if (L.kind != 0 && !inner_started) {
-      if (timed) HP_HIP(hipEventRecord(ev[2], s));
-      inner_started = true;
+  for (int g = 0; g < kGroups; ++g) {
+    const int ns = b->nslot[g];
+    if (ns == 0) continue;
+    hipStream_t s = g == 0 ? b->cs : b->cs1;
+    if (b->nslot[1 - g] > 0) {
+      if (g == 1)
+        HP_HIP(hipStreamWaitEvent(s, tick_event(b, t, 0, 2), 0));
+      else if (t > 0)
+        HP_HIP(hipStreamWaitEvent(s, tick_event(b, t - 1, 1, 2), 0));
     }
-    if (L.kind == 0) {
-      if (L.fp64)
-        launch_w_r<double>(b, L);
-      else
-        launch_w_r<float>(b, L);
-    } else if (L.kind == 1) {
-      const DevMol m = b->mol(L.off, (int)L.len);
+    const int *slot_list = b->d_slot_list + b->slot_off[g];
+    HP_HIP(hipEventRecord(tick_event(b, t, g, 0), s));
+    hipLaunchKernelGGL(k_batch_begin, dim3(ns), dim3(256), 0, s, b->slots, slot_list, b->par_m, b->old_m, b->par_l,
+                       b->old_l, b->fn_norm, b->fn_alpha, b->coef, b->alpha);
+    HP_LAUNCHED();
+    HP_HIP(hipEventRecord(tick_event(b, t, g, 1), s));
+    bool inner_started = false;
+    for (const Launch &L : b->plan[g]) {
+      if (L.kind != 0 && !inner_started) {
+        HP_HIP(hipEventRecord(tick_event(b, t, g, 2), s));
+        inner_started = true;
+      }
+      if (L.kind == 0) {
+        if (L.fp64)
+          launch_w_r<double>(b, L, s);
+        else
+          launch_w_r<float>(b, L, s);
+      } else if (L.kind == 1) {
+        const DevMol m = b->mol(L.off, (int)L.len);
 #define HP_BMBIS(KMV)                                                                                          \
   hipLaunchKernelGGL((k_mbis_inner<KMV>), dim3((unsigned)L.len), dim3(L.block), 0, s, nullptr, m, b->rhobar,    \
                      b->par_m, b->old_m, b->fn_off, b->cfg.inner_threshold, b->cfg.inner_maxiter, b->inner_niter, \
                      b->status, b->charges, b->change_part)
-      if (L.km == 2)
-        HP_BMBIS(2);
-      else if (L.km == 3)
-        HP_BMBIS(3);
-      else
-        HP_BMBIS(7);
+        if (L.km == 2)
+          HP_BMBIS(2);
+        else if (L.km == 3)
+          HP_BMBIS(3);
+        else
+          HP_BMBIS(7);
 #undef HP_BMBIS
-    } else {
-      const DevMol m = b->mol(L.off, (int)L.len);
-      const dim3 grid((unsigned)((L.len + 3) / 4)), block(256);
+      } else {
+        const DevMol m = b->mol(L.off, (int)L.len);
+        const dim3 grid((unsigned)((L.len + 3) / 4)), block(256);
 #define HP_BQP(PPLV)                                                                                               \
   hipLaunchKernelGGL((k_gisa_qp<PPLV>), grid, block, 0, s, nullptr, m, b->rhobar, b->par_l, b->old_l, b->fn_off,   \
                      b->fn_norm, b->fn_alpha, b->fn_order, b->qpP, b->qp_off, b->cfg.population_cutoff,           \
                      b->cfg.negative_cutoff, b->inner_niter, b->status, b->charges, b->change_part)
-      if (L.ppl == 2)
-        HP_BQP(2);
-      else if (L.ppl == 4)
-        HP_BQP(4);
-      else
-        HP_BQP(8);
+        if (L.ppl == 2)
+          HP_BQP(2);
+        else if (L.ppl == 4)
+          HP_BQP(4);
+        else
+          HP_BQP(8);
 #undef HP_BQP
+      }
+      HP_LAUNCHED();
     }
+    if (!inner_started) HP_HIP(hipEventRecord(tick_event(b, t, g, 2), s));
+    HP_HIP(hipEventRecord(tick_event(b, t, g, 3), s));
+    hipLaunchKernelGGL(k_batch_finalize, dim3(ns), dim3(256), 0, s, b->slots, slot_list, b->ent_part, b->change_part,
+                       b->status, b->status_all);
     HP_LAUNCHED();
+    HP_HIP(hipEventRecord(tick_event(b, t, g, 4), s));
   }
-  if (timed && !inner_started) HP_HIP(hipEventRecord(ev[2], s));
-  if (timed) HP_HIP(hipEventRecord(ev[3], s));
-  hipLaunchKernelGGL(k_batch_finalize, dim3(ns), dim3(256), 0, s, b->slots, b->d_slot_list, b->ent_part,
-                     b->change_part, b->status, b->status_all);
-  HP_LAUNCHED();
-  if (timed) HP_HIP(hipEventRecord(ev[4], s));
   return 0;
 }
 
-// GPU time of the previous step's ticks (after the stream was synchronised).
+// GPU time of the previous step (after the streams were synchronised).
 int account_ticks(hp_batch *b) {
-  if (b->cfg.use_graphs) {
-    // the whole replay: events around the graph launch
-    float ms = 0.f;
-    HP_HIP(hipEventElapsedTime(&ms, b->tev[0], b->tev[1]));
-    b->stats.tick_ms += ms;
-    b->stats.ticks += b->cfg.ticks_per_step;
-    return 0;
-  }
+  float ms = 0.f;
+  HP_HIP(hipEventElapsedTime(&ms, b->ev_step[0], b->ev_step[1]));
+  b->stats.tick_ms += ms;
+  b->stats.ticks += b->cfg.ticks_per_step;
+  if (b->cfg.use_graphs) return 0;  // events inside a replayed graph are not timed
   for (int t = 0; t < b->cfg.ticks_per_step; ++t) {
-    const hipEvent_t *ev = &b->tev[5 * t];
-    float ms_w = 0.f, ms_i = 0.f, ms_t = 0.f;
-    HP_HIP(hipEventElapsedTime(&ms_w, ev[1], ev[2]));
-    HP_HIP(hipEventElapsedTime(&ms_i, ev[2], ev[3]));
-    HP_HIP(hipEventElapsedTime(&ms_t, ev[0], ev[4]));
-    b->stats.weights_ms += ms_w;
-    b->stats.inner_ms += ms_i;
-    b->stats.tick_ms += ms_t;
-    b->stats.ticks += 1;
+    for (int g = 0; g < kGroups; ++g) {
+      if (b->nslot[g] == 0) continue;
+      float ms_w = 0.f, ms_i = 0.f;
+      HP_HIP(hipEventElapsedTime(&ms_w, tick_event(b, t, g, 1), tick_event(b, t, g, 2)));
+      HP_HIP(hipEventElapsedTime(&ms_i, tick_event(b, t, g, 2), tick_event(b, t, g, 3)));
+      b->stats.weights_ms += ms_w;
+      b->stats.inner_ms += ms_i;
+    }
   }
   return 0;
 }
 
-// The ticks of one step and the read-back of the slot records.
+// The ticks of one step and the read-back of the slot records; the second
+// group's stream forks from and joins the compute stream.
 int enqueue_step_work(hp_batch *b) {
+  HP_HIP(hipEventRecord(b->ev_fork, b->cs));
+  HP_HIP(hipStreamWaitEvent(b->cs1, b->ev_fork, 0));
   for (int t = 0; t < b->cfg.ticks_per_step; ++t) {
     int rc = enqueue_tick(b, t);
     if (rc) return rc;
   }
+  HP_HIP(hipEventRecord(b->ev_join, b->cs1));
+  HP_HIP(hipStreamWaitEvent(b->cs, b->ev_join, 0));
   HP_HIP(hipMemcpyAsync(b->h_slots, b->slots, b->cfg.max_inflight * sizeof(HpSlot), hipMemcpyDeviceToHost, b->cs));
   return 0;
 }
@@ -648,23 +706,27 @@ int drop_graph(hp_batch *b) {
 }
 
 int run_step_work(hp_batch *b) {
-  if (!b->cfg.use_graphs) return enqueue_step_work(b);
-  if (!b->gexec) {
-    HP_HIP(hipStreamBeginCapture(b->cs, hipStreamCaptureModeThreadLocal));
+  HP_HIP(hipEventRecord(b->ev_step[0], b->cs));
+  if (!b->cfg.use_graphs) {
     int rc = enqueue_step_work(b);
-    hipGraph_t g = nullptr;
-    hipError_t e = hipStreamEndCapture(b->cs, &g);
-    if (rc) {
-      if (g) (void)hipGraphDestroy(g);
-      return rc;
+    if (rc) return rc;
+  } else {
+    if (!b->gexec) {
+      HP_HIP(hipStreamBeginCapture(b->cs, hipStreamCaptureModeThreadLocal));
+      int rc = enqueue_step_work(b);
+      hipGraph_t g = nullptr;
+      hipError_t e = hipStreamEndCapture(b->cs, &g);
+      if (rc) {
+        if (g) (void)hipGraphDestroy(g);
+        return rc;
+      }
+      if (e != hipSuccess) return fail(HP_E_HIP, std::string("hipStreamEndCapture: ") + hipGetErrorString(e));
+      b->graph = g;
+      HP_HIP(hipGraphInstantiate(&b->gexec, g, nullptr, nullptr, 0));
     }
-    if (e != hipSuccess) return fail(HP_E_HIP, std::string("hipStreamEndCapture: ") + hipGetErrorString(e));
-    b->graph = g;
-    HP_HIP(hipGraphInstantiate(&b->gexec, g, nullptr, nullptr, 0));
+    HP_HIP(hipGraphLaunch(b->gexec, b->cs));
   }
-  HP_HIP(hipEventRecord(b->tev[0], b->cs));
-  HP_HIP(hipGraphLaunch(b->gexec, b->cs));
-  HP_HIP(hipEventRecord(b->tev[1], b->cs));
+  HP_HIP(hipEventRecord(b->ev_step[1], b->cs));
   return 0;
 }
 
@@ -715,6 +777,7 @@ int hp_batch_create(int device, const hp_batch_config *cfg, hp_batch **out) {
   hipError_t e = hipSetDevice(device);
   if (e != hipSuccess) return bail(fail(HP_E_HIP, std::string("hipSetDevice: ") + hipGetErrorString(e)));
   if ((e = hipStreamCreateWithFlags(&b->cs, hipStreamNonBlocking)) != hipSuccess ||
+      (e = hipStreamCreateWithFlags(&b->cs1, hipStreamNonBlocking)) != hipSuccess ||
       (e = hipStreamCreateWithFlags(&b->us, hipStreamNonBlocking)) != hipSuccess)
     return bail(fail(HP_E_HIP, std::string("hipStreamCreate: ") + hipGetErrorString(e)));
   const int64_t NP = cfg->point_capacity, NA = cfg->atom_capacity, NS = cfg->shell_capacity,
@@ -750,9 +813,11 @@ int hp_batch_create(int device, const hp_batch_config *cfg, hp_batch **out) {
   for (auto &ev : b->up_ev)
     if ((e = hipEventCreateWithFlags(&ev, hipEventDisableTiming)) != hipSuccess)
       return bail(fail(HP_E_HIP, "hipEventCreate failed"));
-  b->tev.assign(5 * (size_t)cfg->ticks_per_step, nullptr);
+  b->tev.assign(5 * (size_t)kGroups * cfg->ticks_per_step, nullptr);
   for (auto &ev : b->tev)
     if ((e = hipEventCreate(&ev)) != hipSuccess) return bail(fail(HP_E_HIP, "hipEventCreate failed"));
+  for (hipEvent_t *ev : {&b->ev_fork, &b->ev_join, &b->ev_step[0], &b->ev_step[1]})
+    if ((e = hipEventCreate(ev)) != hipSuccess) return bail(fail(HP_E_HIP, "hipEventCreate failed"));
   b->last_niter.assign(MI, 0);
   for (int i = (int)MI - 1; i >= 0; --i) b->free_slots.push_back(i);
   b->pts.init(NP);
@@ -772,6 +837,7 @@ int hp_batch_destroy(hp_batch *b) {
   if (!b) return fail(HP_E_ARG, "NULL batch");
   (void)hipSetDevice(b->device);
   (void)hipStreamSynchronize(b->cs);
+  (void)hipStreamSynchronize(b->cs1);
   (void)hipStreamSynchronize(b->us);
   free_batch(b);
   return 0;

@@ -303,34 +303,119 @@ __device__ __forceinline__ void promol_at(const DevMol &m, const DevFn &fn, int 
 
 // ---------------------------------------------------------------------------
 // K1: fused weights pass, one workgroup per radial shell.
-// The atoms that can reach the shell (all atoms, or the neighbour list when a
-// cut-off is set) are visited in tiles staged in LDS: centre, function count,
-// coefficients and exponents per atom, read by every lane as a broadcast.
-// FP64 path: distances from absolute coordinates.  FP32 path: coordinates
-// relative to the shell's atom (point offsets formed once in FP64), so the
-// per-pair arithmetic is all FP32.
-// LDS layout (dynamic): R cx[tile], cy[tile], cz[tile], coef[tile*kslot],
-// alpha[tile*kslot]; int idx[tile], nf[tile].
+// The atoms that can reach the shell (all atoms of its molecule, or the
+// neighbour list when a cut-off is set) are visited in tiles staged in LDS as
+// fixed-size records {x, y, z, nf, coef[KM], alpha[KM]}, read by every lane
+// as a broadcast (one wide read per record).  Each thread owns PT points.
+// FP64 path: distances from absolute coordinates, exp().  FP32 path:
+// coordinates relative to the shell's atom (offsets formed in FP64), raw
+// v_sqrt_f32, and exponents pre-scaled by log2(e) so that each function is one
+// multiply and one v_exp_f32.  The promolecule is accumulated in atom order
+// with the reference's 1e-100 floor after every atom (FP64 only: in FP32 the
+// floor rounds away).
 // ---------------------------------------------------------------------------
 template <typename R>
-__host__ __device__ constexpr size_t weights_lds_per_atom(int kslot) {
-  return (3 + 2 * (size_t)kslot) * sizeof(R) + 2 * sizeof(int);
+__host__ __device__ constexpr int weights_record(int km) {
+  return 4 + 2 * km;
+}
+template <typename R>
+__host__ __device__ constexpr size_t weights_lds_per_atom(int km) {
+  return (size_t)weights_record<R>(km) * sizeof(R) + sizeof(int);
 }
 
-template <typename R, int FK, bool GT2, int PT>
+// Functions per atom record: the smallest of 2, 4, 8, 16 that holds the
+// largest atom (general orders always use 16).
+inline __host__ int weights_km(int maxK, int fk) {
+  if (fk == HP_FN_GENERAL) return 16;
+  return maxK <= 2 ? 2 : (maxK <= 4 ? 4 : (maxK <= 8 ? 8 : 16));
+}
+
+// Atoms staged in LDS at once (records of km functions in R, <= 40 KB).
+template <typename R>
+inline __host__ int weights_tile_size(int km, int natom_max) {
+  const size_t per_atom = weights_lds_per_atom<R>(km);
+  const int tile = (int)((40 * 1024) / per_atom < 256 ? (40 * 1024) / per_atom : 256);
+  return tile < 1 ? 1 : (tile < natom_max ? tile : (natom_max < 1 ? 1 : natom_max));
+}
+
+template <typename R>
+__device__ __forceinline__ R fast_sqrt(R x);
+template <>
+__device__ __forceinline__ double fast_sqrt<double>(double x) { return sqrt(x); }
+template <>
+__device__ __forceinline__ float fast_sqrt<float>(float x) { return __builtin_amdgcn_sqrtf(x); }
+
+// exp of -(alpha * arg): FP64 exp(); FP32 exp2 with alpha pre-scaled by log2(e)
+template <typename R>
+__device__ __forceinline__ R neg_exp_scaled(R al, R arg);
+template <>
+__device__ __forceinline__ double neg_exp_scaled<double>(double al, double arg) { return exp(-al * arg); }
+template <>
+__device__ __forceinline__ float neg_exp_scaled<float>(float al, float arg) { return __builtin_amdgcn_exp2f(-al * arg); }
+
+template <typename R, int FK, int PT, int KM, bool CULL>
+__device__ __forceinline__ void weights_tile(const R *__restrict__ rec, const int *__restrict__ s_idx, int tn, int own_i,
+                                             const DevFn &fn, R rc, const R (&qx)[PT], const R (&qy)[PT],
+                                             const R (&qz)[PT], const double (&px)[PT], const double (&py)[PT],
+                                             const double (&pz)[PT], bool two, R (&promol)[PT], R (&own)[PT]) {
+  constexpr int RS = weights_record<R>(KM);
+  for (int i = 0; i < tn; ++i) {
+    const R *q = rec + (size_t)i * RS;
+    R v[RS];
+#pragma unroll
+    for (int u = 0; u < RS; ++u) v[u] = q[u];
+    const int nf = __builtin_amdgcn_readfirstlane((int)v[3]);
+    const bool is_own = i == own_i;
+#pragma unroll
+    for (int j = 0; j < PT; ++j) {
+      if (j > 0 && !two) break;
+      R dx, dy, dz;
+      if (sizeof(R) == sizeof(double)) {
+        dx = (R)(px[j] - (double)v[0]);
+        dy = (R)(py[j] - (double)v[1]);
+        dz = (R)(pz[j] - (double)v[2]);
+      } else {
+        dx = qx[j] - v[0];
+        dy = qy[j] - v[1];
+        dz = qz[j] - v[2];
+      }
+      const R r2 = dx * dx + dy * dy + dz * dz;
+      const R r = (FK == HP_FN_GAUSS && !CULL) ? (R)0 : fast_sqrt<R>(r2);
+      R pb = (R)0;
+#pragma unroll
+      for (int k = 0; k < KM; ++k) {
+        if (k < nf) {
+          R arg;
+          if (FK == HP_FN_SLATER) {
+            arg = r;
+          } else if (FK == HP_FN_GAUSS) {
+            arg = r2;
+          } else {
+            const double n = fn.order[fn.fn_off[s_idx[i]] + k];
+            arg = (n == 1.0) ? r : ((n == 2.0) ? r2 : (R)pow((double)r, n));
+          }
+          pb += v[4 + k] * neg_exp_scaled<R>(v[4 + KM + k], arg);
+        }
+      }
+      if (CULL && r > rc) pb = (R)0;
+      promol[j] += pb;
+      if (sizeof(R) == sizeof(double)) promol[j] += hp_floor_add<R>();
+      if (is_own) own[j] = pb;
+    }
+  }
+}
+
+template <typename R, int FK, bool GT2, int PT, int KM>
 __global__ __launch_bounds__(512) void k_weights(const int *__restrict__ done, DevMol m, DevFn fn,
                                                  double *__restrict__ rhobar, double *__restrict__ ent_part,
-                                                 double *__restrict__ promol_out, double dc, int tile, int kslot) {
-  extern __shared__ double smem_d[];
+                                                 double *__restrict__ promol_out, double dc, int tile) {
+  extern __shared__ __align__(16) double smem_d[];
   __shared__ double lds[32];
+  __shared__ int s_own;  // list position of the shell's own atom
   if (done && *done) return;
-  R *s_cx = reinterpret_cast<R *>(smem_d);
-  R *s_cy = s_cx + tile;
-  R *s_cz = s_cy + tile;
-  R *s_coef = s_cz + tile;
-  R *s_alpha = s_coef + (size_t)tile * kslot;
-  int *s_idx = reinterpret_cast<int *>(s_alpha + (size_t)tile * kslot);
-  int *s_nf = s_idx + tile;
+  constexpr int RS = weights_record<R>(KM);
+  R *s_rec = reinterpret_cast<R *>(smem_d);
+  int *s_idx = reinterpret_cast<int *>(s_rec + (size_t)tile * RS);
 
   int s = blockIdx.x;
   int a_begin = 0, a_end = m.natom;  // atoms of this shell's molecule
@@ -347,16 +432,16 @@ __global__ __launch_bounds__(512) void k_weights(const int *__restrict__ done, D
   const int64_t p0 = m.shell_off[s], p1 = m.shell_off[s + 1];
   int nb_n = a_end - a_begin;
   const int *nb = nullptr;
-  if (fn.rc > 0.0) {
+  const bool cull = fn.rc > 0.0;
+  if (cull) {
     nb = fn.nb_list + (fn.nb_off ? fn.nb_off[a] : (int64_t)a * m.natom);
     nb_n = fn.shell_nb[s];
   }
-  const bool cull = fn.rc > 0.0;
   const R rc = (R)fn.rc;
   const double ax = m.cx[a], ay = m.cy[a], az = m.cz[a];
   int64_t pidx[PT];
   double px[PT], py[PT], pz[PT];
-  R rx[PT], ry[PT], rz[PT];
+  R qx[PT], qy[PT], qz[PT];
   R promol[PT], own[PT];
 #pragma unroll
   for (int j = 0; j < PT; ++j) {
@@ -365,76 +450,48 @@ __global__ __launch_bounds__(512) void k_weights(const int *__restrict__ done, D
     px[j] = ok ? m.px[pidx[j]] : ax;
     py[j] = ok ? m.py[pidx[j]] : ay;
     pz[j] = ok ? m.pz[pidx[j]] : az;
-    rx[j] = (R)(px[j] - ax);
-    ry[j] = (R)(py[j] - ay);
-    rz[j] = (R)(pz[j] - az);
+    qx[j] = (R)(px[j] - ax);
+    qy[j] = (R)(py[j] - ay);
+    qz[j] = (R)(pz[j] - az);
     promol[j] = (R)0;
     own[j] = (R)0;
   }
   const bool two = p0 + blockDim.x < p1;  // second point slot used by any thread
+  constexpr double log2e = 1.4426950408889634;
+  if (threadIdx.x == 0) s_own = -1;
   for (int t0 = 0; t0 < nb_n; t0 += tile) {
     const int tn = min(tile, nb_n - t0);
     __syncthreads();
     for (int i = threadIdx.x; i < tn; i += blockDim.x) {
       const int b = nb ? nb[t0 + i] : a_begin + t0 + i;
+      R *q = s_rec + (size_t)i * RS;
       s_idx[i] = b;
+      if (b == a) s_own = t0 + i;
       if (sizeof(R) == sizeof(double)) {
-        s_cx[i] = (R)m.cx[b];
-        s_cy[i] = (R)m.cy[b];
-        s_cz[i] = (R)m.cz[b];
+        q[0] = (R)m.cx[b];
+        q[1] = (R)m.cy[b];
+        q[2] = (R)m.cz[b];
       } else {
-        s_cx[i] = (R)(m.cx[b] - ax);
-        s_cy[i] = (R)(m.cy[b] - ay);
-        s_cz[i] = (R)(m.cz[b] - az);
+        q[0] = (R)(m.cx[b] - ax);
+        q[1] = (R)(m.cy[b] - ay);
+        q[2] = (R)(m.cz[b] - az);
       }
       const int f0 = fn.fn_off[b], nf = fn.fn_off[b + 1] - f0;
-      s_nf[i] = nf;
-      for (int k = 0; k < nf; ++k) {
-        s_coef[i * kslot + k] = (R)fn.coef[f0 + k];
-        s_alpha[i * kslot + k] = (R)fn.alpha[f0 + k];
+      q[3] = (R)nf;
+#pragma unroll
+      for (int k = 0; k < KM; ++k) {
+        const bool on = k < nf;
+        q[4 + k] = on ? (R)fn.coef[f0 + k] : (R)0;
+        // FP32: exp(-alpha x) = exp2(-(alpha log2 e) x)
+        q[4 + KM + k] = on ? (R)(sizeof(R) == sizeof(double) ? fn.alpha[f0 + k] : fn.alpha[f0 + k] * log2e) : (R)0;
       }
     }
     __syncthreads();
-    for (int i = 0; i < tn; ++i) {
-      const int b = s_idx[i];
-      const int nf = s_nf[i];
-      const R bx = s_cx[i], by = s_cy[i], bz = s_cz[i];
-#pragma unroll
-      for (int j = 0; j < PT; ++j) {
-        if (j > 0 && !two) break;
-        R dx, dy, dz;
-        if (sizeof(R) == sizeof(double)) {
-          dx = (R)(px[j] - (double)bx);
-          dy = (R)(py[j] - (double)by);
-          dz = (R)(pz[j] - (double)bz);
-        } else {
-          dx = rx[j] - bx;
-          dy = ry[j] - by;
-          dz = rz[j] - bz;
-        }
-        const R r2 = dx * dx + dy * dy + dz * dz;
-        const R r = (FK == HP_FN_GAUSS && !cull) ? (R)0 : hp_sqrt<R>(r2);
-        R pb = (R)0;
-        if (!cull || r <= rc) {
-          for (int k = 0; k < nf; ++k) {
-            const R al = s_alpha[i * kslot + k];
-            R arg;
-            if (FK == HP_FN_SLATER) {
-              arg = r;
-            } else if (FK == HP_FN_GAUSS) {
-              arg = r2;
-            } else {
-              const double n = fn.order[fn.fn_off[b] + k];
-              arg = (n == 1.0) ? r : ((n == 2.0) ? r2 : (R)pow((double)r, n));
-            }
-            pb += s_coef[i * kslot + k] * hp_exp<R>(-al * arg);
-          }
-        }
-        promol[j] += pb;
-        promol[j] += hp_floor_add<R>();
-        if (b == a) own[j] = pb;
-      }
-    }
+    const int own_i = s_own - t0;  // outside [0, tn) when not in this tile
+    if (cull)
+      weights_tile<R, FK, PT, KM, true>(s_rec, s_idx, tn, own_i, fn, rc, qx, qy, qz, px, py, pz, two, promol, own);
+    else
+      weights_tile<R, FK, PT, KM, false>(s_rec, s_idx, tn, own_i, fn, rc, qx, qy, qz, px, py, pz, two, promol, own);
   }
   double acc_avg = 0.0, acc_ent = 0.0;
 #pragma unroll
@@ -1474,4 +1531,39 @@ static __global__ __launch_bounds__(256) void k_batch_admit(HpAdmit ad, const do
       }
     }
   }
+}
+
+// ---------------------------------------------------------------------------
+// Measurement probes: a plain copy (HBM bandwidth and counter calibration)
+// and the throughput of the exponential the weights pass uses.
+// ---------------------------------------------------------------------------
+static __global__ __launch_bounds__(256) void k_copy16(const double2 *__restrict__ src, double2 *__restrict__ dst,
+                                                       int64_t n) {
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) dst[i] = src[i];
+}
+
+#define HP_EXP_RATE_CHAINS 8
+// iters x 8 independent exponentials per thread, arguments in [-30, 0]
+template <typename R>
+__global__ __launch_bounds__(256) void k_exp_rate(int iters, double *__restrict__ out) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  R x[HP_EXP_RATE_CHAINS], acc[HP_EXP_RATE_CHAINS];
+#pragma unroll
+  for (int c = 0; c < HP_EXP_RATE_CHAINS; ++c) {
+    x[c] = (R)(-0.5 - 3.5 * c - 1e-3 * (t & 255));
+    acc[c] = (R)0;
+  }
+  const R step = (R)(1.0 / 1024.0);
+  for (int i = 0; i < iters; ++i) {
+#pragma unroll
+    for (int c = 0; c < HP_EXP_RATE_CHAINS; ++c) {
+      acc[c] += hp_exp<R>(x[c]);
+      x[c] -= step;
+    }
+  }
+  R s = (R)0;
+#pragma unroll
+  for (int c = 0; c < HP_EXP_RATE_CHAINS; ++c) s += acc[c];
+  out[t] = (double)s;
 }

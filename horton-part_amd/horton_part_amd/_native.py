@@ -62,6 +62,8 @@ EXPORTED_SYMBOLS = [
     "hp_host_free",
     "hp_becke_weights",
     "hp_slater_density",
+    "hp_stream_copy",
+    "hp_exp_rate",
 ]
 
 BATCH_CONVERGED = 1
@@ -220,6 +222,8 @@ def load():
     lib.hp_batch_poll.argtypes = [ctypes.c_void_p, _P(BatchResult), ctypes.c_int32, _pi32]
     lib.hp_batch_get_stats.argtypes = [ctypes.c_void_p, _P(BatchStats)]
     lib.hp_host_alloc.argtypes = [ctypes.c_int64, _P(ctypes.c_void_p)]
+    lib.hp_stream_copy.argtypes = [ctypes.c_int, ctypes.c_int64, ctypes.c_int, _pd, _pd]
+    lib.hp_exp_rate.argtypes = [ctypes.c_int, ctypes.c_int, _pd]
     lib.hp_host_free.argtypes = [ctypes.c_void_p]
     lib.hp_becke_weights.argtypes = [ctypes.c_int, ctypes.c_int, _pd, _pd, ctypes.c_int64, _pd, _pi64, _pd]
     lib.hp_slater_density.argtypes = [ctypes.c_int, ctypes.c_int, _pd, _pi32, _pd, _pd, ctypes.c_int64, _pd, _pd]
@@ -415,3 +419,20 @@ def pinned_copy(a):
     out._owner = owner
     out[...] = a
     return out
+
+
+def stream_copy(nbytes=1 << 30, reps=20, device=0):
+    """(GB/s, ms per copy) of a plain device copy of ``nbytes``."""
+    lib = load()
+    g = ctypes.c_double()
+    ms = ctypes.c_double()
+    check(lib.hp_stream_copy(int(device), int(nbytes), int(reps), ctypes.byref(g), ctypes.byref(ms)))
+    return g.value, ms.value
+
+
+def exp_rate(precision="fp64", device=0):
+    """Exponentials per second of the weights pass's exp in ``precision``."""
+    lib = load()
+    r = ctypes.c_double()
+    check(lib.hp_exp_rate(int(device), PREC[precision], ctypes.byref(r)))
+    return r.value

@@ -305,6 +305,19 @@ int hp_becke_weights(int device, int natom, const double *coordinates, const dou
 int hp_slater_density(int device, int natom, const double *coordinates, const int32_t *fn_offsets,
                       const double *coef, const double *alpha, int64_t npoint, const double *points, double *out);
 
+/* ---- measurement probes (no reference counterpart) ---- */
+
+/* Device-to-device copy of nbytes, reps times: achieved GB/s (read + write
+ * bytes per second) and milliseconds per copy.  The HBM bandwidth that a
+ * streaming kernel reaches on this device, and the reference point for
+ * calibrating memory counters. */
+int hp_stream_copy(int device, int64_t nbytes, int reps, double *gbps, double *ms_per_copy);
+
+/* Exponentials per second of the weights pass's exp (precision
+ * HP_PREC_FP64: the FP64 exp; HP_PREC_MIXED: the FP32 exp), independent
+ * arguments in [-30, 0] on every lane of the device. */
+int hp_exp_rate(int device, int precision, double *exps_per_s);
+
 #ifdef __cplusplus
 }
 #endif
