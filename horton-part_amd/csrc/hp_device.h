@@ -510,7 +510,12 @@ __global__ __launch_bounds__(512) void k_weights(const int *__restrict__ done, D
     } else {
       promol_out[p] = pm;
     }
-    if (!(pm < dc || rho < dc)) acc_ent += m.wmol[p] * rho * log(rho / pm);
+    // entropy integrand (a diagnostic): the log in R, the sum in FP64
+    if (!(pm < dc || rho < dc)) {
+      const double lg =
+          sizeof(R) == sizeof(double) ? log(rho / pm) : (double)__logf((float)rho / (float)pm_r);
+      acc_ent += m.wmol[p] * rho * lg;
+    }
   }
   block_sum2(acc_avg, acc_ent, lds);
   if (threadIdx.x == 0) {
