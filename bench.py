@@ -44,22 +44,39 @@ def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
-def build_workload(name, device):
-    from horton_part_amd import workloads
+PROTEIN_PROBS = {"C": 0.31, "H": 0.50, "N": 0.09, "O": 0.09, "S": 0.01}
 
-    if name == "chain20":
-        return workloads.chain(natom=20, seed=1001, nrad=150, nang=302, device=device), "mbis", 1e-6
+
+def build_workload(name, device):
+    """(workload, scheme class, keyword arguments, description) of a
+    single-molecule benchmark configuration."""
+    from horton_part_amd import ExpBasisFuncHelper, GaussianISAWPart, LinearISAWPart, MBISWPart, workloads
+
     if name == "water":
-        return workloads.water(seed=1000, nrad=100, nang=194, device=device), "mbis", 1e-8
+        w = workloads.water(seed=1000, nrad=100, nang=194, device=device)
+        return w, MBISWPart, dict(threshold=1e-8, maxiter=1000), "MBIS, threshold 1e-8"
+    if name == "chain20":
+        w = workloads.chain(natom=20, seed=1001, nrad=150, nang=302, device=device)
+        return w, MBISWPart, dict(threshold=1e-6, maxiter=1000), "MBIS, threshold 1e-6"
+    if name == "chain20-gisa":
+        w = workloads.chain(natom=20, seed=1001, nrad=150, nang=302, device=device)
+        return w, GaussianISAWPart, dict(threshold=1e-6, maxiter=1000), "GISA (gauss.json), threshold 1e-6"
     if name == "cluster200":
-        return workloads.cluster(natom=200, seed=1002, nrad=100, nang=194, device=device), "mbis", 1e-6
+        w = workloads.cluster(natom=200, seed=1002, nrad=100, nang=194, device=device)
+        alphas = 0.5 * 100.0 ** (np.arange(6) / 5.0)  # geometric 0.5 .. 50 bohr^-1
+        basis = ExpBasisFuncHelper.uniform(w.numbers, alphas, order=1.0)
+        kw = dict(threshold=1e-6, maxiter=200, solver="m-newton", basis_func=basis, grid_type=2)
+        return w, LinearISAWPart, kw, "aLISA, 6 Slater functions (0.5..50), m-newton on the molecular grid, threshold 1e-6"
+    if name == "protein2000":
+        # Becke weights skipped: on atom grids they enter only the entropy
+        w = workloads.cluster(natom=2000, seed=1003, nrad=75, nang=110, probs=PROTEIN_PROBS, becke=False, device=device)
+        kw = dict(threshold=1e-6, maxiter=1000, cutoff_radius=12.0)
+        return w, MBISWPart, kw, "MBIS, 12-bohr cut-off lists, threshold 1e-6 (molecular weights without Becke factors)"
     raise SystemExit(f"unknown workload {name}")
 
 
-def make_part(w, scheme, threshold, precision, maxiter=1000):
-    from horton_part_amd import MBISWPart
-
-    part = MBISWPart(w.coordinates, w.numbers, w.pseudo_numbers, w.grid, w.density, threshold=threshold, maxiter=maxiter, precision=precision)
+def make_part(w, cls, kw, precision):
+    part = cls(w.coordinates, w.numbers, w.pseudo_numbers, w.grid, w.density, precision=precision, **kw)
     dev = part._ensure_device()
     return part, dev
 
@@ -80,36 +97,67 @@ def time_steps(dev, propars0, precision, steps, warmup):
     return np.array(lat), out
 
 
-def cpu_baseline(w, threshold, gpu_niter, budget_s=20.0):
-    """Time the numpy oracle (reference structure, 1 thread) on a bounded
-    sample of outer iterations and extrapolate to a full partition."""
-    from oracle.horton_oracle import OracleMBIS
+def _oracle_for(w, cls, kw, maxiter):
+    """The oracle object of the same scheme and settings."""
+    from horton_part_amd import GaussianISAWPart, LinearISAWPart
+    from oracle.horton_oracle import OracleBasis, OracleGISA, OracleLISA, OracleMBIS
 
+    okw = {k: v for k, v in kw.items() if k in ("threshold", "grid_type", "cutoff_radius")}
+    okw["maxiter"] = maxiter
+    if cls is GaussianISAWPart or cls is LinearISAWPart:
+        part = cls(w.coordinates, w.numbers, w.pseudo_numbers, w.grid, w.density, **kw)
+        bs = part.bs_helper
+        okw["basis"] = OracleBasis(bs.orders, bs.exponents, bs.initials)
+        if cls is LinearISAWPart:
+            okw["solver"] = kw.get("solver", "sc")
+            return OracleLISA(w.coordinates, w.numbers, w.pseudo_numbers, w.grid, w.density, **okw)
+        return OracleGISA(w.coordinates, w.numbers, w.pseudo_numbers, w.grid, w.density, **okw)
+    return OracleMBIS(w.coordinates, w.numbers, w.pseudo_numbers, w.grid, w.density, **okw)
+
+
+def cpu_baseline(w, cls, kw, gpu_niter, name, budget_s=20.0, max_point_atoms=2e8):
+    """Time the numpy oracle (reference structure, 1 thread) on a bounded
+    number of outer iterations and extrapolate to a full partition with the
+    GPU's iteration count.  Workloads whose points x atoms exceed
+    ``max_point_atoms`` (the oracle stores every atom's distance to every
+    point, as the reference does) are sampled on a smaller molecule of the
+    same kind and extrapolated per point x atom."""
     import warnings
 
+    from horton_part_amd import workloads
+
     warnings.simplefilter("ignore")
-    # time a growing number of outer iterations until the budget is used
-    n = 1
-    per_iter = None
-    used = 0.0
+    sample_w, note = w, "the same workload"
+    if w.npoint * w.natom > max_point_atoms:
+        # points x atoms of the sample: about max_point_atoms / 4
+        nsmall = max(3, int(np.sqrt(max_point_atoms / 4 / (w.npoint / w.natom))))
+        nrad = w.grid.atgrids[0].rgrid.size
+        nang = w.grid.atgrids[0].size // nrad
+        probs = PROTEIN_PROBS if name == "protein2000" else None
+        sample_w = workloads.cluster(natom=nsmall, seed=1, nrad=nrad, nang=nang, probs=probs,
+                                     becke=name != "protein2000", device=0)
+        note = f"a {nsmall}-atom cluster of the same kind ({sample_w.npoint} points)"
+    n, used = 1, 0.0
     while True:
-        o = OracleMBIS(w.coordinates, w.numbers, w.pseudo_numbers, w.grid, w.density, threshold=threshold, maxiter=n)
+        o = _oracle_for(sample_w, cls, kw, n)
         t0 = time.perf_counter()
         o.do_partitioning()
         dt = time.perf_counter() - t0
         used += dt
-        per_iter = dt / n
         if used + 2 * dt > budget_s or n >= gpu_niter:
             break
         n *= 2
-    total = per_iter * gpu_niter
+    per_iter = dt / n
+    scale = (w.npoint * w.natom) / (sample_w.npoint * sample_w.natom)
+    total = per_iter * scale * gpu_niter
     return {
         "value": 1.0 / total,
         "unit": "partitions/s",
         "cores": 1,
         "kind": "port",
-        "sample": f"{n} outer iteration(s) of the numpy oracle (reference structure) on the same workload, "
-        f"{per_iter:.3f} s/iteration, extrapolated to the GPU's {gpu_niter} iterations per partition",
+        "sample": f"{n} outer iteration(s) of the numpy oracle (reference structure) on {note}: {per_iter:.3f} s/iteration"
+        + (f", scaled by points x atoms (x{scale:.1f})" if scale != 1 else "")
+        + f", extrapolated to the GPU's {gpu_niter} FP64 iterations per partition",
         "host_cpus": len(os.sched_getaffinity(0)),
         "cpu_model": platform.processor() or platform.machine(),
     }
@@ -387,7 +435,8 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=None, help="timed requests (serving) or partitions")
     ap.add_argument("--warmup", type=int, default=None)
-    ap.add_argument("--workload", default="serving", choices=["serving", "chain20", "water", "cluster200"])
+    ap.add_argument("--workload", default="serving",
+                    choices=["serving", "water", "chain20", "chain20-gisa", "cluster200", "protein2000"])
     ap.add_argument("--ticks", type=int, default=4, help="serving: outer iterations per host step")
     ap.add_argument("--graphs", action="store_true", help="serving: replay the ticks of a step as a hipGraph (times whole steps only)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
@@ -408,45 +457,78 @@ def main():
     args.warmup = 1 if args.warmup is None else args.warmup
     log(f"building workload {args.workload} on the GPU")
     t0 = time.perf_counter()
-    w, scheme, threshold = build_workload(args.workload, device=0)
+    w, cls, kw, desc = build_workload(args.workload, device=0)
     log(f"  {w.natom} atoms, {w.npoint} points, built in {time.perf_counter() - t0:.1f} s")
 
     results = {}
     for precision in ("fp64", "mixed"):
-        part, dev = make_part(w, scheme, threshold, precision)
+        part, dev = make_part(w, cls, kw, precision)
         propars0 = part._init_propars()
         log(f"timing {precision}: warmup {args.warmup}, steps {args.steps}")
         lat, (propars, charges, change, _) = time_steps(dev, propars0, precision, args.steps, args.warmup)
-        kw_ms, kw_n = dev.kernel_time(_native.KERNEL_WEIGHTS)
-        ki_ms, ki_n = dev.kernel_time(_native.KERNEL_INNER)
+        k = {name: dev.kernel_time(code) for name, code in
+             (("weights", _native.KERNEL_WEIGHTS), ("inner", _native.KERNEL_INNER), ("grid", _native.KERNEL_LISA_GRID))}
         results[precision] = {
             "lat": lat,
             "niter": dev.niter,
             "charges": charges,
-            "weights_kernel_ms": kw_ms / max(kw_n, 1),
-            "inner_kernel_ms": ki_ms / max(ki_n, 1),
-            "weights_launches": kw_n,
+            "kernels": k,
+            "work": dev.work(),
         }
         log(f"  {precision}: {np.mean(lat) * 1e3:.2f} ms/partition, niter {dev.niter}, "
-            f"weights {kw_ms / max(kw_n, 1) * 1e3:.1f} us/launch, inner {ki_ms / max(ki_n, 1) * 1e3:.1f} us/launch")
+            + ", ".join(f"{n} {ms / max(c, 1) * 1e3:.1f} us/launch" for n, (ms, c) in k.items() if c))
         dev.close()
 
     fp64, mixed = results["fp64"], results["mixed"]
     dq = float(np.max(np.abs(mixed["charges"] - fp64["charges"])))
     lat = mixed["lat"]
     value = 1.0 / float(np.mean(lat))
+    peaks = measured_peaks()
 
-    # roofline of the fused weights pass (FP64 path): algorithmic bytes per
-    # launch = 48 B per point (x, y, z, rho, angular weight, molecular weight)
-    # + 16 B per radial shell written; exp terms = points x sum_a K_a.
-    nshell = sum(g.rgrid.size for g in w.grid.atgrids)
-    from horton_part_amd.mbis import get_nshell
+    def per_launch(r, name):
+        ms, n = r["kernels"][name]
+        return ms / max(n, 1)
 
-    sum_k = int(sum(get_nshell(z) for z in w.numbers))
-    bytes_per_launch = 48.0 * w.npoint + 16.0 * nshell
-    t_launch = fp64["weights_kernel_ms"] * 1e-3
-    achieved = bytes_per_launch / t_launch / 1e9
-    exp_terms = float(w.npoint) * sum_k
+    wk = fp64["work"]
+    t_w = per_launch(fp64, "weights") * 1e-3
+    achieved = wk["bytes_per_pass"] / t_w / 1e9
+    roof = {
+        "kernel": "k_weights<double> (fused pro-atom/promolecule/weights/spherical average)",
+        "bound": "hbm",
+        "achieved": achieved,
+        "peak": HBM_PEAK_GBPS,
+        "unit": "GB/s",
+        "frac": achieved / HBM_PEAK_GBPS,
+        "traffic": None,
+        "bytes_per_launch": wk["bytes_per_pass"],
+        "launch_us": t_w * 1e6,
+        "evals_per_launch": wk["evals_per_pass"],
+        "evals_per_s": wk["evals_per_pass"] / t_w,
+        "exp_frac": wk["evals_per_pass"] / peaks["exp_fp64_per_s"] / t_w,
+        "copy_frac": achieved / peaks["copy_gbps"],
+    }
+    if fp64["kernels"]["grid"][1]:
+        # molecular-grid Newton passes: the Hessian B diag(w) B^T on MFMA
+        ms, n = fp64["kernels"]["grid"]
+        tf = wk["hessian_flops"] / (ms * 1e-3) / 1e12 if ms > 0 else 0.0
+        roof = {
+            "kernel": "k_gt2_pass<double, Newton> (per-atom gradient and Hessian over the molecular grid, FP64 MFMA)",
+            "bound": "mfma",
+            "achieved": tf,
+            "peak": 78.6,
+            "unit": "TFLOP/s",
+            "frac": tf / 78.6,
+            "traffic": None,
+            "hessian_flops": wk["hessian_flops"],
+            "hessian_passes": wk["hessian_passes"],
+            "pass_launches": n,
+            "pass_us": ms / max(n, 1) * 1e3,
+            "weights_pass": roof,
+        }
+        wm = mixed["work"]
+        ms32, n32 = mixed["kernels"]["grid"]
+        if ms32 > 0 and wm["hessian_flops"] > 0:
+            roof["mixed_fp32_mfma_tflops"] = wm["hessian_flops"] / (ms32 * 1e-3) / 1e12
     out = {
         "metric": METRIC,
         "value": value,
@@ -461,11 +543,10 @@ def main():
         "dtype": "f32+f64",
         "data": "synthetic",
         "config": {
-            "workload": f"{args.workload}: MBIS, {w.natom} atoms, {w.npoint} grid points, threshold {threshold:g}",
-            "scheme": scheme,
+            "workload": f"{args.workload}: {desc}, {w.natom} atoms, {w.npoint} grid points",
             "natom": w.natom,
             "npoint": int(w.npoint),
-            "precision": "mixed (FP32 grid arithmetic, FP64 sums/solves) + 1 FP64 polish iteration",
+            "precision": "mixed (FP32 grid arithmetic, FP64 sums/solves) + FP64 polish",
         },
         "p50_ms": float(np.percentile(lat, 50)) * 1e3,
         "p99_ms": float(np.percentile(lat, 99)) * 1e3,
@@ -478,27 +559,14 @@ def main():
         },
         "max_dq_mixed_vs_fp64": dq,
         "speedup_mixed_vs_fp64": float(np.mean(fp64["lat"]) / np.mean(lat)),
-        "kernels_us": {
-            "weights_fp64": fp64["weights_kernel_ms"] * 1e3,
-            "weights_mixed": mixed["weights_kernel_ms"] * 1e3,
-            "inner_fp64": fp64["inner_kernel_ms"] * 1e3,
-            "inner_mixed": mixed["inner_kernel_ms"] * 1e3,
-        },
-        "roofline": {
-            "kernel": "k_weights<double> (fused pro-atom/promolecule/weights/spherical average)",
-            "bound": "hbm",
-            "achieved": achieved,
-            "peak": HBM_PEAK_GBPS,
-            "unit": "GB/s",
-            "frac": achieved / HBM_PEAK_GBPS,
-            "traffic": None,
-            "exp_terms_per_launch": exp_terms,
-            "exp_terms_per_s": exp_terms / t_launch,
-        },
+        "kernels_us": {f"{n}_{p}": per_launch(results[p], n) * 1e3 for p in results for n in ("weights", "inner", "grid")
+                       if results[p]["kernels"][n][1]},
+        "roofline": roof,
+        "measured": peaks,
     }
     if not args.no_cpu:
         log("CPU baseline (numpy oracle, 1 thread, bounded sample)")
-        out["cpu_baseline"] = cpu_baseline(w, threshold, fp64["niter"])
+        out["cpu_baseline"] = cpu_baseline(w, cls, kw, fp64["niter"], args.workload)
     print(json.dumps(out))
 
 

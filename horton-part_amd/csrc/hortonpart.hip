@@ -70,6 +70,8 @@ struct hp_part {
   Gt2Atom *gt2 = nullptr;
   double *gt2_partial = nullptr;
   int *gt2_active = nullptr;
+  int *gt2_nhess = nullptr;  // per atom: Newton passes that formed a Hessian
+  double evals_per_pass = 0.0;  // pro-atom function evaluations of one weights pass
   int gt2_nchunk = 0;
   double *hpar = nullptr, *hch = nullptr, *hent = nullptr, *hchg = nullptr;
   int hist_cap = 0;
@@ -77,7 +79,9 @@ struct hp_part {
   int *state = nullptr;     // device: done flag, iteration counter
   static constexpr int kSlots = 32;
   hipEvent_t slot_ev[kSlots][3] = {};
-  double mixed_switch = 1e-5;
+  double mixed_switch = 0.0;  // > 0: leave the FP32 stage below this change
+  int mixed_refresh = 0;      // FP64 refresh of the FP32 correction every this many iterations
+  double *corr = nullptr;     // per radial shell: FP64 - FP32 spherical average
   // host state
   int niter = 0;
   double last_change = 0.0, last_entropy = 0.0;
@@ -134,7 +138,7 @@ void free_part(hp_part *p) {
                   p->fn_order, p->coef,       p->alpha,   p->qpP,       p->qp_off,      p->par,      p->old_par,
                   p->rhobar,  p->ent_part,    p->promol,  p->charges,   p->change_part, p->out,      p->mom_part,
                   p->mom_out, p->scratch_pts, p->inner_niter, p->status, p->status_all, p->hpar,    p->hch,
-                  p->hent,    p->hchg};
+                  p->hent,    p->hchg,    p->corr};
   for (void *b : bufs)
     if (b) (void)hipFree(b);
   if (p->h_out) (void)hipHostFree(p->h_out);
@@ -142,6 +146,7 @@ void free_part(hp_part *p) {
   if (p->gt2) (void)hipFree(p->gt2);
   if (p->gt2_partial) (void)hipFree(p->gt2_partial);
   if (p->gt2_active) (void)hipFree(p->gt2_active);
+  if (p->gt2_nhess) (void)hipFree(p->gt2_nhess);
   for (auto &e : p->ev)
     if (e) (void)hipEventDestroy(e);
   for (auto &slot : p->slot_ev)
@@ -154,56 +159,56 @@ void free_part(hp_part *p) {
 // ---- kernel launch helpers -------------------------------------------------
 
 template <typename R, int FK, bool GT2, int PT, int KM>
-void launch_weights_km(hp_part *p, int bs, int tile) {
+void launch_weights_km(hp_part *p, int bs, int tile, int corr_mode) {
   const size_t lds = weights_lds_per_atom<R>(KM) * tile + 16;
   hipLaunchKernelGGL((k_weights<R, FK, GT2, PT, KM>), dim3(p->nshell), dim3(bs), lds, p->stream, p->state, p->mol(),
-                     p->fn(), p->rhobar, p->ent_part, p->promol, p->dc, tile);
+                     p->fn(), p->rhobar, p->ent_part, p->promol, p->dc, tile, p->corr, corr_mode);
 }
 
 template <typename R, int FK, bool GT2, int PT>
-void launch_weights_pt(hp_part *p, int bs) {
+void launch_weights_pt(hp_part *p, int bs, int corr_mode) {
   const int km = weights_km(p->maxK, FK);
   const int tile = weights_tile_size<R>(km, p->natom);
   if constexpr (FK == HP_FN_GENERAL) {
-    launch_weights_km<R, FK, GT2, PT, 16>(p, bs, tile);
+    launch_weights_km<R, FK, GT2, PT, 16>(p, bs, tile, corr_mode);
   } else {
     if (km == 2)
-      launch_weights_km<R, FK, GT2, PT, 2>(p, bs, tile);
+      launch_weights_km<R, FK, GT2, PT, 2>(p, bs, tile, corr_mode);
     else if (km == 4)
-      launch_weights_km<R, FK, GT2, PT, 4>(p, bs, tile);
+      launch_weights_km<R, FK, GT2, PT, 4>(p, bs, tile, corr_mode);
     else if (km == 8)
-      launch_weights_km<R, FK, GT2, PT, 8>(p, bs, tile);
+      launch_weights_km<R, FK, GT2, PT, 8>(p, bs, tile, corr_mode);
     else
-      launch_weights_km<R, FK, GT2, PT, 16>(p, bs, tile);
+      launch_weights_km<R, FK, GT2, PT, 16>(p, bs, tile, corr_mode);
   }
 }
 
 template <typename R, int FK>
-void launch_weights_fk(hp_part *p, bool gt2) {
+void launch_weights_fk(hp_part *p, bool gt2, int corr_mode) {
   // one workgroup per radial shell, each thread one or two points
   const int pt = weights_pt(p->maxL);
   const int bs = hp::weights_block(p->maxL);
   if (gt2) {
     if (pt == 1)
-      launch_weights_pt<R, FK, true, 1>(p, bs);
+      launch_weights_pt<R, FK, true, 1>(p, bs, HP_CORR_NONE);
     else
-      launch_weights_pt<R, FK, true, 2>(p, bs);
+      launch_weights_pt<R, FK, true, 2>(p, bs, HP_CORR_NONE);
   } else {
     if (pt == 1)
-      launch_weights_pt<R, FK, false, 1>(p, bs);
+      launch_weights_pt<R, FK, false, 1>(p, bs, corr_mode);
     else
-      launch_weights_pt<R, FK, false, 2>(p, bs);
+      launch_weights_pt<R, FK, false, 2>(p, bs, corr_mode);
   }
 }
 
 template <typename R>
-void launch_weights(hp_part *p, bool gt2) {
+void launch_weights(hp_part *p, bool gt2, int corr_mode = HP_CORR_NONE) {
   if (p->fnkind == HP_FN_SLATER)
-    launch_weights_fk<R, HP_FN_SLATER>(p, gt2);
+    launch_weights_fk<R, HP_FN_SLATER>(p, gt2, corr_mode);
   else if (p->fnkind == HP_FN_GAUSS)
-    launch_weights_fk<R, HP_FN_GAUSS>(p, gt2);
+    launch_weights_fk<R, HP_FN_GAUSS>(p, gt2, corr_mode);
   else
-    launch_weights_fk<R, HP_FN_GENERAL>(p, gt2);
+    launch_weights_fk<R, HP_FN_GENERAL>(p, gt2, corr_mode);
 }
 
 template <int PPL>
@@ -259,17 +264,28 @@ int launch_inner(hp_part *p) {
 // Enqueue one outer iteration (update_at_weights, per-atom updates, entropy,
 // change, stop test).  Every kernel returns at once when the device flag says
 // the loop has stopped, so iterations can be enqueued ahead of the decision.
-int enqueue_iteration(hp_part *p, int precision, double thr, int min_stop, int max_iter, int slot) {
+int enqueue_iteration(hp_part *p, int precision, double thr, int min_stop, int max_iter, int slot, bool refresh,
+                      int rollback) {
   hipStream_t s = p->stream;
   const int n = std::max(p->npar, p->nfn);
   hipLaunchKernelGGL(k_iter_begin, dim3((n + 255) / 256), dim3(256), 0, s, p->state, p->scheme == HP_SCHEME_MBIS ? 0 : 1,
                      p->npar, p->nfn, p->par, p->old_par, p->fn_norm, p->fn_alpha, p->coef, p->alpha);
   HP_LAUNCHED();
   HP_HIP(hipEventRecord(p->slot_ev[slot][0], s));
-  if (precision == HP_PREC_MIXED)
-    launch_weights<float>(p, false);
-  else
+  if (precision == HP_PREC_MIXED) {
+    if (p->mixed_refresh <= 0) {
+      launch_weights<float>(p, false, HP_CORR_NONE);
+    } else if (refresh) {
+      // FP64 iteration that also calibrates the FP32 correction
+      launch_weights<double>(p, false, HP_CORR_NONE);
+      HP_LAUNCHED();
+      launch_weights<float>(p, false, HP_CORR_CALIBRATE);
+    } else {
+      launch_weights<float>(p, false, HP_CORR_APPLY);
+    }
+  } else {
     launch_weights<double>(p, false);
+  }
   HP_LAUNCHED();
   HP_HIP(hipEventRecord(p->slot_ev[slot][1], s));
   int rc = launch_inner(p);
@@ -278,14 +294,14 @@ int enqueue_iteration(hp_part *p, int precision, double thr, int min_stop, int m
   HP_HIP(hipEventRecord(p->slot_ev[slot][2], s));
   hipLaunchKernelGGL(k_finalize, dim3(1), dim3(256), 0, s, p->state, thr, min_stop, max_iter, p->nshell, p->ent_part,
                      p->natom, p->change_part, p->status, p->status_all, p->par, p->npar, p->charges, p->hpar, p->hch,
-                     p->hent, p->hchg, p->out);
+                     p->hent, p->hchg, p->out, p->old_par, rollback);
   HP_LAUNCHED();
   return 0;
 }
 
 // Run iterations until the device stop test fires: batches of speculative
 // iterations, one host synchronisation per batch.
-int run_stage(hp_part *p, int precision, double thr, int min_stop, int max_iter) {
+int run_stage(hp_part *p, int precision, double thr, int min_stop, int max_iter, int rollback = 0) {
   hipStream_t s = p->stream;
   HP_HIP(hipMemsetAsync(p->state, 0, sizeof(int), s));
   int batch = 2;
@@ -294,7 +310,9 @@ int run_stage(hp_part *p, int precision, double thr, int min_stop, int max_iter)
     const int remaining = max_iter - before;
     const int nb = std::max(1, std::min(batch, remaining));
     for (int c = 0; c < nb; ++c) {
-      int rc = enqueue_iteration(p, precision, thr, min_stop, max_iter, c);
+      // iteration before + c, if the loop has not stopped by then
+      const bool refresh = p->mixed_refresh > 0 && (before + c) % p->mixed_refresh == 0;
+      int rc = enqueue_iteration(p, precision, thr, min_stop, max_iter, c, refresh, rollback);
       if (rc) return rc;
     }
     HP_HIP(hipMemcpyAsync(p->h_out, p->out, 4 * sizeof(double), hipMemcpyDeviceToHost, s));
@@ -392,7 +410,7 @@ int gt2_iteration(hp_part *p, int precision, double thr, int min_stop, int max_i
       }
       hipLaunchKernelGGL(k_gt2_update, dim3(p->natom), dim3(128), 0, s, p->state, p->gt2_nchunk, p->gt2, p->fn_off,
                          p->qpP, p->qp_off, p->gt2_partial, scheme, p->solver, p->inner_threshold, p->inner_maxiter,
-                         p->neg_cutoff, p->pop_cutoff, p->tau, p->nlin, p->gt2_active);
+                         p->neg_cutoff, p->pop_cutoff, p->tau, p->nlin, p->gt2_active, p->gt2_nhess);
       HP_LAUNCHED();
       HP_HIP(hipEventRecord(p->slot_ev[c][2], s));
     }
@@ -421,7 +439,7 @@ int gt2_iteration(hp_part *p, int precision, double thr, int min_stop, int max_i
   HP_LAUNCHED();
   hipLaunchKernelGGL(k_finalize, dim3(1), dim3(256), 0, s, p->state, thr, min_stop, max_iter, p->nshell, p->ent_part,
                      p->natom, p->change_part, p->status, p->status_all, p->par, p->npar, p->charges, p->hpar, p->hch,
-                     p->hent, p->hchg, p->out);
+                     p->hent, p->hchg, p->out, p->old_par, 0);
   HP_LAUNCHED();
   HP_HIP(hipMemcpyAsync(p->h_out, p->out, 4 * sizeof(double), hipMemcpyDeviceToHost, s));
   HP_HIP(hipMemcpyAsync(p->h_out + 4, p->state, 2 * sizeof(int), hipMemcpyDeviceToHost, s));
@@ -563,7 +581,8 @@ int hp_part_create(int device, const hp_molecule_desc *md, const hp_scheme_desc 
   p->tau = sd->newton_tau;
   p->nlin = sd->newton_linspace;
   p->check_mono = sd->check_mono;
-  p->mixed_switch = sd->mixed_switch > 0.0 ? sd->mixed_switch : 1e-5;
+  p->mixed_switch = sd->mixed_switch > 0.0 ? sd->mixed_switch : 0.0;
+  p->mixed_refresh = sd->mixed_refresh > 0 ? sd->mixed_refresh : 0;
   p->maxiter = sd->maxiter;
   p->inner_maxiter = sd->inner_maxiter;
   p->h_shell_off.assign(md->shell_offsets, md->shell_offsets + S + 1);
@@ -636,8 +655,10 @@ int hp_part_create(int device, const hp_molecule_desc *md, const hp_scheme_desc 
   }
   // neighbour lists for the cut-off: atoms sorted by distance to each atom,
   // and per shell the number of atoms that can reach any of its points
+  std::vector<int> nb_h, shell_nb_h;
   if (p->rc > 0.0) {
-    std::vector<int> nb((size_t)A * A);
+    std::vector<int> &nb = nb_h;
+    nb.resize((size_t)A * A);
     std::vector<double> dist((size_t)A * A);
     for (int a = 0; a < A; ++a) {
       std::vector<std::pair<double, int>> v(A);
@@ -651,7 +672,8 @@ int hp_part_create(int device, const hp_molecule_desc *md, const hp_scheme_desc 
         dist[(size_t)a * A + t] = v[t].first;
       }
     }
-    std::vector<int> shell_nb(S);
+    std::vector<int> &shell_nb = shell_nb_h;
+    shell_nb.resize(S);
     for (int si = 0; si < S; ++si) {
       const int a = shell_atom[si];
       double rmax = 0.0;
@@ -666,6 +688,24 @@ int hp_part_create(int device, const hp_molecule_desc *md, const hp_scheme_desc 
     if ((rc = dupload(&p->nb_list, nb.data(), nb.size(), s)) || (rc = dupload(&p->shell_nb, shell_nb.data(), S, s)))
       return bail(rc);
   }
+  // algorithmic work of one weights pass: function evaluations per point
+  // over the atoms that reach its shell
+  if (p->rc > 0.0) {
+    double ev = 0.0;
+    for (int si = 0; si < S; ++si) {
+      const int a = shell_atom[si];
+      const double L = (double)(md->shell_offsets[si + 1] - md->shell_offsets[si]);
+      int64_t kf = 0;
+      for (int t = 0; t < shell_nb_h[si]; ++t) {
+        const int b = nb_h[(size_t)a * A + t];
+        kf += fn_off[b + 1] - fn_off[b];
+      }
+      ev += L * (double)kf;
+    }
+    p->evals_per_pass = ev;
+  } else {
+    p->evals_per_pass = (double)N * (double)nfn;
+  }
   // state
   p->hist_cap = 2 * p->maxiter + 64;
   if ((rc = dalloc(&p->state, 2))) return bail(rc);
@@ -674,12 +714,13 @@ int hp_part_create(int device, const hp_molecule_desc *md, const hp_scheme_desc 
       (rc = dalloc(&p->charges, A)) || (rc = dalloc(&p->change_part, A)) || (rc = dalloc(&p->out, 4)) ||
       (rc = dalloc(&p->inner_niter, A)) || (rc = dalloc(&p->status, A)) || (rc = dalloc(&p->status_all, A)) ||
       (rc = dalloc(&p->hpar, (size_t)p->hist_cap * p->npar)) || (rc = dalloc(&p->hch, (size_t)p->hist_cap * A)) ||
-      (rc = dalloc(&p->hent, p->hist_cap)) || (rc = dalloc(&p->hchg, p->hist_cap)))
+      (rc = dalloc(&p->hent, p->hist_cap)) || (rc = dalloc(&p->hchg, p->hist_cap)) || (rc = dalloc(&p->corr, S)))
     return bail(rc);
   if (p->grid_type == 2) {
     p->gt2_nchunk = (int)((N + HP_GT2_CHUNK - 1) / HP_GT2_CHUNK);
     if ((rc = dalloc(&p->promol, N)) || (rc = dalloc(&p->gt2, A)) ||
-        (rc = dalloc(&p->gt2_partial, (size_t)A * p->gt2_nchunk * HP_GT2_NV)) || (rc = dalloc(&p->gt2_active, 1)))
+        (rc = dalloc(&p->gt2_partial, (size_t)A * p->gt2_nchunk * HP_GT2_NV)) || (rc = dalloc(&p->gt2_active, 1)) ||
+        (rc = dalloc(&p->gt2_nhess, A)))
       return bail(rc);
   }
   e = hipMemsetAsync(p->state, 0, sizeof(int) * 2, s);
@@ -725,6 +766,7 @@ int hp_part_run(hp_part *p, int precision, int n_polish, int *niter) {
   }
   HP_HIP(hipMemsetAsync(p->status_all, 0, sizeof(int) * p->natom, p->stream));
   HP_HIP(hipMemsetAsync(p->state, 0, 2 * sizeof(int), p->stream));
+  if (p->gt2_nhess) HP_HIP(hipMemsetAsync(p->gt2_nhess, 0, sizeof(int) * p->natom, p->stream));
   p->niter = 0;
   int rc = 0;
   auto stage = [&](int prec, double thr, int min_stop, int max_iter) {
@@ -735,11 +777,20 @@ int hp_part_run(hp_part *p, int precision, int n_polish, int *niter) {
     // iterstock.py:172-188: stop when change < threshold or counter >= maxiter
     rc = stage(HP_PREC_FP64, p->threshold, 1, p->maxiter);
   } else {
-    // mixed stage until the change is below max(threshold, switch level),
-    // then FP64 iterations (at least n_polish) until change < threshold;
+    if (p->grid_type == 2) {
+      // FP32 stage until change < max(threshold, switch level (1e-5 unless
+      // set)), then FP64 iterations (at least n_polish) until change < threshold
+      const double sw = p->mixed_switch > 0.0 ? p->mixed_switch : 1e-5;
+      rc = run_stage_gt2(p, HP_PREC_MIXED, std::max(p->threshold, sw), 1, p->maxiter);
+    } else {
+      // FP32 stage (with the FP64 defect correction every mixed_refresh
+      // iterations) until change < max(threshold * HP_MIXED_MARGIN, switch
+      // level); the iteration that passes is discarded and redone in FP64,
+      // and FP64 iterations (at least n_polish) follow until change < threshold
+      const double thr1 = std::max(p->threshold * HP_MIXED_MARGIN, p->mixed_switch);
+      rc = run_stage(p, HP_PREC_MIXED, thr1, 1, p->maxiter, 1);
+    }
     // maxiter caps the total unless the polish needs more
-    const double thr1 = std::max(p->threshold, p->mixed_switch);
-    rc = stage(HP_PREC_MIXED, thr1, 1, p->maxiter);
     if (!rc) {
       const int start = p->niter;
       const int np = std::max(1, n_polish);
@@ -885,6 +936,28 @@ int hp_part_kernel_time(hp_part *p, int kernel, double *total_ms, int64_t *launc
   if (!p || kernel < 0 || kernel > 2) return fail(HP_E_ARG, "bad argument");
   if (total_ms) *total_ms = p->k_ms[kernel];
   if (launches) *launches = p->k_n[kernel];
+  return 0;
+}
+
+int hp_part_get_work(hp_part *p, hp_part_work *out) {
+  if (!p || !out) return fail(HP_E_ARG, "NULL argument");
+  HP_HIP(hipSetDevice(p->device));
+  out->points = p->npoint;
+  out->evals_per_pass = p->evals_per_pass;
+  out->bytes_per_pass = 48.0 * (double)p->npoint + 16.0 * (double)p->nshell;
+  out->hessian_passes = 0;
+  out->hessian_flops = 0.0;
+  if (p->gt2_nhess) {
+    std::vector<int> nh(p->natom);
+    HP_HIP(hipMemcpy(nh.data(), p->gt2_nhess, sizeof(int) * p->natom, hipMemcpyDeviceToHost));
+    std::vector<int> off(p->natom + 1);
+    HP_HIP(hipMemcpy(off.data(), p->fn_off, sizeof(int) * (p->natom + 1), hipMemcpyDeviceToHost));
+    for (int a = 0; a < p->natom; ++a) {
+      const double K = off[a + 1] - off[a];
+      out->hessian_passes += nh[a];
+      out->hessian_flops += nh[a] * 2.0 * K * K * (double)p->npoint;
+    }
+  }
   return 0;
 }
 

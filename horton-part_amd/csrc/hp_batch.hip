@@ -96,6 +96,7 @@ struct Request {
   int slot = -1, a0 = 0, s0 = 0, f0 = 0;
   int64_t n0 = 0, q0 = 0;
   int stage = HP_RUN_FP64;  // run value the host believes the slot has
+  int corr_period = 0;      // FP32 stage: FP64 refresh period of the correction
 };
 
 // One kernel launch of a tick.
@@ -120,7 +121,7 @@ struct hp_batch {
   // shells
   int *shell_atom = nullptr;
   int64_t *shell_off = nullptr;
-  double *shell_r = nullptr, *shell_w = nullptr, *rhobar = nullptr, *ent_part = nullptr;
+  double *shell_r = nullptr, *shell_w = nullptr, *rhobar = nullptr, *ent_part = nullptr, *corr = nullptr;
   // atoms
   double *cx = nullptr, *cy = nullptr, *cz = nullptr, *pseudo = nullptr, *charges = nullptr, *change_part = nullptr;
   int *atom_shell_off = nullptr, *fn_off = nullptr, *inner_niter = nullptr, *status = nullptr, *status_all = nullptr;
@@ -213,6 +214,7 @@ void free_batch(hp_batch *b) {
   if (b->graph) (void)hipGraphDestroy(b->graph);
   void *bufs[] = {b->px,         b->py,          b->pz,          b->rho,         b->wmol,      b->wsph,
                   b->shell_atom, b->shell_off,   b->shell_r,     b->shell_w,     b->rhobar,    b->ent_part,
+                  b->corr,
                   b->cx,         b->cy,          b->cz,          b->pseudo,      b->charges,   b->change_part,
                   b->atom_shell_off, b->fn_off,  b->inner_niter, b->status,      b->status_all, b->qp_off,
                   b->coef,       b->alpha,       b->fn_norm,     b->fn_alpha,    b->fn_order,  b->par_l,
@@ -439,9 +441,12 @@ int upload(hp_batch *b, Request *r) {
   h.n_polish = r->n_polish;
   h.maxiter = r->sd.maxiter;
   h.threshold = r->sd.threshold;
-  const double sw = r->sd.mixed_switch > 0.0 ? r->sd.mixed_switch : 1e-5;
-  h.thr = r->precision == HP_PREC_MIXED ? std::max(r->sd.threshold, sw) : r->sd.threshold;
+  const double sw = r->sd.mixed_switch > 0.0 ? r->sd.mixed_switch : 0.0;
+  h.thr = r->precision == HP_PREC_MIXED ? std::max(r->sd.threshold * HP_MIXED_MARGIN, sw) : r->sd.threshold;
+  h.corr_period = r->precision == HP_PREC_MIXED && r->sd.mixed_refresh > 0 ? r->sd.mixed_refresh : 0;
+  h.refresh = 0;
   r->stage = r->precision == HP_PREC_MIXED ? HP_RUN_FP32 : HP_RUN_FP64;
+  r->corr_period = h.corr_period;
   HP_HIP(hipMemcpyAsync(b->slots + r->slot, &h, sizeof(HpSlot), hipMemcpyHostToDevice, s));
   HP_HIP(hipEventRecord(b->up_ev[r->slot], s));
   b->slot_req[r->slot] = r;
@@ -458,18 +463,22 @@ double weights_cost(const Request *r) { return (double)r->N * (double)r->F * (r-
 // Launches of one group of molecules; their work entries go to h_work[w...].
 void plan_group(hp_batch *b, const std::vector<int> &active, std::vector<Launch> &plan, int64_t &w) {
   plan.clear();
-  // weights launches: one per (precision, function kind, workgroup shape)
+  // weights launches: one per (precision, function kind, workgroup shape),
+  // the FP64 ones first: an FP32-stage molecule with the defect correction
+  // is in both, and on its refresh iterations its FP32 pass reads the
+  // averages of its FP64 pass
   std::map<std::tuple<int, int, int, int>, std::vector<int>> groups;
   for (int sl : active) {
     const Request *r = b->slot_req[sl];
     const int pt = hp::weights_pt(r->maxL), bs = hp::weights_block(r->maxL);
     const int fk = r->scheme01 == 0 ? HP_FN_SLATER : HP_FN_GAUSS;
-    groups[{r->stage == HP_RUN_FP64 ? 1 : 0, fk, bs, pt}].push_back(sl);
+    if (r->stage == HP_RUN_FP64 || r->corr_period > 0) groups[{0, fk, bs, pt}].push_back(sl);
+    if (r->stage == HP_RUN_FP32) groups[{1, fk, bs, pt}].push_back(sl);
   }
   for (auto &g : groups) {
     Launch L{};
     L.kind = 0;
-    L.fp64 = std::get<0>(g.first);
+    L.fp64 = std::get<0>(g.first) == 0;
     L.fk = std::get<1>(g.first);
     L.bs = std::get<2>(g.first);
     L.pt = std::get<3>(g.first);
@@ -556,7 +565,8 @@ int build_plan(hp_batch *b) {
 template <typename R, int FK, int PT, int KM>
 void launch_w(hp_batch *b, const Launch &L, hipStream_t s) {
   hipLaunchKernelGGL((k_weights<R, FK, false, PT, KM>), dim3((unsigned)L.len), dim3(L.bs), L.lds, s, nullptr,
-                     b->mol(L.off, 0), b->fn(), b->rhobar, b->ent_part, nullptr, b->cfg.density_cutoff, L.tile);
+                     b->mol(L.off, 0), b->fn(), b->rhobar, b->ent_part, nullptr, b->cfg.density_cutoff, L.tile,
+                     b->corr, HP_CORR_SLOT);
 }
 
 template <typename R, int FK, int PT>
@@ -655,7 +665,7 @@ int enqueue_tick(hp_batch *b, int t) {
     if (!inner_started) HP_HIP(hipEventRecord(tick_event(b, t, g, 2), s));
     HP_HIP(hipEventRecord(tick_event(b, t, g, 3), s));
     hipLaunchKernelGGL(k_batch_finalize, dim3(ns), dim3(256), 0, s, b->slots, slot_list, b->ent_part, b->change_part,
-                       b->status, b->status_all);
+                       b->status, b->status_all, b->par_m, b->old_m, b->par_l, b->old_l);
     HP_LAUNCHED();
     HP_HIP(hipEventRecord(tick_event(b, t, g, 4), s));
   }
@@ -789,6 +799,7 @@ int hp_batch_create(int device, const hp_batch_config *cfg, hp_batch **out) {
       (rc = dalloc(&b->rho, NP)) || (rc = dalloc(&b->wmol, NP)) || (rc = dalloc(&b->wsph, NP)) ||
       (rc = dalloc(&b->shell_atom, NS)) || (rc = dalloc(&b->shell_off, NS)) || (rc = dalloc(&b->shell_r, NS)) ||
       (rc = dalloc(&b->shell_w, NS)) || (rc = dalloc(&b->rhobar, NS)) || (rc = dalloc(&b->ent_part, NS)) ||
+      (rc = dalloc(&b->corr, NS)) ||
       (rc = dalloc(&b->cx, NA)) || (rc = dalloc(&b->cy, NA)) || (rc = dalloc(&b->cz, NA)) ||
       (rc = dalloc(&b->pseudo, NA)) || (rc = dalloc(&b->charges, NA)) || (rc = dalloc(&b->change_part, NA)) ||
       (rc = dalloc(&b->atom_shell_off, NA)) || (rc = dalloc(&b->fn_off, NA)) || (rc = dalloc(&b->inner_niter, NA)) ||
@@ -891,6 +902,13 @@ int hp_batch_step(hp_batch *b, int32_t *n_inflight, int32_t *n_queued) {
         if (r->stage == HP_RUN_FP32) {
           b->stats.points_fp32 += pts;
           b->stats.evals_fp32 += ev;
+          if (r->corr_period > 0 && d > 0) {
+            // refresh iterations among [niter - d, niter) also ran in FP64
+            const int m = r->corr_period, n1 = h.niter, n0 = n1 - d;
+            const int nr = (n1 + m - 1) / m - (n0 + m - 1) / m;
+            b->stats.points_fp64 += (double)nr * r->N;
+            b->stats.evals_fp64 += (double)nr * r->N * r->F;
+          }
         } else {
           b->stats.points_fp64 += pts;
           b->stats.evals_fp64 += ev;

@@ -35,6 +35,11 @@
 #define HP_ST_SINGULAR 256
 #define HP_ST_NONFINITE 512
 #define HP_PI 3.14159265358979323846
+// The FP32 stage of the mixed path ends when the change is below
+// threshold * HP_MIXED_MARGIN: just above the final test, so that the FP64
+// redo of that iteration starts no later than the plain FP64 path stops.
+#define HP_MIXED_MARGIN 1.01
+
 // status bits that end the outer loop (the reference raises on them)
 #define HP_ST_FATAL                                                                                         \
   (HP_ST_NEGDENS | HP_ST_LINESEARCH | HP_ST_NEWTONMAX | HP_ST_QPFAIL | HP_ST_NOTMONO | HP_ST_SINGULAR | \
@@ -65,6 +70,9 @@ struct HpSlot {
   int min_stop, max_iter;  // bounds of the current stage
   int n_polish, maxiter;   // the request's settings
   int err, any;        // status bits: fatal ones of the last iteration, OR of all
+  int corr_period;     // FP32 stage: FP64 refresh of the correction every this many iterations (0: none)
+  int refresh;         // this iteration refreshes the correction (set by k_batch_begin)
+  int pad[2];
   double thr;          // threshold of the current stage
   double threshold;    // the request's threshold
   double change, entropy;
@@ -405,10 +413,24 @@ __device__ __forceinline__ void weights_tile(const R *__restrict__ rec, const in
   }
 }
 
+// Defect correction of the FP32 pass (corr_mode):
+//   HP_CORR_NONE      rhobar = this pass's average
+//   HP_CORR_APPLY     rhobar = FP32 average + corr (the correction of the last refresh)
+//   HP_CORR_CALIBRATE corr = rhobar - FP32 average, where rhobar was just written by
+//                     the FP64 pass at the same parameters (rhobar, entropy kept)
+//   HP_CORR_SLOT      batched: per molecule, CALIBRATE on its refresh iterations,
+//                     APPLY otherwise (FP32 launches); FP64 launches also run the
+//                     FP32-stage molecules that refresh in this iteration.
+#define HP_CORR_NONE 0
+#define HP_CORR_APPLY 1
+#define HP_CORR_CALIBRATE 2
+#define HP_CORR_SLOT -1
+
 template <typename R, int FK, bool GT2, int PT, int KM>
 __global__ __launch_bounds__(512) void k_weights(const int *__restrict__ done, DevMol m, DevFn fn,
                                                  double *__restrict__ rhobar, double *__restrict__ ent_part,
-                                                 double *__restrict__ promol_out, double dc, int tile) {
+                                                 double *__restrict__ promol_out, double dc, int tile,
+                                                 double *__restrict__ corr, int corr_mode) {
   extern __shared__ __align__(16) double smem_d[];
   __shared__ double lds[32];
   __shared__ int s_own;  // list position of the shell's own atom
@@ -419,11 +441,19 @@ __global__ __launch_bounds__(512) void k_weights(const int *__restrict__ done, D
 
   int s = blockIdx.x;
   int a_begin = 0, a_end = m.natom;  // atoms of this shell's molecule
+  int mode = corr_mode;
   if (m.work) {
     const int2 wk = m.work[blockIdx.x];
     const HpSlot &sl = m.slots[wk.y];
-    // a molecule takes part only in the launches of its current stage
-    if (sl.run != (sizeof(R) == sizeof(double) ? HP_RUN_FP64 : HP_RUN_FP32)) return;
+    // a molecule takes part in the launches of its current stage, and an
+    // FP32-stage molecule also in the FP64 launch of a refresh iteration
+    if (sizeof(R) == sizeof(double)) {
+      if (!(sl.run == HP_RUN_FP64 || (sl.run == HP_RUN_FP32 && sl.refresh))) return;
+      mode = HP_CORR_NONE;
+    } else {
+      if (sl.run != HP_RUN_FP32) return;
+      if (mode == HP_CORR_SLOT) mode = sl.corr_period > 0 ? (sl.refresh ? HP_CORR_CALIBRATE : HP_CORR_APPLY) : HP_CORR_NONE;
+    }
     s = wk.x;
     a_begin = sl.a0;
     a_end = sl.a0 + sl.na;
@@ -519,8 +549,14 @@ __global__ __launch_bounds__(512) void k_weights(const int *__restrict__ done, D
   }
   block_sum2(acc_avg, acc_ent, lds);
   if (threadIdx.x == 0) {
-    if (!GT2) rhobar[s] = acc_avg;
-    ent_part[s] = acc_ent;
+    if (GT2) {
+      ent_part[s] = acc_ent;
+    } else if (mode == HP_CORR_CALIBRATE) {
+      corr[s] = rhobar[s] - acc_avg;
+    } else {
+      rhobar[s] = mode == HP_CORR_APPLY ? acc_avg + corr[s] : acc_avg;
+      ent_part[s] = acc_ent;
+    }
   }
 }
 
@@ -1206,11 +1242,13 @@ __global__ __launch_bounds__(256) void k_lisa_inner(const int *__restrict__ done
 static __global__ __launch_bounds__(256) void k_finalize(int *__restrict__ state, double thr, int min_stop, int max_iter,
                                                   int nshell, const double *__restrict__ ent_part, int natom,
                                                   const double *__restrict__ change_part, const int *__restrict__ status,
-                                                  int *__restrict__ status_all, const double *__restrict__ par, int npar,
+                                                  int *__restrict__ status_all, double *__restrict__ par, int npar,
                                                   const double *__restrict__ charges, double *__restrict__ hpar,
                                                   double *__restrict__ hch, double *__restrict__ hent,
-                                                  double *__restrict__ hchg, double *__restrict__ out) {
+                                                  double *__restrict__ hchg, double *__restrict__ out,
+                                                  const double *__restrict__ old_par, int rollback) {
   __shared__ double lds[32];
+  __shared__ int s_rollback;
   if (state[0]) return;
   const int iter = state[1];
   double e = 0.0, dummy = 0.0;
@@ -1239,8 +1277,20 @@ static __global__ __launch_bounds__(256) void k_finalize(int *__restrict__ state
     out[3] = (double)any;
     const int n = iter + 1;
     state[1] = n;
-    if (err != 0 || n >= max_iter || (n >= min_stop && ch < thr)) state[0] = 1;
+    s_rollback = 0;
+    if (err != 0 || n >= max_iter || (n >= min_stop && ch < thr)) {
+      state[0] = 1;
+      // the iteration that ends a stage with rollback is discarded: the
+      // next stage redoes it from the parameters it started from
+      if (rollback && err == 0) {
+        state[1] = iter;
+        s_rollback = 1;
+      }
+    }
   }
+  __syncthreads();
+  if (s_rollback)
+    for (int i = threadIdx.x; i < npar; i += blockDim.x) par[i] = old_par[i];
 }
 
 // ---------------------------------------------------------------------------
@@ -1388,14 +1438,17 @@ static __global__ void k_slater_sum(int natom, const double *__restrict__ cx, co
 // ---------------------------------------------------------------------------
 
 // k_iter_begin for every molecule in flight: one workgroup per slot.
-static __global__ __launch_bounds__(256) void k_batch_begin(const HpSlot *__restrict__ slots, const int *__restrict__ slot_list,
+static __global__ __launch_bounds__(256) void k_batch_begin(HpSlot *__restrict__ slots, const int *__restrict__ slot_list,
                                                      const double *__restrict__ par_m, double *__restrict__ old_m,
                                                      const double *__restrict__ par_l, double *__restrict__ old_l,
                                                      const double *__restrict__ norm,
                                                      const double *__restrict__ alpha_fixed, double *__restrict__ coef,
                                                      double *__restrict__ alpha) {
-  const HpSlot &sl = slots[slot_list[blockIdx.x]];
+  HpSlot &sl = slots[slot_list[blockIdx.x]];
   if (sl.run == HP_RUN_IDLE) return;
+  __syncthreads();  // every thread has read run before thread 0 writes the slot
+  if (threadIdx.x == 0)
+    sl.refresh = sl.run == HP_RUN_FP32 && sl.corr_period > 0 && sl.niter % sl.corr_period == 0;
   for (int i = threadIdx.x; i < sl.nf; i += blockDim.x) {
     const int f = sl.f0 + i;
     if (sl.scheme == 0) {
@@ -1419,11 +1472,14 @@ static __global__ __launch_bounds__(256) void k_batch_begin(const HpSlot *__rest
 static __global__ __launch_bounds__(256) void k_batch_finalize(HpSlot *__restrict__ slots, const int *__restrict__ slot_list,
                                                         const double *__restrict__ ent_part,
                                                         const double *__restrict__ change_part,
-                                                        const int *__restrict__ status, int *__restrict__ status_all) {
+                                                        const int *__restrict__ status, int *__restrict__ status_all,
+                                                        double *__restrict__ par_m, const double *__restrict__ old_m,
+                                                        double *__restrict__ par_l, const double *__restrict__ old_l) {
   __shared__ double lds[32];
+  __shared__ int s_rollback;
   HpSlot &sl = slots[slot_list[blockIdx.x]];
   if (sl.run == HP_RUN_IDLE) return;
-  const int s0 = sl.s0, ns = sl.ns, a0 = sl.a0, na = sl.na;
+  const int s0 = sl.s0, ns = sl.ns, a0 = sl.a0, na = sl.na, f0 = sl.f0, nf = sl.nf, scheme = sl.scheme;
   double e = 0.0, dummy = 0.0;
   for (int s = threadIdx.x; s < ns; s += blockDim.x) e += ent_part[s0 + s];
   block_sum2(e, dummy, lds);
@@ -1443,20 +1499,38 @@ static __global__ __launch_bounds__(256) void k_batch_finalize(HpSlot *__restric
     sl.entropy = e;
     sl.err = err;
     sl.any |= any;
+    s_rollback = 0;
     if (err != 0) {
       sl.event = HP_EV_ERROR;
       sl.run = HP_RUN_IDLE;
     } else if (n >= sl.max_iter || (n >= sl.min_stop && ch < sl.thr)) {
       if (sl.run == HP_RUN_FP32) {
+        // end of the FP32 stage: this iteration is redone in FP64, so the
+        // FP64 stage continues from the parameters it started from
+        s_rollback = 1;
+        const int m1 = n - 1;
         const int np = max(1, sl.n_polish);
-        sl.min_stop = n + np;
-        sl.max_iter = max(sl.maxiter, n + np);
+        sl.niter = m1;
+        sl.min_stop = m1 + np;
+        sl.max_iter = max(sl.maxiter, m1 + np);
         sl.thr = sl.threshold;
         sl.event = HP_EV_SWITCH;
       } else {
         sl.event = (n >= sl.min_stop && ch < sl.thr) ? HP_EV_CONVERGED : HP_EV_MAXITER;
       }
       sl.run = HP_RUN_IDLE;
+    }
+  }
+  __syncthreads();
+  if (s_rollback) {
+    for (int i = threadIdx.x; i < nf; i += blockDim.x) {
+      const int f = f0 + i;
+      if (scheme == 0) {
+        par_m[2 * f] = old_m[2 * f];
+        par_m[2 * f + 1] = old_m[2 * f + 1];
+      } else {
+        par_l[f] = old_l[f];
+      }
     }
   }
 }

@@ -285,7 +285,8 @@ __global__ __launch_bounds__(128) void k_gt2_update(const int *__restrict__ done
                                                     const double *__restrict__ qpP, const int64_t *__restrict__ qp_off,
                                                     const double *__restrict__ partial, int scheme, int solver,
                                                     double thr, int maxit, double neg_cutoff, double pop_cutoff,
-                                                    double tau, int nlin, int *__restrict__ active) {
+                                                    double tau, int nlin, int *__restrict__ active,
+                                                    int *__restrict__ nhess) {
   constexpr int KM = HP_GT2_KM, NS = HP_GT2_NS, NV = HP_GT2_NV;
   __shared__ double v[NV];
   __shared__ double M[(HP_KMAX + 1) * (HP_KMAX + 2)];
@@ -317,6 +318,7 @@ __global__ __launch_bounds__(128) void k_gt2_update(const int *__restrict__ done
   __syncthreads();
   if (threadIdx.x != 0) return;
   const int K = fn_off[a + 1] - fn_off[a];
+  if (newton && phase == 0) nhess[a] += 1;  // this round's pass formed the Hessian
   bool keep = true;
   if (phase == 2) {
     S.change = v[3];

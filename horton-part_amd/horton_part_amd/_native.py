@@ -35,6 +35,7 @@ ST_NONFINITE = 512
 
 KERNEL_WEIGHTS = 0
 KERNEL_INNER = 1
+KERNEL_LISA_GRID = 2
 
 EXPORTED_SYMBOLS = [
     "hp_abi_version",
@@ -52,6 +53,7 @@ EXPORTED_SYMBOLS = [
     "hp_part_promoldens",
     "hp_part_moments",
     "hp_part_kernel_time",
+    "hp_part_get_work",
     "hp_batch_create",
     "hp_batch_destroy",
     "hp_batch_submit",
@@ -119,6 +121,17 @@ class SchemeDesc(ctypes.Structure):
         ("newton_linspace", ctypes.c_int),
         ("check_mono", ctypes.c_int),
         ("mixed_switch", _d),
+        ("mixed_refresh", ctypes.c_int),
+    ]
+
+
+class PartWork(ctypes.Structure):
+    _fields_ = [
+        ("points", ctypes.c_int64),
+        ("evals_per_pass", _d),
+        ("bytes_per_pass", _d),
+        ("hessian_passes", ctypes.c_int64),
+        ("hessian_flops", _d),
     ]
 
 
@@ -215,6 +228,7 @@ def load():
     lib.hp_part_promoldens.argtypes = [ctypes.c_void_p, _pd, ctypes.c_int64]
     lib.hp_part_moments.argtypes = [ctypes.c_void_p, ctypes.c_int, _pd, _pd, _pd]
     lib.hp_part_kernel_time.argtypes = [ctypes.c_void_p, ctypes.c_int, _pd, _pi64]
+    lib.hp_part_get_work.argtypes = [ctypes.c_void_p, _P(PartWork)]
     lib.hp_batch_create.argtypes = [ctypes.c_int, _P(BatchConfig), _P(ctypes.c_void_p)]
     lib.hp_batch_destroy.argtypes = [ctypes.c_void_p]
     lib.hp_batch_submit.argtypes = [ctypes.c_void_p, _P(MoleculeDesc), _P(SchemeDesc), _pd, ctypes.c_int, ctypes.c_int, _pd, _pd, _pi64]
@@ -380,6 +394,12 @@ class DevicePartition:
         rad = np.empty((self.natom, 4))
         check(self.lib.hp_part_moments(self.handle, int(lmax), ptr(cart), ptr(pure), ptr(rad)))
         return cart, pure, rad
+
+    def work(self):
+        """Algorithmic work counts (``hp_part_get_work``) as a dict."""
+        w = PartWork()
+        check(self.lib.hp_part_get_work(self.handle, ctypes.byref(w)))
+        return {name: getattr(w, name) for name, _ in w._fields_}
 
     def kernel_time(self, kernel=KERNEL_WEIGHTS):
         ms = ctypes.c_double()

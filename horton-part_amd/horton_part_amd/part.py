@@ -261,9 +261,13 @@ class AbstractISAWPart(WPart):
     Extra keyword arguments (defaults reproduce the reference):
       precision: "fp64" (default) or "mixed" (FP32 grid arithmetic, then FP64 polish)
       n_polish: minimum FP64 iterations after the mixed stage (default 1)
-      mixed_switch: the mixed stage ends when the change is below
-        max(threshold, mixed_switch) (default 1e-5); FP64 iterations then run
-        until the change is below threshold
+      mixed_switch: the FP32 stage ends when the change is below
+        max(1.01 threshold, mixed_switch) (default 1e-5); its last iteration
+        is redone in FP64 and FP64 iterations run until the change is below
+        threshold (grid_type 2: max(threshold, mixed_switch))
+      mixed_refresh: every this many FP32-stage iterations one runs in FP64
+        and refreshes the per-shell correction FP64 - FP32 of the spherical
+        averages (default 0: no correction)
       cutoff_radius: pro-atom tails beyond this distance are zero (default None)
       device: GPU ordinal (default 0)
     """
@@ -288,6 +292,7 @@ class AbstractISAWPart(WPart):
         precision="fp64",
         n_polish=1,
         mixed_switch=1e-5,
+        mixed_refresh=0,
         cutoff_radius=None,
         device=0,
         **kwargs,
@@ -300,6 +305,7 @@ class AbstractISAWPart(WPart):
         self._precision = precision
         self._n_polish = n_polish
         self._mixed_switch = mixed_switch
+        self._mixed_refresh = mixed_refresh
         self._cutoff_radius = cutoff_radius
         self._device = device
         self._dev = None
@@ -366,6 +372,7 @@ class AbstractISAWPart(WPart):
             "newton_linspace": 20,
             "check_mono": 0,
             "mixed_switch": float(self._mixed_switch),
+            "mixed_refresh": int(self._mixed_refresh),
         }
 
     def _init_propars(self):

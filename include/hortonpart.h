@@ -102,9 +102,15 @@ typedef struct {
   double newton_tau;         /* m-newton line-search start (alisa.py:772) */
   int newton_linspace;       /* m-newton line-search size (alisa.py:773) */
   int check_mono;            /* alisa.py:774 */
-  double mixed_switch;       /* HP_PREC_MIXED: leave the mixed stage once the
-                                change is below max(threshold, mixed_switch);
-                                <= 0 selects 1e-5 */
+  double mixed_switch;       /* HP_PREC_MIXED: leave the FP32 stage once the
+                                change is below max(1.01 threshold, mixed_switch)
+                                (grid_type 2: max(threshold, mixed_switch), where
+                                <= 0 selects 1e-5); the Python classes pass 1e-5 */
+  int mixed_refresh;         /* HP_PREC_MIXED, grid_type 1: every this many
+                                FP32-stage iterations one runs in FP64 and
+                                refreshes the per-shell correction FP64 - FP32
+                                that the FP32 iterations add to their spherical
+                                averages; <= 0: no correction */
 } hp_scheme_desc;
 
 typedef struct hp_part hp_part;
@@ -138,9 +144,11 @@ int hp_part_set_propars(hp_part *part, const double *propars, int64_t n);
  * gisa.py:251-318, alisa.py:1304-1335), the entropy (stockholder.py:145-151)
  * and compute_change (iterstock.py:32-45).  precision: HP_PREC_*.  With
  * HP_PREC_MIXED the loop runs in mixed precision until the change is below
- * max(threshold, mixed_switch), then in FP64 (at least n_polish iterations)
- * until the change is below threshold; maxiter caps the total number of
- * iterations unless the polish needs more.  Returns the number of
+ * max(1.01 threshold, mixed_switch) (FP32 bulk with the defect correction
+ * of mixed_refresh); the iteration that reaches it is discarded and redone in
+ * FP64, and FP64 iterations (at least n_polish) follow until the change is
+ * below threshold; maxiter caps the total number of iterations unless the
+ * polish needs more.  Returns the number of
  * outer iterations through *niter.  HP_E_SOLVER if a per-atom solver raised;
  * the loop stops at that iteration. */
 int hp_part_run(hp_part *part, int precision, int n_polish, int *niter);
@@ -178,6 +186,21 @@ int hp_part_moments(hp_part *part, int lmax, double *cartesian, double *pure, do
 /* Timing of the pro-atom/weights kernel over the last hp_part_run, measured
  * with HIP events on the library's stream: total milliseconds and launches. */
 int hp_part_kernel_time(hp_part *part, int kernel, double *total_ms, int64_t *launches);
+
+/* Algorithmic work of a partitioning, from its inputs (for rooflines). */
+typedef struct {
+  int64_t points;          /* grid points read by one weights pass */
+  double evals_per_pass;   /* pro-atom function evaluations of one weights pass
+                              (points x functions of the atoms that reach them) */
+  double bytes_per_pass;   /* HBM bytes one weights pass must move: 48 per point
+                              (x, y, z, density, angular and molecular weights)
+                              + 16 per radial shell written */
+  int64_t hessian_passes;  /* grid_type 2 Newton solvers, last hp_part_run: atom
+                              passes over the grid that formed B diag(w) B^T */
+  double hessian_flops;    /* ... their 2 K_a^2 N flops in total */
+} hp_part_work;
+
+int hp_part_get_work(hp_part *part, hp_part_work *out);
 
 #define HP_KERNEL_WEIGHTS 0 /* fused pro-atom/promolecule/weights/sphavg pass */
 #define HP_KERNEL_INNER 1   /* per-atom inner solvers */

@@ -62,8 +62,8 @@ def _part(scheme, wname, **kw):
     return cls(w.coordinates, w.numbers, w.pseudo_numbers, w.grid, w.density, **kw)
 
 
-def _single(scheme, wname, precision):
-    part = _part(scheme, wname, precision=precision)
+def _single(scheme, wname, precision, **kw):
+    part = _part(scheme, wname, precision=precision, **kw)
     part.do_partitioning()
     out = (part["charges"].copy(), part["propars"].copy(), int(part["niter"]))
     part.close()
@@ -82,13 +82,16 @@ def _server(**kw):
 
 
 @pytest.mark.parametrize("scheme", ["mbis", "gisa"])
-@pytest.mark.parametrize("precision", ["fp64", "mixed"])
-def test_batch_alone_equals_single(scheme, precision):
+@pytest.mark.parametrize("precision,refresh", [("fp64", 0), ("mixed", 0), ("mixed", 3)])
+def test_batch_alone_equals_single(scheme, precision, refresh):
+    """refresh > 0: the FP32 stage with the FP64 defect correction every
+    `refresh` iterations (FP64 and calibrating FP32 passes of a molecule in
+    one tick)."""
     require_gpu()
-    _say(f"case: batch alone vs hp_part_run, {scheme} {precision} chain5")
-    q1, p1, n1 = _single(scheme, "chain5", precision)
+    _say(f"case: batch alone vs hp_part_run, {scheme} {precision} refresh {refresh} chain5")
+    q1, p1, n1 = _single(scheme, "chain5", precision, mixed_refresh=refresh)
     with _server() as srv:
-        srv.submit(_part(scheme, "chain5"), precision=precision)
+        srv.submit(_part(scheme, "chain5", mixed_refresh=refresh), precision=precision)
         (done,) = srv.run_until_done(timeout=120)
     dq = np.max(np.abs(done.charges - q1))
     dp = np.max(np.abs(done.propars - p1))
@@ -177,6 +180,12 @@ def test_batch_against_oracle(scheme):
     assert dq64 <= 1e-10
     assert rel <= 1e-9
     assert dq32 <= 1e-6
+    # the FP32 stage with the defect correction, in the single-molecule path
+    part = _part(scheme, "water", precision="mixed", mixed_refresh=4)
+    part.do_partitioning()
+    dqc = np.max(np.abs(part["charges"] - cache["charges"]))
+    _say(f"  mixed with FP64 correction every 4 iterations: niter={part['niter']} |dq|={dqc:.1e}")
+    assert dqc <= 1e-6
 
 
 def test_batch_rejects_what_it_cannot_run():
