@@ -21,6 +21,9 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#define HP_HD __host__ __device__
+#include "hp_exp_tab.h"
+
 #define HP_KMAX 16      // functions per atom handled by the per-atom kernels
 
 // per-atom status bits (same values as HP_ST_* in include/hortonpart.h)
@@ -346,6 +349,12 @@ inline __host__ int weights_tile_size(int km, int natom_max) {
   return tile < 1 ? 1 : (tile < natom_max ? tile : (natom_max < 1 ? 1 : natom_max));
 }
 
+// FP64 exp of the weights pass: Tang's table method (hp_exp_tab.h), with the
+// 64 table values staged in LDS by each workgroup.
+static __constant__ double hp_exp2_64[64] = HP_EXP2_64_VALUES;
+
+__device__ __forceinline__ double exp_tab(double x, const double *__restrict__ t64) { return hp_exp_tab_core(x, t64); }
+
 template <typename R>
 __device__ __forceinline__ R fast_sqrt(R x);
 template <>
@@ -353,16 +362,22 @@ __device__ __forceinline__ double fast_sqrt<double>(double x) { return sqrt(x); 
 template <>
 __device__ __forceinline__ float fast_sqrt<float>(float x) { return __builtin_amdgcn_sqrtf(x); }
 
-// exp of -(alpha * arg): FP64 exp(); FP32 exp2 with alpha pre-scaled by log2(e)
+// exp of -(alpha * arg): FP64 by the table method; FP32 exp2 with alpha
+// pre-scaled by log2(e)
 template <typename R>
-__device__ __forceinline__ R neg_exp_scaled(R al, R arg);
+__device__ __forceinline__ R neg_exp_scaled(R al, R arg, const double *t64);
 template <>
-__device__ __forceinline__ double neg_exp_scaled<double>(double al, double arg) { return exp(-al * arg); }
+__device__ __forceinline__ double neg_exp_scaled<double>(double al, double arg, const double *t64) {
+  return exp_tab(-al * arg, t64);
+}
 template <>
-__device__ __forceinline__ float neg_exp_scaled<float>(float al, float arg) { return __builtin_amdgcn_exp2f(-al * arg); }
+__device__ __forceinline__ float neg_exp_scaled<float>(float al, float arg, const double *) {
+  return __builtin_amdgcn_exp2f(-al * arg);
+}
 
 template <typename R, int FK, int PT, int KM, bool CULL>
-__device__ __forceinline__ void weights_tile(const R *__restrict__ rec, const int *__restrict__ s_idx, int tn, int own_i,
+__device__ __forceinline__ void weights_tile(const R *__restrict__ rec, const int *__restrict__ s_idx,
+                                             const double *__restrict__ t64, int tn, int own_i,
                                              const DevFn &fn, R rc, const R (&qx)[PT], const R (&qy)[PT],
                                              const R (&qz)[PT], const double (&px)[PT], const double (&py)[PT],
                                              const double (&pz)[PT], bool two, R (&promol)[PT], R (&own)[PT]) {
@@ -402,7 +417,7 @@ __device__ __forceinline__ void weights_tile(const R *__restrict__ rec, const in
             const double n = fn.order[fn.fn_off[s_idx[i]] + k];
             arg = (n == 1.0) ? r : ((n == 2.0) ? r2 : (R)pow((double)r, n));
           }
-          pb += v[4 + k] * neg_exp_scaled<R>(v[4 + KM + k], arg);
+          pb += v[4 + k] * neg_exp_scaled<R>(v[4 + KM + k], arg, t64);
         }
       }
       if (CULL && r > rc) pb = (R)0;
@@ -434,6 +449,7 @@ __global__ __launch_bounds__(512) void k_weights(const int *__restrict__ done, D
   extern __shared__ __align__(16) double smem_d[];
   __shared__ double lds[32];
   __shared__ int s_own;  // list position of the shell's own atom
+  __shared__ double s_t64[64];  // 2^(j/64) for the FP64 exponential
   if (done && *done) return;
   constexpr int RS = weights_record<R>(KM);
   R *s_rec = reinterpret_cast<R *>(smem_d);
@@ -489,6 +505,7 @@ __global__ __launch_bounds__(512) void k_weights(const int *__restrict__ done, D
   const bool two = p0 + blockDim.x < p1;  // second point slot used by any thread
   constexpr double log2e = 1.4426950408889634;
   if (threadIdx.x == 0) s_own = -1;
+  if (sizeof(R) == sizeof(double) && threadIdx.x < 64) s_t64[threadIdx.x] = hp_exp2_64[threadIdx.x];
   for (int t0 = 0; t0 < nb_n; t0 += tile) {
     const int tn = min(tile, nb_n - t0);
     __syncthreads();
@@ -519,9 +536,11 @@ __global__ __launch_bounds__(512) void k_weights(const int *__restrict__ done, D
     __syncthreads();
     const int own_i = s_own - t0;  // outside [0, tn) when not in this tile
     if (cull)
-      weights_tile<R, FK, PT, KM, true>(s_rec, s_idx, tn, own_i, fn, rc, qx, qy, qz, px, py, pz, two, promol, own);
+      weights_tile<R, FK, PT, KM, true>(s_rec, s_idx, s_t64, tn, own_i, fn, rc, qx, qy, qz, px, py, pz, two, promol,
+                                        own);
     else
-      weights_tile<R, FK, PT, KM, false>(s_rec, s_idx, tn, own_i, fn, rc, qx, qy, qz, px, py, pz, two, promol, own);
+      weights_tile<R, FK, PT, KM, false>(s_rec, s_idx, s_t64, tn, own_i, fn, rc, qx, qy, qz, px, py, pz, two, promol,
+                                         own);
   }
   double acc_avg = 0.0, acc_ent = 0.0;
 #pragma unroll
@@ -1626,6 +1645,9 @@ static __global__ __launch_bounds__(256) void k_copy16(const double2 *__restrict
 // iters x 8 independent exponentials per thread, arguments in [-30, 0]
 template <typename R>
 __global__ __launch_bounds__(256) void k_exp_rate(int iters, double *__restrict__ out) {
+  __shared__ double s_t64[64];
+  if (threadIdx.x < 64) s_t64[threadIdx.x] = hp_exp2_64[threadIdx.x];
+  __syncthreads();
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
   R x[HP_EXP_RATE_CHAINS], acc[HP_EXP_RATE_CHAINS];
 #pragma unroll
@@ -1637,7 +1659,7 @@ __global__ __launch_bounds__(256) void k_exp_rate(int iters, double *__restrict_
   for (int i = 0; i < iters; ++i) {
 #pragma unroll
     for (int c = 0; c < HP_EXP_RATE_CHAINS; ++c) {
-      acc[c] += hp_exp<R>(x[c]);
+      acc[c] += neg_exp_scaled<R>((R)-1, x[c], s_t64);
       x[c] -= step;
     }
   }

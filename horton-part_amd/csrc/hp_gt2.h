@@ -63,6 +63,11 @@ __device__ __forceinline__ void gt2_terms(bool mbis, int K, const double *par, c
   }
 }
 
+// Line-search candidate c + x delta, rounded after the product and after the
+// sum (as numpy's propars + x * delta, alisa.py:846-847): the line search and
+// the update that adopts its step compute the same coefficients bit for bit.
+__device__ __forceinline__ double gt2_step(double c, double x, double delta) { return __dadd_rn(c, __dmul_rn(x, delta)); }
+
 template <typename R>
 __device__ __forceinline__ R gt2_sum(bool mbis, int K, const R *T, const double *c) {
   R s = (R)0;
@@ -250,7 +255,7 @@ __global__ __launch_bounds__(256) void k_gt2_linesearch(const int *__restrict__ 
     const int t = threadIdx.x / KM, k = threadIdx.x % KM;
     const double step = (0.0 - tau) / (double)(nlin - 1);
     const double x = (t == nlin - 1) ? 0.0 : (double)t * step + tau;
-    cand[t][k] = S.c[k] + x * S.delta[k];
+    cand[t][k] = gt2_step(S.c[k], x, S.delta[k]);
   }
   __syncthreads();
   int bad = 0;
@@ -341,7 +346,7 @@ __global__ __launch_bounds__(128) void k_gt2_update(const int *__restrict__ done
       const double x = (t_ok == nlin - 1) ? 0.0 : (double)t_ok * step + tau;
       for (int k = 0; k < K; ++k) {
         S.cprev[k] = S.c[k];
-        S.c[k] = S.c[k] + x * S.delta[k];
+        S.c[k] = gt2_step(S.c[k], x, S.delta[k]);
       }
       S.have_old = 1;
       S.phase = 0;
