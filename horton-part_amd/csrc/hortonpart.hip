@@ -400,12 +400,9 @@ int gt2_iteration(hp_part *p, int precision, double thr, int min_stop, int max_i
       HP_LAUNCHED();
       HP_HIP(hipEventRecord(p->slot_ev[c][1], s));
       if (p->solver == HP_SOLVER_MNEWTON) {
-        if (precision == HP_PREC_MIXED)
-          hipLaunchKernelGGL(k_gt2_linesearch<float>, grid, block, 0, s, p->state, p->mol(), p->gt2, p->fn_off,
-                             p->fn_norm, p->fn_alpha, p->fn_order, p->tau, p->nlin, p->neg_cutoff, p->gt2_partial);
-        else
-          hipLaunchKernelGGL(k_gt2_linesearch<double>, grid, block, 0, s, p->state, p->mol(), p->gt2, p->fn_off,
-                             p->fn_norm, p->fn_alpha, p->fn_order, p->tau, p->nlin, p->neg_cutoff, p->gt2_partial);
+        // FP64 in both precisions: it tests the same pro-atom values as the pass
+        hipLaunchKernelGGL(k_gt2_linesearch<double>, grid, block, 0, s, p->state, p->mol(), p->gt2, p->fn_off,
+                           p->fn_norm, p->fn_alpha, p->fn_order, p->tau, p->nlin, p->neg_cutoff, p->gt2_partial);
         HP_LAUNCHED();
       }
       hipLaunchKernelGGL(k_gt2_update, dim3(p->natom), dim3(128), 0, s, p->state, p->gt2_nchunk, p->gt2, p->fn_off,

@@ -357,8 +357,22 @@ __device__ __forceinline__ double exp_tab(double x, const double *__restrict__ t
 
 template <typename R>
 __device__ __forceinline__ R fast_sqrt(R x);
+// FP64: v_rsq_f64 refined by the Newton-Raphson steps of the correctly
+// rounded sqrt, without its rescaling of inputs below 2^-767 (a squared
+// distance is either 0 or far above that); 0 gives 0.
 template <>
-__device__ __forceinline__ double fast_sqrt<double>(double x) { return sqrt(x); }
+__device__ __forceinline__ double fast_sqrt<double>(double x) {
+  const double y = __builtin_amdgcn_rsq(x);
+  double s = x * y, h = 0.5 * y;
+  const double e = fma(-h, s, 0.5);
+  s = fma(s, e, s);
+  h = fma(h, e, h);
+  double d = fma(-s, s, x);
+  s = fma(d, h, s);
+  d = fma(-s, s, x);
+  s = fma(d, h, s);
+  return x > 0.0 ? s : 0.0;
+}
 template <>
 __device__ __forceinline__ float fast_sqrt<float>(float x) { return __builtin_amdgcn_sqrtf(x); }
 

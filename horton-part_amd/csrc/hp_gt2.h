@@ -17,6 +17,7 @@
 //   phase 2  outer change: sum_p w_p (rho0_a,new - rho0_a,old)^2
 //   phase 3  done
 #pragma once
+#include <type_traits>
 #include "hp_device.h"
 
 typedef double hp_d4 __attribute__((ext_vector_type(4)));
@@ -68,12 +69,14 @@ __device__ __forceinline__ void gt2_terms(bool mbis, int K, const double *par, c
 // the update that adopts its step compute the same coefficients bit for bit.
 __device__ __forceinline__ double gt2_step(double c, double x, double delta) { return __dadd_rn(c, __dmul_rn(x, delta)); }
 
+// Pro-atom value from its terms, summed in FP64 (the same arithmetic as the
+// line search's candidates, whatever precision the terms are in).
 template <typename R>
-__device__ __forceinline__ R gt2_sum(bool mbis, int K, const R *T, const double *c) {
-  R s = (R)0;
+__device__ __forceinline__ double gt2_sum(bool mbis, int K, const R *T, const double *c) {
+  double s = 0.0;
 #pragma unroll
   for (int k = 0; k < HP_GT2_KM; ++k)
-    if (k < K) s += mbis ? T[k] : T[k] * (R)c[k];
+    if (k < K) s += mbis ? (double)T[k] : (double)T[k] * c[k];
   return s;
 }
 
@@ -89,6 +92,10 @@ __global__ __launch_bounds__(256) void k_gt2_pass(const int *__restrict__ done, 
                                                   const double *__restrict__ fn_order, int scheme, int solver,
                                                   double dc, double *__restrict__ partial) {
   constexpr int KM = HP_GT2_KM, NS = HP_GT2_NS, NV = HP_GT2_NV;
+  // Newton: the basis terms that give the pro-atom (its negativity test, the
+  // line search's candidates, the ratios) are FP64 in both precisions; R is
+  // the precision of the MFMA inputs and accumulation of the Hessian
+  using TR = typename std::conditional<KIND == 2, double, R>::type;
   __shared__ R s_g[4][KM][64];
   __shared__ R s_b[4][KM][64];
   __shared__ double s_red[4][NV];
@@ -117,20 +124,20 @@ __global__ __launch_bounds__(256) void k_gt2_pass(const int *__restrict__ done, 
     const bool ok = p < p_end;
     const int64_t pp = ok ? p : p_begin;
     const double dx = m.px[pp] - ax, dy = m.py[pp] - ay, dz = m.pz[pp] - az;
-    const R r = (R)sqrt(dx * dx + dy * dy + dz * dz);
+    const TR r = (TR)sqrt(dx * dx + dy * dy + dz * dz);
     const double wmol = ok ? m.wmol[pp] : 0.0;
     const double rho = ok ? m.rho[pp] : 0.0;
     const double pm = promol[pp];
-    R T[KM], Told[KM];
-    gt2_terms<R>(mbis, K, S.cold, norm, alpha, order, r, Told);
-    const R proold = gt2_sum<R>(mbis, K, Told, S.cold);
+    TR T[KM], Told[KM];
+    gt2_terms<TR>(mbis, K, S.cold, norm, alpha, order, r, Told);
+    const double proold = gt2_sum<TR>(mbis, K, Told, S.cold);
     if (mbis) {
-      gt2_terms<R>(true, K, S.c, norm, alpha, order, r, T);
+      gt2_terms<TR>(true, K, S.c, norm, alpha, order, r, T);
     } else {
 #pragma unroll
       for (int k = 0; k < KM; ++k) T[k] = Told[k];  // fixed basis
     }
-    const double pro = (double)gt2_sum<R>(mbis, K, T, S.c);
+    const double pro = gt2_sum<TR>(mbis, K, T, S.c);
     if (phase == 2) {
       const double d = pro - (double)proold;
       acc[3] += wmol * d * d;
@@ -143,11 +150,11 @@ __global__ __launch_bounds__(256) void k_gt2_pass(const int *__restrict__ done, 
     if (have_old) {
       double prev;
       if (mbis) {
-        R Tp[KM];
-        gt2_terms<R>(true, K, S.cprev, norm, alpha, order, r, Tp);
-        prev = (double)gt2_sum<R>(true, K, Tp, S.cprev);
+        TR Tp[KM];
+        gt2_terms<TR>(true, K, S.cprev, norm, alpha, order, r, Tp);
+        prev = gt2_sum<TR>(true, K, Tp, S.cprev);
       } else {
-        prev = (double)gt2_sum<R>(false, K, T, S.cprev);
+        prev = gt2_sum<TR>(false, K, T, S.cprev);
       }
       const double e = prev - pro;
       acc[0] += wmol * e * e;
@@ -173,7 +180,7 @@ __global__ __launch_bounds__(256) void k_gt2_pass(const int *__restrict__ done, 
         const double integ = (double)T[k] * ratio;
         acc[4 + k] += integ * wmol;
         s_g[wv][k][lane] = sick ? (R)0 : (R)(integ / pro * wmol);
-        s_b[wv][k][lane] = T[k];
+        s_b[wv][k][lane] = (R)T[k];
       }
     }
     if (newton) {
@@ -194,8 +201,11 @@ __global__ __launch_bounds__(256) void k_gt2_pass(const int *__restrict__ done, 
       __syncthreads();
     }
   }
+  // slot 2 is a minimum (the pro-atom's smallest value), the others sums
+  const double lane_min = acc[2];
+  acc[2] = 0.0;
   wave_sum_n<NS>(acc);
-  acc[2] = wave_min(acc[2]);
+  acc[2] = wave_min(lane_min);
   if (lane == 0) {
     for (int q = 0; q < NS; ++q) s_red[wv][q] = acc[q];
   }

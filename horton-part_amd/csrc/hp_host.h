@@ -26,14 +26,11 @@ int dalloc(T **p, size_t n) {
   return 0;
 }
 
-// Points per thread of the weights pass: 1 or 2, whichever idles fewer lanes
-// for the largest shell (ties go to 2, which halves the LDS reads per point).
-inline int weights_pt(int maxL) {
-  auto lanes = [](int n) { return ((n + 63) / 64) * 64; };
-  const double e1 = (double)maxL / lanes(maxL);
-  const double e2 = (double)maxL / (2.0 * lanes((maxL + 1) / 2));
-  return (maxL > 512 || e2 >= e1) ? 2 : 1;
-}
+// Points per thread of the weights pass.  Every atom record is a broadcast
+// LDS read shared by a thread's points: with one point per thread the reads
+// of the four SIMDs nearly saturate the CU's LDS, so shells of more than one
+// wavefront of points take two points per thread.
+inline int weights_pt(int maxL) { return maxL > 64 ? 2 : 1; }
 
 // Workgroup size of the weights pass for a molecule whose largest radial
 // shell has maxL points.

@@ -257,6 +257,19 @@ def test_gt2_mbis():
         _compare(part, cache, oracle, qtol=1e-10, ptol=1e-8)
 
 
+def test_gt2_lisa_cluster_multichunk():
+    """grid_type 2 Newton on a cluster whose molecular grid spans many point
+    chunks, with far-field pro-atom values near zero: the smallest pro-atom
+    value (negativity check, alisa.py:889-897) is a minimum over all chunks."""
+    require_gpu()
+    _say("case: grid_type 2 LISA m-newton slater6, cluster 12 atoms seed 5, 30x50 (18,000 points)")
+    w = workloads.cluster(natom=12, seed=5, nrad=30, nang=50)
+    part, oracle = _lisa_pair(w, "slater6", "m-newton", maxiter=30, grid_type=2)
+    cache = _run_pair(part, oracle)
+    if cache is not None:
+        _compare(part, cache, oracle, qtol=1e-10, ptol=1e-8)
+
+
 def test_gt2_lisa_mixed_mfma_fp32():
     """Mixed path on grid_type 2: FP32 pro-atoms and FP32 MFMA Hessian, FP64
     solves, then FP64 polish; charges within 1e-6 e of the FP64 oracle."""
