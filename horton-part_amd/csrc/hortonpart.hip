@@ -81,6 +81,9 @@ struct hp_part {
   hipEvent_t slot_ev[kSlots][3] = {};
   double mixed_switch = 0.0;  // > 0: leave the FP32 stage below this change
   int mixed_refresh = 0;      // FP64 refresh of the FP32 correction every this many iterations
+  int tail_cull = 1;          // weights pass may skip atoms bounded below one rounding
+  int params_nonneg = 1;      // the parameters last set are all >= 0 (tail skipping needs it)
+  unsigned long long *evals_done = nullptr;  // [FP32, FP64] evaluations of the last run
   double *corr = nullptr;     // per radial shell: FP64 - FP32 spherical average
   // host state
   int niter = 0;
@@ -123,6 +126,9 @@ struct hp_part {
     f.shell_nb = shell_nb;
     f.nb_off = nullptr;
     f.rc = rc;
+    // MBIS and GISA updates keep non-negative coefficients non-negative
+    f.tail_cull = scheme != HP_SCHEME_LISA && fnkind != HP_FN_GENERAL && tail_cull && params_nonneg;
+    f.evals_done = evals_done;
     return f;
   }
 };
@@ -138,7 +144,7 @@ void free_part(hp_part *p) {
                   p->fn_order, p->coef,       p->alpha,   p->qpP,       p->qp_off,      p->par,      p->old_par,
                   p->rhobar,  p->ent_part,    p->promol,  p->charges,   p->change_part, p->out,      p->mom_part,
                   p->mom_out, p->scratch_pts, p->inner_niter, p->status, p->status_all, p->hpar,    p->hch,
-                  p->hent,    p->hchg,    p->corr};
+                  p->hent,    p->hchg,    p->corr,      p->evals_done};
   for (void *b : bufs)
     if (b) (void)hipFree(b);
   if (p->h_out) (void)hipHostFree(p->h_out);
@@ -580,6 +586,7 @@ int hp_part_create(int device, const hp_molecule_desc *md, const hp_scheme_desc 
   p->check_mono = sd->check_mono;
   p->mixed_switch = sd->mixed_switch > 0.0 ? sd->mixed_switch : 0.0;
   p->mixed_refresh = sd->mixed_refresh > 0 ? sd->mixed_refresh : 0;
+  p->tail_cull = sd->full_sum ? 0 : 1;
   p->maxiter = sd->maxiter;
   p->inner_maxiter = sd->inner_maxiter;
   p->h_shell_off.assign(md->shell_offsets, md->shell_offsets + S + 1);
@@ -711,7 +718,8 @@ int hp_part_create(int device, const hp_molecule_desc *md, const hp_scheme_desc 
       (rc = dalloc(&p->charges, A)) || (rc = dalloc(&p->change_part, A)) || (rc = dalloc(&p->out, 4)) ||
       (rc = dalloc(&p->inner_niter, A)) || (rc = dalloc(&p->status, A)) || (rc = dalloc(&p->status_all, A)) ||
       (rc = dalloc(&p->hpar, (size_t)p->hist_cap * p->npar)) || (rc = dalloc(&p->hch, (size_t)p->hist_cap * A)) ||
-      (rc = dalloc(&p->hent, p->hist_cap)) || (rc = dalloc(&p->hchg, p->hist_cap)) || (rc = dalloc(&p->corr, S)))
+      (rc = dalloc(&p->hent, p->hist_cap)) || (rc = dalloc(&p->hchg, p->hist_cap)) || (rc = dalloc(&p->corr, S)) ||
+      (rc = dalloc(&p->evals_done, HP_EVAL_SLOTS)))
     return bail(rc);
   if (p->grid_type == 2) {
     p->gt2_nchunk = (int)((N + HP_GT2_CHUNK - 1) / HP_GT2_CHUNK);
@@ -742,6 +750,9 @@ int hp_part_destroy(hp_part *part) {
 int hp_part_set_propars(hp_part *p, const double *propars, int64_t n) {
   if (!p || !propars) return fail(HP_E_ARG, "NULL argument");
   if (n != p->npar) return fail(HP_E_ARG, "propars length mismatch");
+  bool nonneg = true;
+  for (int64_t k = 0; k < n; ++k) nonneg = nonneg && propars[k] >= 0.0;
+  p->params_nonneg = nonneg ? 1 : 0;
   HP_HIP(hipSetDevice(p->device));
   HP_HIP(hipMemcpyAsync(p->par, propars, sizeof(double) * n, hipMemcpyHostToDevice, p->stream));
   HP_HIP(hipMemcpyAsync(p->old_par, propars, sizeof(double) * n, hipMemcpyHostToDevice, p->stream));
@@ -764,6 +775,7 @@ int hp_part_run(hp_part *p, int precision, int n_polish, int *niter) {
   HP_HIP(hipMemsetAsync(p->status_all, 0, sizeof(int) * p->natom, p->stream));
   HP_HIP(hipMemsetAsync(p->state, 0, 2 * sizeof(int), p->stream));
   if (p->gt2_nhess) HP_HIP(hipMemsetAsync(p->gt2_nhess, 0, sizeof(int) * p->natom, p->stream));
+  HP_HIP(hipMemsetAsync(p->evals_done, 0, HP_EVAL_SLOTS * sizeof(unsigned long long), p->stream));
   p->niter = 0;
   int rc = 0;
   auto stage = [&](int prec, double thr, int min_stop, int max_iter) {
@@ -944,6 +956,7 @@ int hp_part_get_work(hp_part *p, hp_part_work *out) {
   out->bytes_per_pass = 48.0 * (double)p->npoint + 16.0 * (double)p->nshell;
   out->hessian_passes = 0;
   out->hessian_flops = 0.0;
+  if (int rc = hp::read_eval_counters(p->evals_done, &out->evals_done_fp32, &out->evals_done_fp64)) return rc;
   if (p->gt2_nhess) {
     std::vector<int> nh(p->natom);
     HP_HIP(hipMemcpy(nh.data(), p->gt2_nhess, sizeof(int) * p->natom, hipMemcpyDeviceToHost));

@@ -130,6 +130,7 @@ struct hp_batch {
   double *coef = nullptr, *alpha = nullptr, *fn_norm = nullptr, *fn_alpha = nullptr, *fn_order = nullptr;
   double *par_l = nullptr, *old_l = nullptr, *par_m = nullptr, *old_m = nullptr;
   double *qpP = nullptr;
+  unsigned long long *evals_done = nullptr;  // [FP32, FP64] evaluations performed
   // slots
   HpSlot *slots = nullptr, *h_slots = nullptr, *h_slot_init = nullptr;
   std::vector<int> slot_state;
@@ -201,6 +202,8 @@ struct hp_batch {
     f.shell_nb = nullptr;
     f.nb_off = nullptr;
     f.rc = 0.0;
+    f.tail_cull = 1;  // MBIS and GISA only; per request through HpSlot::full_sum
+    f.evals_done = evals_done;
     return f;
   }
 };
@@ -219,7 +222,8 @@ void free_batch(hp_batch *b) {
                   b->atom_shell_off, b->fn_off,  b->inner_niter, b->status,      b->status_all, b->qp_off,
                   b->coef,       b->alpha,       b->fn_norm,     b->fn_alpha,    b->fn_order,  b->par_l,
                   b->old_l,      b->par_m,       b->old_m,       b->qpP,         b->slots,     b->st_points,
-                  b->st_coords,  b->st_shell_atom, b->st_fn_off, b->st_shell_off, b->d_work,  b->d_slot_list};
+                  b->st_coords,  b->st_shell_atom, b->st_fn_off, b->st_shell_off, b->d_work,  b->d_slot_list,
+                  b->evals_done};
   for (void *p : bufs)
     if (p) (void)hipFree(p);
   if (b->h_slots) (void)hipHostFree(b->h_slots);
@@ -445,6 +449,11 @@ int upload(hp_batch *b, Request *r) {
   h.thr = r->precision == HP_PREC_MIXED ? std::max(r->sd.threshold * HP_MIXED_MARGIN, sw) : r->sd.threshold;
   h.corr_period = r->precision == HP_PREC_MIXED && r->sd.mixed_refresh > 0 ? r->sd.mixed_refresh : 0;
   h.refresh = 0;
+  // tail skipping needs non-negative coefficients: MBIS and GISA updates keep
+  // them so, from non-negative initial parameters
+  bool nonneg = true;
+  for (int64_t k = 0; k < r->npar; ++k) nonneg = nonneg && r->propars0[k] >= 0.0;
+  h.full_sum = (r->sd.full_sum || !nonneg) ? 1 : 0;
   r->stage = r->precision == HP_PREC_MIXED ? HP_RUN_FP32 : HP_RUN_FP64;
   r->corr_period = h.corr_period;
   HP_HIP(hipMemcpyAsync(b->slots + r->slot, &h, sizeof(HpSlot), hipMemcpyHostToDevice, s));
@@ -810,8 +819,11 @@ int hp_batch_create(int device, const hp_batch_config *cfg, hp_batch **out) {
       (rc = dalloc(&b->qpP, NQ)) || (rc = dalloc(&b->slots, MI)) || (rc = dalloc(&b->st_points, 3 * MP)) ||
       (rc = dalloc(&b->st_coords, 3 * (NA - 1))) || (rc = dalloc(&b->st_shell_atom, NS)) ||
       (rc = dalloc(&b->st_fn_off, NA)) || (rc = dalloc(&b->st_shell_off, NS)) ||
-      (rc = dalloc(&b->d_work, b->work_cap)) || (rc = dalloc(&b->d_slot_list, MI)))
+      (rc = dalloc(&b->d_work, b->work_cap)) || (rc = dalloc(&b->d_slot_list, MI)) ||
+      (rc = dalloc(&b->evals_done, HP_EVAL_SLOTS)))
     return bail(rc);
+  if ((e = hipMemset(b->evals_done, 0, HP_EVAL_SLOTS * sizeof(unsigned long long))) != hipSuccess)
+    return bail(fail(HP_E_HIP, std::string("hipMemset: ") + hipGetErrorString(e)));
   if ((e = hipHostMalloc((void **)&b->h_slots, MI * sizeof(HpSlot), hipHostMallocDefault)) != hipSuccess ||
       (e = hipHostMalloc((void **)&b->h_slot_init, MI * sizeof(HpSlot), hipHostMallocDefault)) != hipSuccess ||
       (e = hipHostMalloc((void **)&b->h_work, b->work_cap * sizeof(int2), hipHostMallocDefault)) != hipSuccess ||
@@ -1009,7 +1021,14 @@ int hp_host_free(void *ptr) {
 
 int hp_batch_get_stats(hp_batch *b, hp_batch_stats *out) {
   if (!b || !out) return fail(HP_E_ARG, "NULL argument");
+  HP_HIP(hipSetDevice(b->device));
+  HP_HIP(hipStreamSynchronize(b->cs1));
+  HP_HIP(hipStreamSynchronize(b->cs));
+  double d32 = 0.0, d64 = 0.0;
+  if (int rc = hp::read_eval_counters(b->evals_done, &d32, &d64)) return rc;
   *out = b->stats;
+  out->done_fp32 = d32;
+  out->done_fp64 = d64;
   return 0;
 }
 

@@ -75,7 +75,8 @@ struct HpSlot {
   int err, any;        // status bits: fatal ones of the last iteration, OR of all
   int corr_period;     // FP32 stage: FP64 refresh of the correction every this many iterations (0: none)
   int refresh;         // this iteration refreshes the correction (set by k_batch_begin)
-  int pad[2];
+  int full_sum;        // weights pass: every atom on every point (no tail skipping)
+  int pad;
   double thr;          // threshold of the current stage
   double threshold;    // the request's threshold
   double change, entropy;
@@ -109,7 +110,17 @@ struct DevFn {
   const int *shell_nb;   // nshell: candidate count of each shell (culling)
   const int64_t *nb_off; // per atom: start of its list (null: atom * natom)
   double rc;             // cutoff radius; <= 0 disables culling
+  int tail_cull;         // weights pass: skip atoms whose bounded contribution is
+                         // negligible (schemes with non-negative coefficients)
+  unsigned long long *evals_done;  // HP_EVAL_SLOTS counters per precision (may be null)
 };
+
+// Counters of performed function evaluations: 64 per precision, one per
+// 128-byte line, picked by workgroup, so that the workgroups of a launch do
+// not all add to one address (the host sums them).
+#define HP_EVAL_LANES 64
+#define HP_EVAL_STRIDE 16  // unsigned long longs per 128-byte line
+#define HP_EVAL_SLOTS (2 * HP_EVAL_LANES * HP_EVAL_STRIDE)
 
 // Function kinds of the promolecule pass.
 #define HP_FN_SLATER 0   // all functions exp(-a r)
@@ -389,19 +400,22 @@ __device__ __forceinline__ float neg_exp_scaled<float>(float al, float arg, cons
   return __builtin_amdgcn_exp2f(-al * arg);
 }
 
+// Returns the number of functions of the tile's atoms (uniform).
 template <typename R, int FK, int PT, int KM, bool CULL>
-__device__ __forceinline__ void weights_tile(const R *__restrict__ rec, const int *__restrict__ s_idx,
+__device__ __forceinline__ int weights_tile(const R *__restrict__ rec, const int *__restrict__ s_idx,
                                              const double *__restrict__ t64, int tn, int own_i,
                                              const DevFn &fn, R rc, const R (&qx)[PT], const R (&qy)[PT],
                                              const R (&qz)[PT], const double (&px)[PT], const double (&py)[PT],
                                              const double (&pz)[PT], bool two, R (&promol)[PT], R (&own)[PT]) {
   constexpr int RS = weights_record<R>(KM);
+  int nfn = 0;
   for (int i = 0; i < tn; ++i) {
     const R *q = rec + (size_t)i * RS;
     R v[RS];
 #pragma unroll
     for (int u = 0; u < RS; ++u) v[u] = q[u];
     const int nf = __builtin_amdgcn_readfirstlane((int)v[3]);
+    nfn += nf;
     const bool is_own = i == own_i;
 #pragma unroll
     for (int j = 0; j < PT; ++j) {
@@ -440,6 +454,82 @@ __device__ __forceinline__ void weights_tile(const R *__restrict__ rec, const in
       if (is_own) own[j] = pb;
     }
   }
+  return nfn;
+}
+
+// One atom record of the weights pass: position (FP64: absolute; FP32:
+// relative to the shell's atom, formed in FP64), function count, prefactors
+// and exponents (FP32: pre-scaled by log2 e).
+template <typename R, int KM>
+__device__ __forceinline__ void stage_record(R *__restrict__ q, int *__restrict__ idx, int b, const DevMol &m,
+                                             const DevFn &fn, double ax, double ay, double az) {
+  constexpr double log2e = 1.4426950408889634;
+  *idx = b;
+  if (sizeof(R) == sizeof(double)) {
+    q[0] = (R)m.cx[b];
+    q[1] = (R)m.cy[b];
+    q[2] = (R)m.cz[b];
+  } else {
+    q[0] = (R)(m.cx[b] - ax);
+    q[1] = (R)(m.cy[b] - ay);
+    q[2] = (R)(m.cz[b] - az);
+  }
+  const int f0 = fn.fn_off[b], nf = fn.fn_off[b + 1] - f0;
+  q[3] = (R)nf;
+#pragma unroll
+  for (int k = 0; k < KM; ++k) {
+    const bool on = k < nf;
+    q[4 + k] = on ? (R)fn.coef[f0 + k] : (R)0;
+    q[4 + KM + k] = on ? (R)(sizeof(R) == sizeof(double) ? fn.alpha[f0 + k] : fn.alpha[f0 + k] * log2e) : (R)0;
+  }
+}
+
+// Bounded tails.  With non-negative coefficients (MBIS populations, GISA
+// simplex coefficients) every pro-atom is a decreasing function of the
+// distance.  For the points of one radial shell of atom a, at distances up to
+// rhi from R_a, atom b then contributes at most
+//   U_b = sum_k c_bk f_k(x_b),   x_b = max(0, |R_b - R_a| - rhi),
+// and the promolecule is at least L = sum_k c_ak f_k(rhi) + (floors).  An
+// atom with U_b < eps L moves the promolecule of every point of the shell by
+// less than eps relative, so the pass may skip it (its 1e-100 floor is still
+// added).  The test runs in the log2 domain in FP32 (hardware log2), with
+//   log2 U_b <= log2 nf + max_k (log2 c_bk - alpha_bk log2(e) g(x_b)),
+//   log2 L   >= max(max_k (log2 c_ak - alpha_ak log2(e) g(rhi)), log2 floors),
+// g(x) = x (Slater) or x^2 (Gauss), and 2 units of margin for the FP32
+// logarithms and products; distances carry 1e-6 relative margins, which also
+// cover the rounding of the kernel's FP32 distances.
+#define HP_TAIL_MIN_ATOMS 32  // fewer candidates: the bound costs more than it saves
+#define HP_TAIL_LOG2_EPS64 (-64.0f)
+#define HP_TAIL_LOG2_EPS32 (-40.0f)
+
+template <int FK>
+__device__ __forceinline__ float tail_log2_term(double c, double al, float x) {
+  const float g = FK == HP_FN_GAUSS ? x * x : x;
+  return __builtin_amdgcn_logf((float)c) - (float)(al * 1.4426950408889634) * g;
+}
+
+// log2 of the lower bound of the promolecule on the shell (without margin)
+template <int FK>
+__device__ __forceinline__ float tail_log2_own(const DevFn &fn, int a, float rhi, float log2_floors) {
+  const int f0 = fn.fn_off[a], nf = fn.fn_off[a + 1] - f0;
+  const float x = rhi * (1.0f + 1e-6f);
+  float m = log2_floors;
+  for (int k = 0; k < nf; ++k) m = fmaxf(m, tail_log2_term<FK>(fn.coef[f0 + k], fn.alpha[f0 + k], x));
+  return m;
+}
+
+// The test for one candidate (its values are consumed as they arrive, so
+// few registers stay live beside the points').
+template <int FK>
+__device__ __forceinline__ bool tail_negligible(const DevMol &m, const DevFn &fn, int b, double ax, double ay,
+                                                double az, float rhi, float log2_thr) {
+  const float dx = (float)(m.cx[b] - ax), dy = (float)(m.cy[b] - ay), dz = (float)(m.cz[b] - az);
+  const float d = __builtin_amdgcn_sqrtf(dx * dx + dy * dy + dz * dz);
+  const float x = fmaxf(0.0f, d * (1.0f - 1e-6f) - rhi * (1.0f + 1e-6f));
+  const int f0 = fn.fn_off[b], nf = fn.fn_off[b + 1] - f0;
+  float u = -__builtin_inff();
+  for (int k = 0; k < nf; ++k) u = fmaxf(u, tail_log2_term<FK>(fn.coef[f0 + k], fn.alpha[f0 + k], x));
+  return u + __builtin_amdgcn_logf((float)nf) < log2_thr;
 }
 
 // Defect correction of the FP32 pass (corr_mode):
@@ -464,6 +554,9 @@ __global__ __launch_bounds__(512) void k_weights(const int *__restrict__ done, D
   __shared__ double lds[32];
   __shared__ int s_own;  // list position of the shell's own atom
   __shared__ double s_t64[64];  // 2^(j/64) for the FP64 exponential
+  __shared__ float s_rhi[8];    // per wave: largest point distance from the shell's atom
+  __shared__ int s_cnt[8];      // per wave: candidates kept in this staging round
+  __shared__ int s_own_tile, s_own_pos;  // tail path: staging round and slot of the own atom
   if (done && *done) return;
   constexpr int RS = weights_record<R>(KM);
   R *s_rec = reinterpret_cast<R *>(smem_d);
@@ -472,9 +565,11 @@ __global__ __launch_bounds__(512) void k_weights(const int *__restrict__ done, D
   int s = blockIdx.x;
   int a_begin = 0, a_end = m.natom;  // atoms of this shell's molecule
   int mode = corr_mode;
+  int full = 0;
   if (m.work) {
     const int2 wk = m.work[blockIdx.x];
     const HpSlot &sl = m.slots[wk.y];
+    full = sl.full_sum;
     // a molecule takes part in the launches of its current stage, and an
     // FP32-stage molecule also in the FP64 launch of a refresh iteration
     if (sizeof(R) == sizeof(double)) {
@@ -517,44 +612,96 @@ __global__ __launch_bounds__(512) void k_weights(const int *__restrict__ done, D
     own[j] = (R)0;
   }
   const bool two = p0 + blockDim.x < p1;  // second point slot used by any thread
-  constexpr double log2e = 1.4426950408889634;
-  if (threadIdx.x == 0) s_own = -1;
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6, nw = blockDim.x >> 6;
+  if (threadIdx.x == 0) {
+    s_own = -1;
+    s_own_tile = -1;
+  }
   if (sizeof(R) == sizeof(double) && threadIdx.x < 64) s_t64[threadIdx.x] = hp_exp2_64[threadIdx.x];
-  for (int t0 = 0; t0 < nb_n; t0 += tile) {
-    const int tn = min(tile, nb_n - t0);
-    __syncthreads();
-    for (int i = threadIdx.x; i < tn; i += blockDim.x) {
-      const int b = nb ? nb[t0 + i] : a_begin + t0 + i;
-      R *q = s_rec + (size_t)i * RS;
-      s_idx[i] = b;
-      if (b == a) s_own = t0 + i;
-      if (sizeof(R) == sizeof(double)) {
-        q[0] = (R)m.cx[b];
-        q[1] = (R)m.cy[b];
-        q[2] = (R)m.cz[b];
-      } else {
-        q[0] = (R)(m.cx[b] - ax);
-        q[1] = (R)(m.cy[b] - ay);
-        q[2] = (R)(m.cz[b] - az);
-      }
-      const int f0 = fn.fn_off[b], nf = fn.fn_off[b + 1] - f0;
-      q[3] = (R)nf;
+  // bounded tails: the shell's largest point distance (published by the
+  // first staging barrier).  The host enables them only for coefficients that
+  // stay non-negative (MBIS, GISA, from non-negative initial parameters).
+  const bool tail_req = fn.tail_cull && !full && FK != HP_FN_GENERAL && p0 < p1 && nb_n >= HP_TAIL_MIN_ATOMS;
+  if (tail_req) {
+    float rhi = -__builtin_inff();
 #pragma unroll
-      for (int k = 0; k < KM; ++k) {
-        const bool on = k < nf;
-        q[4 + k] = on ? (R)fn.coef[f0 + k] : (R)0;
-        // FP32: exp(-alpha x) = exp2(-(alpha log2 e) x)
-        q[4 + KM + k] = on ? (R)(sizeof(R) == sizeof(double) ? fn.alpha[f0 + k] : fn.alpha[f0 + k] * log2e) : (R)0;
+    for (int j = 0; j < PT; ++j) {
+      if (pidx[j] < p1) {
+        const float dx = (float)(px[j] - ax), dy = (float)(py[j] - ay), dz = (float)(pz[j] - az);
+        rhi = fmaxf(rhi, __builtin_amdgcn_sqrtf(dx * dx + dy * dy + dz * dz));
       }
     }
+    rhi = (float)-wave_min(-(double)rhi);
+    if (lane == 0) s_rhi[wv] = rhi;
+  }
+  const bool tail = tail_req;
+  float tail_rhi = 0.0f, tail_log2_thr = 0.0f;
+  int n_cut = 0;  // candidates skipped by the tail bound
+  int n_fn = 0;  // functions of the staged atoms (each evaluated on every point of the shell)
+  // the tail path stages one candidate per thread and round
+  const int step = tail_req ? min(tile, (int)blockDim.x) : tile;
+  for (int t0 = 0, round = 0; t0 < nb_n; t0 += step, ++round) {
+    const int tn = min(step, nb_n - t0);
     __syncthreads();
-    const int own_i = s_own - t0;  // outside [0, tn) when not in this tile
+    int tk = tn, own_i;
+    if (!tail) {
+      for (int i = threadIdx.x; i < tn; i += blockDim.x) {
+        const int b = nb ? nb[t0 + i] : a_begin + t0 + i;
+        stage_record<R, KM>(s_rec + (size_t)i * RS, s_idx + i, b, m, fn, ax, ay, az);
+        if (b == a) s_own = t0 + i;
+      }
+      __syncthreads();
+      own_i = __builtin_amdgcn_readfirstlane(s_own - t0);  // outside [0, tn) when not in this tile
+    } else {
+      // keep flags, then the kept candidates packed in list order
+      const int i = threadIdx.x;
+      int b = -1;
+      bool keep = false;
+      if (i < tn) b = nb ? nb[t0 + i] : a_begin + t0 + i;
+      if (t0 == 0) {
+        // the threshold (its loads overlap the candidates')
+        float rhi = -__builtin_inff();
+        for (int w = 0; w < nw; ++w) rhi = fmaxf(rhi, s_rhi[w]);
+        tail_rhi = rhi;
+        // log2 of the floors' sum (FP64: n x 1e-100 = 2^(log2 n - 332.19); FP32 adds none)
+        const float log2_floors = sizeof(R) == sizeof(double)
+                                      ? __builtin_amdgcn_logf((float)(a_end - a_begin)) - 332.19281f
+                                      : -__builtin_inff();
+        const float log2_eps = sizeof(R) == sizeof(double) ? HP_TAIL_LOG2_EPS64 : HP_TAIL_LOG2_EPS32;
+        tail_log2_thr = log2_eps - 2.0f + tail_log2_own<FK>(fn, a, rhi, log2_floors);
+      }
+      if (i < tn) keep = b == a || !tail_negligible<FK>(m, fn, b, ax, ay, az, tail_rhi, tail_log2_thr);
+      const unsigned long long bal = __ballot(keep);
+      if (lane == 0) s_cnt[wv] = __popcll(bal);
+      __syncthreads();
+      int base = 0;
+      tk = 0;
+      for (int w = 0; w < nw; ++w) {
+        const int c = s_cnt[w];
+        base += w < wv ? c : 0;
+        tk += c;
+      }
+      if (keep) {
+        const int pos =
+            base + (int)__builtin_amdgcn_mbcnt_hi((unsigned)(bal >> 32), __builtin_amdgcn_mbcnt_lo((unsigned)bal, 0u));
+        stage_record<R, KM>(s_rec + (size_t)pos * RS, s_idx + pos, b, m, fn, ax, ay, az);
+        if (b == a) {
+          s_own_tile = round;
+          s_own_pos = pos;
+        }
+      }
+      __syncthreads();
+      // uniform across the block: scalar registers keep the atom loop uniform
+      tk = __builtin_amdgcn_readfirstlane(tk);
+      own_i = __builtin_amdgcn_readfirstlane(s_own_tile == round ? s_own_pos : -1);
+      n_cut += tn - tk;
+    }
     if (cull)
-      weights_tile<R, FK, PT, KM, true>(s_rec, s_idx, s_t64, tn, own_i, fn, rc, qx, qy, qz, px, py, pz, two, promol,
-                                        own);
+      n_fn += weights_tile<R, FK, PT, KM, true>(s_rec, s_idx, s_t64, tk, own_i, fn, rc, qx, qy, qz, px, py, pz, two,
+                                                promol, own);
     else
-      weights_tile<R, FK, PT, KM, false>(s_rec, s_idx, s_t64, tn, own_i, fn, rc, qx, qy, qz, px, py, pz, two, promol,
-                                         own);
+      n_fn += weights_tile<R, FK, PT, KM, false>(s_rec, s_idx, s_t64, tk, own_i, fn, rc, qx, qy, qz, px, py, pz, two,
+                                                 promol, own);
   }
   double acc_avg = 0.0, acc_ent = 0.0;
 #pragma unroll
@@ -562,8 +709,9 @@ __global__ __launch_bounds__(512) void k_weights(const int *__restrict__ done, D
     if (pidx[j] >= p1) continue;
     const int64_t p = pidx[j];
     R pm_r = promol[j];
-    // culled atoms still contribute their 1e-100 floor
-    if (nb) pm_r += (R)(a_end - a_begin - nb_n) * hp_floor_add<R>();
+    // culled and skipped atoms still contribute their 1e-100 floor
+    const int n_floor = (a_end - a_begin - nb_n) + n_cut;
+    if (n_floor) pm_r += (R)n_floor * hp_floor_add<R>();
     const double pm = (double)pm_r;
     const double rho = m.rho[p];
     if (!GT2) {
@@ -582,6 +730,10 @@ __global__ __launch_bounds__(512) void k_weights(const int *__restrict__ done, D
   }
   block_sum2(acc_avg, acc_ent, lds);
   if (threadIdx.x == 0) {
+    if (fn.evals_done && p1 > p0) {
+      const int slot = (sizeof(R) == sizeof(double) ? HP_EVAL_LANES : 0) + (int)(blockIdx.x & (HP_EVAL_LANES - 1));
+      atomicAdd(fn.evals_done + (size_t)slot * HP_EVAL_STRIDE, (unsigned long long)n_fn * (unsigned long long)(p1 - p0));
+    }
     if (GT2) {
       ent_part[s] = acc_ent;
     } else if (mode == HP_CORR_CALIBRATE) {

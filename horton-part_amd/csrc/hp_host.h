@@ -7,6 +7,7 @@
 #include <string>
 
 #include "hortonpart.h"
+#include "hp_device.h"
 
 namespace hp {
 
@@ -45,6 +46,21 @@ inline int pick_ppl(int R) {
   if (R <= 128) return 2;
   if (R <= 256) return 4;
   if (R <= 512) return 8;
+  return 0;
+}
+
+// Sum the spread evaluation counters of the weights pass (HP_EVAL_SLOTS
+// device values) into [FP32, FP64] totals.
+inline int read_eval_counters(const unsigned long long *dev, double *fp32, double *fp64) {
+  static_assert(HP_EVAL_SLOTS == 2 * HP_EVAL_LANES * HP_EVAL_STRIDE, "counter layout");
+  unsigned long long h[HP_EVAL_SLOTS];
+  hipError_t e = hipMemcpy(h, dev, sizeof(h), hipMemcpyDeviceToHost);
+  if (e != hipSuccess) return fail(HP_E_HIP, std::string("hipMemcpy: ") + hipGetErrorString(e));
+  unsigned long long t[2] = {0, 0};
+  for (int p = 0; p < 2; ++p)
+    for (int l = 0; l < HP_EVAL_LANES; ++l) t[p] += h[((size_t)p * HP_EVAL_LANES + l) * HP_EVAL_STRIDE];
+  *fp32 = (double)t[0];
+  *fp64 = (double)t[1];
   return 0;
 }
 

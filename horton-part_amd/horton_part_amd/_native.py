@@ -122,6 +122,7 @@ class SchemeDesc(ctypes.Structure):
         ("check_mono", ctypes.c_int),
         ("mixed_switch", _d),
         ("mixed_refresh", ctypes.c_int),
+        ("full_sum", ctypes.c_int),
     ]
 
 
@@ -132,6 +133,8 @@ class PartWork(ctypes.Structure):
         ("bytes_per_pass", _d),
         ("hessian_passes", ctypes.c_int64),
         ("hessian_flops", _d),
+        ("evals_done_fp32", _d),
+        ("evals_done_fp64", _d),
     ]
 
 
@@ -178,6 +181,8 @@ class BatchStats(ctypes.Structure):
         ("points_fp64", _d),
         ("evals_fp32", _d),
         ("evals_fp64", _d),
+        ("done_fp32", _d),
+        ("done_fp64", _d),
     ]
 
 
@@ -244,7 +249,7 @@ def load():
     for name in EXPORTED_SYMBOLS:
         if name not in ("hp_abi_version", "hp_last_error"):
             getattr(lib, name).restype = ctypes.c_int
-    if lib.hp_abi_version() != 1:
+    if lib.hp_abi_version() != 2:
         raise RuntimeError("libhortonpart ABI version mismatch")
     _LIB = lib
     return lib

@@ -269,6 +269,10 @@ class AbstractISAWPart(WPart):
         and refreshes the per-shell correction FP64 - FP32 of the spherical
         averages (default 0: no correction)
       cutoff_radius: pro-atom tails beyond this distance are zero (default None)
+      full_sum: evaluate every pro-atom on every point (default False: MBIS
+        and GISA skip, per radial shell, atoms whose pro-atom is bounded there
+        below 2^-64 of the promolecule, which changes no result beyond
+        rounding; LISA always sums every atom)
       device: GPU ordinal (default 0)
     """
 
@@ -294,6 +298,7 @@ class AbstractISAWPart(WPart):
         mixed_switch=1e-5,
         mixed_refresh=0,
         cutoff_radius=None,
+        full_sum=False,
         device=0,
         **kwargs,
     ):
@@ -307,6 +312,7 @@ class AbstractISAWPart(WPart):
         self._mixed_switch = mixed_switch
         self._mixed_refresh = mixed_refresh
         self._cutoff_radius = cutoff_radius
+        self._full_sum = bool(full_sum)
         self._device = device
         self._dev = None
         super().__init__(coordinates, numbers, pseudo_numbers, grid, moldens, spindens, lmax, logger, grid_type=grid_type, **kwargs)
@@ -373,6 +379,7 @@ class AbstractISAWPart(WPart):
             "check_mono": 0,
             "mixed_switch": float(self._mixed_switch),
             "mixed_refresh": int(self._mixed_refresh),
+            "full_sum": int(self._full_sum),
         }
 
     def _init_propars(self):

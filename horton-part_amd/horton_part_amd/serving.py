@@ -10,10 +10,18 @@ molecule leaves on its own convergence test.  Each result is the one
 ``do_partitioning()`` of the same scheme object gives with the same
 precision.
 
+Precision per request: "fp64", "mixed" (FP32 grid arithmetic, FP64 sums,
+solves and polish), or "auto": mixed for GISA requests and FP64 for MBIS
+requests.  MBIS's inner fixed point turns FP32 noise in the spherical
+averages into small jumps of its iteration counts, and on slowly converging
+molecules those can move the outer loop's stop by an iteration (charges then
+differ from the FP64 result by up to a few 1e-6 e); GISA's QP has no such
+switch, and its mixed results stay within 1e-6 e of FP64.
+
 Usage::
 
     server = PartitionServer(device=0)
-    t = server.submit(MBISWPart(...), precision="mixed")
+    t = server.submit(MBISWPart(...), precision="auto")
     while server.pending:
         server.step()
         for done in server.poll():
@@ -59,6 +67,16 @@ def request_arrays(part):
     """(molecule arrays, scheme arrays, initial parameters) of a scheme object
     that has not been partitioned yet: what ``submit`` sends to the device."""
     return part._molecule_arrays(), part._scheme_arrays(), np.ascontiguousarray(part._init_propars(), dtype=float)
+
+
+def resolve_precision(precision, scheme):
+    """The precision a request runs in: "auto" is mixed for GISA, FP64 for
+    MBIS (``scheme`` is the ``hp_scheme_desc`` scheme number)."""
+    if precision == "auto":
+        return "mixed" if scheme == _native.SCHEME["gisa"] else "fp64"
+    if precision not in _native.PREC:
+        raise ValueError(f"unknown precision {precision!r}")
+    return precision
 
 
 class PartitionServer:
@@ -114,15 +132,16 @@ class PartitionServer:
         self.n_queued = 0
 
     # -- requests ---------------------------------------------------------------
-    def submit(self, part, precision="mixed", n_polish=1, tag=None):
+    def submit(self, part, precision="auto", n_polish=1, tag=None):
         """Queue the partitioning of a scheme object (not yet partitioned)."""
         mol, sch, p0 = request_arrays(part)
         return self.submit_arrays(mol, sch, p0, precision=precision, n_polish=n_polish, tag=tag)
 
-    def submit_arrays(self, mol_arrays, scheme_arrays, propars0, precision="mixed", n_polish=1, tag=None):
+    def submit_arrays(self, mol_arrays, scheme_arrays, propars0, precision="auto", n_polish=1, tag=None):
         """Queue a request given as ``hp_molecule_desc``/``hp_scheme_desc``
         arrays (see ``request_arrays``).  The arrays are read when the request
         is admitted; they are kept referenced until its result is polled."""
+        precision = resolve_precision(precision, scheme_arrays["scheme"])
         md = _native.MoleculeDesc()
         sd = _native.SchemeDesc()
         keep = []

@@ -24,7 +24,7 @@
 extern "C" {
 #endif
 
-#define HP_ABI_VERSION 1
+#define HP_ABI_VERSION 2
 
 /* error codes (negative returns) */
 #define HP_E_ARG -1      /* invalid argument or shape */
@@ -111,6 +111,16 @@ typedef struct {
                                 refreshes the per-shell correction FP64 - FP32
                                 that the FP32 iterations add to their spherical
                                 averages; <= 0: no correction */
+  int full_sum;              /* MBIS, GISA: 0 (default) lets the weights pass
+                                skip, per radial shell, every atom whose
+                                pro-atom is bounded there by 2^-64 (FP32 stage:
+                                2^-40) of the promolecule's lower bound (the
+                                shell atom's pro-atom at the shell's largest
+                                radius plus the 1e-100 floors): the skipped
+                                terms move the promolecule by less than one
+                                rounding.  1: every atom on every point, as the
+                                reference evaluates them.  LISA always sums
+                                every atom (its coefficients may be negative). */
 } hp_scheme_desc;
 
 typedef struct hp_part hp_part;
@@ -198,6 +208,9 @@ typedef struct {
   int64_t hessian_passes;  /* grid_type 2 Newton solvers, last hp_part_run: atom
                               passes over the grid that formed B diag(w) B^T */
   double hessian_flops;    /* ... their 2 K_a^2 N flops in total */
+  double evals_done_fp32;  /* pro-atom function evaluations the weights passes of
+                              the last hp_part_run performed in FP32 ... */
+  double evals_done_fp64;  /* ... and in FP64 (after cut-off and tail skipping) */
 } hp_part_work;
 
 int hp_part_get_work(hp_part *part, hp_part_work *out);
@@ -303,6 +316,10 @@ typedef struct {
   double points_fp64;      /* grid points through the FP64 weights pass */
   double evals_fp32;       /* pro-atom function evaluations in FP32 */
   double evals_fp64;       /* pro-atom function evaluations in FP64 */
+  double done_fp32;        /* function evaluations the FP32 weights launches
+                              performed (after tail skipping) ... */
+  double done_fp64;        /* ... and the FP64 ones; hp_batch_get_stats waits
+                              for the work enqueued so far to read them */
 } hp_batch_stats;
 
 int hp_batch_get_stats(hp_batch *batch, hp_batch_stats *out);

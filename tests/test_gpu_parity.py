@@ -283,3 +283,62 @@ def test_gt2_lisa_mixed_mfma_fp32():
     dq = np.max(np.abs(part["charges"] - cache["charges"]))
     _say(f"  mixed niter={part['niter']} oracle niter={cache['niter']} max|dq|={dq:.2e}")
     assert dq <= 1e-6
+
+
+# --- bounded-tail skipping of the weights pass (full_sum=False, the default) ---
+
+
+def _evals_done_frac(part):
+    wk = part._dev.work()
+    done = wk["evals_done_fp64"] + wk["evals_done_fp32"]
+    return done / (wk["evals_per_pass"] * part["niter"])
+
+
+@pytest.mark.parametrize("scheme", ["mbis", "gisa"])
+def test_tail_skip_against_oracle(scheme):
+    """A 40-atom chain, where the weights pass skips a large share of the
+    atom/shell pairs, against the oracle (which sums every atom everywhere)."""
+    require_gpu()
+    seed = 21 if scheme == "mbis" else 23
+    _say(f"case: {scheme} tail skipping, chain 40 atoms seed {seed}, 30x26")
+    w = workloads.chain(natom=40, seed=seed, nrad=30, nang=26, device=0)
+    args = (w.coordinates, w.numbers, w.pseudo_numbers, w.grid, w.density)
+    if scheme == "mbis":
+        part = MBISWPart(*args, maxiter=12)
+        oracle = OracleMBIS(*args, maxiter=12)
+    else:
+        part = GaussianISAWPart(*args, maxiter=10)
+        bs = part.bs_helper
+        oracle = OracleGISA(*args, maxiter=10, basis=OracleBasis(bs.orders, bs.exponents, bs.initials))
+    part.do_partitioning()
+    frac = _evals_done_frac(part)
+    _say(f"  function evaluations performed / algorithmic: {frac:.3f}")
+    assert frac < 0.9
+    cache = oracle.do_partitioning()
+    _compare(part, cache, oracle, check_inner=scheme == "mbis")
+
+
+@pytest.mark.parametrize("scheme,natom,seed", [("mbis", 120, 5), ("gisa", 80, 9)])
+def test_tail_skip_equals_full_sum(scheme, natom, seed):
+    """Skipping the bounded tails changes nothing beyond rounding: the same
+    iteration count and charges within 1e-12 e of the full sum."""
+    require_gpu()
+    _say(f"case: {scheme} tail skipping vs full sum, chain {natom} atoms seed {seed}, 40x50")
+    w = workloads.chain(natom=natom, seed=seed, nrad=40, nang=50, device=0)
+    args = (w.coordinates, w.numbers, w.pseudo_numbers, w.grid, w.density)
+    cls = MBISWPart if scheme == "mbis" else GaussianISAWPart
+    res = {}
+    for full in (True, False):
+        part = cls(*args, maxiter=25, full_sum=full)
+        part.do_partitioning()
+        res[full] = (part["niter"], part["charges"].copy(), part["propars"].copy(), _evals_done_frac(part))
+        part.close()
+    dq = np.max(np.abs(res[True][1] - res[False][1]))
+    rel = np.max(np.abs(res[True][2] - res[False][2]) / np.maximum(np.abs(res[True][2]), 1e-300))
+    _say(f"  niter {res[True][0]} / {res[False][0]}; max|dq| {dq:.2e}; rel dpar {rel:.2e}; "
+         f"evaluations performed: full {res[True][3]:.3f}, skipping {res[False][3]:.3f}")
+    assert res[True][3] == pytest.approx(1.0)
+    assert res[False][3] < 0.9
+    assert res[True][0] == res[False][0]
+    assert dq <= 1e-12
+    assert rel <= 1e-10

@@ -157,3 +157,21 @@ def test_missing_library_fails_loudly(tmp_path):
     root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
     out = subprocess.run([sys.executable, "-c", code], env=env, cwd=root, capture_output=True, text=True)
     assert "RAISED" in out.stdout
+
+
+def test_serving_precision_policy():
+    """"auto" runs GISA requests mixed and MBIS requests in FP64; the scheme
+    arrays carry full_sum (0: tail skipping allowed)."""
+    from horton_part_amd.serving import resolve_precision
+
+    assert resolve_precision("auto", 1) == "mixed"
+    assert resolve_precision("auto", 0) == "fp64"
+    assert resolve_precision("fp64", 1) == "fp64"
+    assert resolve_precision("mixed", 0) == "mixed"
+    with pytest.raises(ValueError):
+        resolve_precision("fp16", 0)
+    w = workloads.water(nrad=20, nang=26)
+    part = MBISWPart(w.coordinates, w.numbers, w.pseudo_numbers, w.grid, w.density)
+    assert part._scheme_arrays()["full_sum"] == 0
+    part = MBISWPart(w.coordinates, w.numbers, w.pseudo_numbers, w.grid, w.density, full_sum=True)
+    assert part._scheme_arrays()["full_sum"] == 1

@@ -29,7 +29,7 @@ def test_header_symbols_exported():
 
 def test_abi_version_and_errors():
     lib = _native.load()
-    assert lib.hp_abi_version() == 1
+    assert lib.hp_abi_version() == 2
     n = ctypes.c_int32(-1)
     lib.hp_device_count(ctypes.byref(n))
     assert n.value >= 0
