@@ -168,6 +168,50 @@ __device__ __forceinline__ void wave_sum_n(double *v) {
   for (int i = 0; i < N; ++i) v[i] = lane63(v[i]);
 }
 
+// All-reduce of N doubles over a wavefront, for latency-bound loops: DPP
+// butterflies inside each row of 16 lanes (pairs, quads, half rows, rows),
+// then the gfx950 lane swaps across rows, 32 lanes apart and then 16 apart.
+// Called with one register as both operands, a swap returns the lane's own
+// value and its partner's, whose sum is the pair total; each add sees its
+// two operands in the same order in every lane, so all 64 lanes end with the
+// same total, bit for bit.  All 64 lanes must be active.
+template <int CTRL>
+__device__ __forceinline__ double dpp_row_d(double v) {
+  const int lo = __builtin_amdgcn_mov_dpp(__double2loint(v), CTRL, 0xF, 0xF, true);
+  const int hi = __builtin_amdgcn_mov_dpp(__double2hiint(v), CTRL, 0xF, 0xF, true);
+  return __hiloint2double(hi, lo);
+}
+
+__device__ __forceinline__ double swap32_sum(double v) {
+  const unsigned lo = (unsigned)__double2loint(v), hi = (unsigned)__double2hiint(v);
+  const auto l = __builtin_amdgcn_permlane32_swap(lo, lo, false, false);
+  const auto h = __builtin_amdgcn_permlane32_swap(hi, hi, false, false);
+  return __hiloint2double((int)h[0], (int)l[0]) + __hiloint2double((int)h[1], (int)l[1]);
+}
+
+__device__ __forceinline__ double swap16_sum(double v) {
+  const unsigned lo = (unsigned)__double2loint(v), hi = (unsigned)__double2hiint(v);
+  const auto l = __builtin_amdgcn_permlane16_swap(lo, lo, false, false);
+  const auto h = __builtin_amdgcn_permlane16_swap(hi, hi, false, false);
+  return __hiloint2double((int)h[0], (int)l[0]) + __hiloint2double((int)h[1], (int)l[1]);
+}
+
+template <int N>
+__device__ __forceinline__ void wave_allsum_n(double *v) {
+#pragma unroll
+  for (int i = 0; i < N; ++i) v[i] += dpp_row_d<0xB1>(v[i]);  // quad_perm [1,0,3,2]
+#pragma unroll
+  for (int i = 0; i < N; ++i) v[i] += dpp_row_d<0x4E>(v[i]);  // quad_perm [2,3,0,1]
+#pragma unroll
+  for (int i = 0; i < N; ++i) v[i] += dpp_row_d<0x141>(v[i]);  // row_half_mirror
+#pragma unroll
+  for (int i = 0; i < N; ++i) v[i] += dpp_row_d<0x140>(v[i]);  // row_mirror
+#pragma unroll
+  for (int i = 0; i < N; ++i) v[i] = swap32_sum(v[i]);
+#pragma unroll
+  for (int i = 0; i < N; ++i) v[i] = swap16_sum(v[i]);
+}
+
 __device__ __forceinline__ double wave_sum(double v) {
   wave_sum_n<1>(&v);
   return v;
@@ -400,13 +444,26 @@ __device__ __forceinline__ float neg_exp_scaled<float>(float al, float arg, cons
   return __builtin_amdgcn_exp2f(-al * arg);
 }
 
-// Returns the number of functions of the tile's atoms (uniform).
-template <typename R, int FK, int PT, int KM, bool CULL>
+// Argument of function k of a record: r (Slater), r^2 (Gauss), r^n (general).
+template <typename R, int FK>
+__device__ __forceinline__ R fn_arg(R r, R r2, const DevFn &fn, int b, int k) {
+  if (FK == HP_FN_SLATER) return r;
+  if (FK == HP_FN_GAUSS) return r2;
+  const double n = fn.order[fn.fn_off[b] + k];
+  return (n == 1.0) ? r : ((n == 2.0) ? r2 : (R)pow((double)r, n));
+}
+
+// The atoms of one staged tile on the thread's first NP point slots.  The
+// distances come first; then the functions, two at a time, each uniform
+// branch holding 2 x NP independent exponentials.  Per point the terms are
+// added in function order.  Returns the number of functions of the tile's
+// atoms (uniform).
+template <typename R, int FK, int PT, int NP, int KM, bool CULL>
 __device__ __forceinline__ int weights_tile(const R *__restrict__ rec, const int *__restrict__ s_idx,
                                              const double *__restrict__ t64, int tn, int own_i,
                                              const DevFn &fn, R rc, const R (&qx)[PT], const R (&qy)[PT],
                                              const R (&qz)[PT], const double (&px)[PT], const double (&py)[PT],
-                                             const double (&pz)[PT], bool two, R (&promol)[PT], R (&own)[PT]) {
+                                             const double (&pz)[PT], R (&promol)[PT], R (&own)[PT]) {
   constexpr int RS = weights_record<R>(KM);
   int nfn = 0;
   for (int i = 0; i < tn; ++i) {
@@ -417,9 +474,10 @@ __device__ __forceinline__ int weights_tile(const R *__restrict__ rec, const int
     const int nf = __builtin_amdgcn_readfirstlane((int)v[3]);
     nfn += nf;
     const bool is_own = i == own_i;
+    const int b = FK == HP_FN_GENERAL ? s_idx[i] : 0;
+    R r[NP], r2[NP], pb[NP];
 #pragma unroll
-    for (int j = 0; j < PT; ++j) {
-      if (j > 0 && !two) break;
+    for (int j = 0; j < NP; ++j) {
       R dx, dy, dz;
       if (sizeof(R) == sizeof(double)) {
         dx = (R)(px[j] - (double)v[0]);
@@ -430,28 +488,30 @@ __device__ __forceinline__ int weights_tile(const R *__restrict__ rec, const int
         dy = qy[j] - v[1];
         dz = qz[j] - v[2];
       }
-      const R r2 = dx * dx + dy * dy + dz * dz;
-      const R r = (FK == HP_FN_GAUSS && !CULL) ? (R)0 : fast_sqrt<R>(r2);
-      R pb = (R)0;
+      r2[j] = dx * dx + dy * dy + dz * dz;
+      r[j] = (FK == HP_FN_GAUSS && !CULL) ? (R)0 : fast_sqrt<R>(r2[j]);
+      pb[j] = (R)0;
+    }
 #pragma unroll
-      for (int k = 0; k < KM; ++k) {
-        if (k < nf) {
-          R arg;
-          if (FK == HP_FN_SLATER) {
-            arg = r;
-          } else if (FK == HP_FN_GAUSS) {
-            arg = r2;
-          } else {
-            const double n = fn.order[fn.fn_off[s_idx[i]] + k];
-            arg = (n == 1.0) ? r : ((n == 2.0) ? r2 : (R)pow((double)r, n));
-          }
-          pb += v[4 + k] * neg_exp_scaled<R>(v[4 + KM + k], arg, t64);
+    for (int k = 0; k < KM; k += 2) {
+      if (k + 1 < nf) {
+#pragma unroll
+        for (int j = 0; j < NP; ++j) {
+          pb[j] += v[4 + k] * neg_exp_scaled<R>(v[4 + KM + k], fn_arg<R, FK>(r[j], r2[j], fn, b, k), t64);
+          pb[j] += v[5 + k] * neg_exp_scaled<R>(v[5 + KM + k], fn_arg<R, FK>(r[j], r2[j], fn, b, k + 1), t64);
         }
+      } else if (k < nf) {
+#pragma unroll
+        for (int j = 0; j < NP; ++j)
+          pb[j] += v[4 + k] * neg_exp_scaled<R>(v[4 + KM + k], fn_arg<R, FK>(r[j], r2[j], fn, b, k), t64);
       }
-      if (CULL && r > rc) pb = (R)0;
-      promol[j] += pb;
+    }
+#pragma unroll
+    for (int j = 0; j < NP; ++j) {
+      if (CULL && r[j] > rc) pb[j] = (R)0;
+      promol[j] += pb[j];
       if (sizeof(R) == sizeof(double)) promol[j] += hp_floor_add<R>();
-      if (is_own) own[j] = pb;
+      if (is_own) own[j] = pb[j];
     }
   }
   return nfn;
@@ -696,12 +756,18 @@ __global__ __launch_bounds__(512) void k_weights(const int *__restrict__ done, D
       own_i = __builtin_amdgcn_readfirstlane(s_own_tile == round ? s_own_pos : -1);
       n_cut += tn - tk;
     }
-    if (cull)
-      n_fn += weights_tile<R, FK, PT, KM, true>(s_rec, s_idx, s_t64, tk, own_i, fn, rc, qx, qy, qz, px, py, pz, two,
-                                                promol, own);
-    else
-      n_fn += weights_tile<R, FK, PT, KM, false>(s_rec, s_idx, s_t64, tk, own_i, fn, rc, qx, qy, qz, px, py, pz, two,
-                                                 promol, own);
+    // every point slot in use, or only the first (the shell fits one pass of the block)
+    if (cull) {
+      n_fn += two ? weights_tile<R, FK, PT, PT, KM, true>(s_rec, s_idx, s_t64, tk, own_i, fn, rc, qx, qy, qz, px, py,
+                                                          pz, promol, own)
+                  : weights_tile<R, FK, PT, 1, KM, true>(s_rec, s_idx, s_t64, tk, own_i, fn, rc, qx, qy, qz, px, py,
+                                                         pz, promol, own);
+    } else {
+      n_fn += two ? weights_tile<R, FK, PT, PT, KM, false>(s_rec, s_idx, s_t64, tk, own_i, fn, rc, qx, qy, qz, px, py,
+                                                           pz, promol, own)
+                  : weights_tile<R, FK, PT, 1, KM, false>(s_rec, s_idx, s_t64, tk, own_i, fn, rc, qx, qy, qz, px, py,
+                                                          pz, promol, own);
+    }
   }
   double acc_avg = 0.0, acc_ent = 0.0;
 #pragma unroll
@@ -796,9 +862,9 @@ __device__ __forceinline__ void load_radial(const DevMol &m, const double *rhoba
 // ---------------------------------------------------------------------------
 // K2: MBIS inner fixed point (mbis.py:81-163), one workgroup per atom with one
 // radial point per lane (ceil(R/64) waves).  Each iteration reduces 2K+1
-// values: a DPP tree inside every wave, then the wave totals through LDS in
-// wave order, so all lanes of all waves hold bitwise identical sums and take
-// the same convergence branch.  The LDS slots alternate with the iteration
+// values: an all-reduce inside every wave (DPP and lane swaps), then the wave
+// totals through LDS in wave order, so all lanes of all waves hold bitwise
+// identical sums and take the same convergence branch.  The LDS slots alternate with the iteration
 // parity, which needs one barrier per iteration.
 // KM: compile-time bound on shells per atom (loops unrolled, guarded by K).
 // ---------------------------------------------------------------------------
@@ -806,19 +872,22 @@ template <int NV>
 __device__ __forceinline__ void block_allsum(double *v, double *lds, int parity) {
   // lds: 2 x 16 waves x NV doubles
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6, nw = blockDim.x >> 6;
-  wave_sum_n<NV>(v);
+  wave_allsum_n<NV>(v);
   double *slot = lds + (size_t)parity * 16 * NV;
   if (lane == 0) {
 #pragma unroll
     for (int i = 0; i < NV; ++i) slot[wv * NV + i] = v[i];
   }
   __syncthreads();
+  double t[NV];
 #pragma unroll
-  for (int i = 0; i < NV; ++i) {
-    double t = 0.0;
-    for (int w = 0; w < nw; ++w) t += slot[w * NV + i];
-    v[i] = t;
+  for (int i = 0; i < NV; ++i) t[i] = 0.0;
+  for (int w = 0; w < nw; ++w) {
+#pragma unroll
+    for (int i = 0; i < NV; ++i) t[i] += slot[w * NV + i];
   }
+#pragma unroll
+  for (int i = 0; i < NV; ++i) v[i] = t[i];
 }
 
 template <int KM>
@@ -830,7 +899,12 @@ __global__ __launch_bounds__(512) void k_mbis_inner(const int *__restrict__ done
                                                     double *__restrict__ change_part) {
   constexpr int NV = 2 * KM + 1;
   __shared__ double lds[2 * 16 * NV];
+  __shared__ double s_t64[64];  // 2^(j/64) for the exponential (published by the first reduction)
   if (done && *done) return;
+  // a dependent chain of short steps: its waves issue first on their SIMDs,
+  // beside the throughput-bound weights pass of other molecules
+  __builtin_amdgcn_s_setprio(3);
+  if (threadIdx.x < 64) s_t64[threadIdx.x] = hp_exp2_64[threadIdx.x];
   int a = blockIdx.x;
   if (m.work) {
     const int2 wk = m.work[blockIdx.x];
@@ -869,7 +943,7 @@ __global__ __launch_bounds__(512) void k_mbis_inner(const int *__restrict__ done
       // N S^3 / (8 pi), once per shell instead of a division per point
       const double coef = Np[k] * (Sp[k] * Sp[k] * Sp[k]) / (8.0 * HP_PI);
       if (k < K) {
-        t[k] = coef * exp(-Sp[k] * r);
+        t[k] = coef * exp_tab(-Sp[k] * r, s_t64);
         pro += t[k];
       } else {
         t[k] = 0.0;
@@ -910,8 +984,8 @@ __global__ __launch_bounds__(512) void k_mbis_inner(const int *__restrict__ done
     if (k < K) {
       const double N1 = Np[k], S1 = Sp[k];
       const double N2 = old_par[2 * (f0 + k)], S2 = old_par[2 * (f0 + k) + 1];
-      y1 += N1 * (S1 * S1 * S1) / (8.0 * HP_PI) * exp(-S1 * r);
-      y2 += N2 * (S2 * S2 * S2) / (8.0 * HP_PI) * exp(-S2 * r);
+      y1 += N1 * (S1 * S1 * S1) / (8.0 * HP_PI) * exp_tab(-S1 * r, s_t64);
+      y2 += N2 * (S2 * S2 * S2) / (8.0 * HP_PI) * exp_tab(-S2 * r, s_t64);
     }
   }
   const double dy = y1 - y2;
@@ -1079,6 +1153,7 @@ __global__ __launch_bounds__(256) void k_gisa_qp(const int *__restrict__ done, D
                                                  double *__restrict__ charges, double *__restrict__ change_part) {
   __shared__ double Mlds[4][(HP_KMAX + 1) * (HP_KMAX + 2)];
   if (done && *done) return;
+  __builtin_amdgcn_s_setprio(3);  // latency-bound: issue first (see k_mbis_inner)
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
   const int ai = blockIdx.x * (blockDim.x >> 6) + wv;
   if (ai >= m.natom) return;
@@ -1184,6 +1259,7 @@ __global__ __launch_bounds__(256) void k_lisa_inner(const int *__restrict__ done
                                                     double *__restrict__ change_part) {
   __shared__ double Mlds[4][(HP_KMAX + 1) * (HP_KMAX + 2)];
   if (done && *done) return;
+  __builtin_amdgcn_s_setprio(3);  // latency-bound: issue first (see k_mbis_inner)
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
   const int ai = blockIdx.x * (blockDim.x >> 6) + wv;
   if (ai >= m.natom) return;
