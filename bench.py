@@ -313,6 +313,7 @@ def run_serving(args):
         max_molecule_points=max(npts),
         ticks_per_step=args.ticks,
         use_graphs=args.graphs,
+        groups=args.groups,
     )
     if args.warmup:
         log(f"warmup: {args.warmup} request(s) served together, untimed")
@@ -378,6 +379,7 @@ def run_serving(args):
             "points": int(sum(it["mol"]["npoint"] for it in timed)),
             "ticks_per_step": args.ticks,
             "graphs": args.graphs,
+            "groups": args.groups or 2,
         },
         "p50_ms": float(np.percentile(lat, 50)) * 1e3,
         "p99_ms": float(np.percentile(lat, 99)) * 1e3,
@@ -449,6 +451,7 @@ def main():
     ap.add_argument("--workload", default="serving",
                     choices=["serving", "water", "chain20", "chain20-gisa", "cluster200", "protein2000"])
     ap.add_argument("--ticks", type=int, default=4, help="serving: outer iterations per host step")
+    ap.add_argument("--groups", type=int, default=0, help="serving: molecule groups interleaved per tick (1-4; 0: 2)")
     ap.add_argument("--graphs", action="store_true", help="serving: replay the ticks of a step as a hipGraph (times whole steps only)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--serve-precision", choices=("mixed", "fp64", "auto"), default="auto",

@@ -79,7 +79,8 @@ class RangeAlloc {
 
 enum SlotState { SLOT_FREE = 0, SLOT_UPLOADING, SLOT_ACTIVE };
 
-constexpr int kGroups = 2;  // groups of molecules whose ticks interleave
+constexpr int kMaxGroups = 4;  // groups of molecules whose ticks interleave (one stream each)
+constexpr int kDefaultGroups = 2;
 
 struct Request {
   int64_t ticket = 0;
@@ -146,11 +147,11 @@ struct hp_batch {
   int *d_slot_list = nullptr, *h_slot_list = nullptr;
   int64_t work_cap = 0;
   int nslot_list = 0;
-  // two groups of molecules, launched on cs and cs1
-  std::vector<Launch> plan[kGroups];
-  int nslot[kGroups] = {0, 0}, slot_off[kGroups] = {0, 0};
-  hipStream_t cs1 = nullptr;
-  hipEvent_t ev_fork = nullptr, ev_join = nullptr, ev_step[2] = {nullptr, nullptr};
+  // groups of molecules, group g launched on gs[g] (gs[0] is cs)
+  std::vector<Launch> plan[kMaxGroups];
+  int nslot[kMaxGroups] = {}, slot_off[kMaxGroups] = {};
+  hipStream_t gs[kMaxGroups] = {};
+  hipEvent_t ev_fork = nullptr, ev_join[kMaxGroups] = {}, ev_step[2] = {nullptr, nullptr};
   std::vector<int> pending_fp64;  // slots to flip to FP64 before the next ticks
   // graph of one step's ticks
   hipGraph_t graph = nullptr;
@@ -234,9 +235,12 @@ void free_batch(hp_batch *b) {
     if (e) (void)hipEventDestroy(e);
   for (auto e : b->tev)
     if (e) (void)hipEventDestroy(e);
-  for (hipEvent_t e : {b->ev_fork, b->ev_join, b->ev_step[0], b->ev_step[1]})
+  for (hipEvent_t e : {b->ev_fork, b->ev_step[0], b->ev_step[1]})
     if (e) (void)hipEventDestroy(e);
-  if (b->cs1) (void)hipStreamDestroy(b->cs1);
+  for (int g = 1; g < kMaxGroups; ++g) {
+    if (b->ev_join[g]) (void)hipEventDestroy(b->ev_join[g]);
+    if (b->gs[g]) (void)hipStreamDestroy(b->gs[g]);
+  }
   for (auto *r : b->slot_req) delete r;
   for (auto *r : b->queue) delete r;
   if (b->cs) (void)hipStreamDestroy(b->cs);
@@ -538,8 +542,8 @@ void plan_group(hp_batch *b, const std::vector<int> &active, std::vector<Launch>
   }
 }
 
-// Split the molecules in flight into two groups of about equal weights cost
-// (largest first, each to the lighter group) and build their launches.
+// Split the molecules in flight into cfg.groups groups of about equal weights
+// cost (largest first, each to the lightest group) and build their launches.
 int build_plan(hp_batch *b) {
   std::vector<int> active;
   for (int sl = 0; sl < b->cfg.max_inflight; ++sl)
@@ -548,16 +552,19 @@ int build_plan(hp_batch *b) {
     const double cx = weights_cost(b->slot_req[x]), cy = weights_cost(b->slot_req[y]);
     return cx != cy ? cx > cy : x < y;
   });
-  std::vector<int> grp[kGroups];
-  double load[kGroups] = {0.0, 0.0};
+  const int G = b->cfg.groups;
+  std::vector<int> grp[kMaxGroups];
+  double load[kMaxGroups] = {};
   for (int sl : active) {
-    const int g = load[0] <= load[1] ? 0 : 1;
+    int g = 0;
+    for (int h = 1; h < G; ++h)
+      if (load[h] < load[g]) g = h;
     grp[g].push_back(sl);
     load[g] += weights_cost(b->slot_req[sl]);
   }
   int64_t w = 0;
   int ns = 0;
-  for (int g = 0; g < kGroups; ++g) {
+  for (int g = 0; g < kMaxGroups; ++g) {
     std::sort(grp[g].begin(), grp[g].end());
     b->slot_off[g] = ns;
     b->nslot[g] = (int)grp[g].size();
@@ -607,22 +614,28 @@ void launch_w_r(hp_batch *b, const Launch &L, hipStream_t s) {
 
 // Event e of group g in tick t of a step: 0 start, 1 after k_batch_begin,
 // 2 after the weights launches, 3 after the per-atom launches, 4 end.
-inline hipEvent_t &tick_event(hp_batch *b, int t, int g, int e) { return b->tev[((size_t)t * kGroups + g) * 5 + e]; }
+inline hipEvent_t &tick_event(hp_batch *b, int t, int g, int e) {
+  return b->tev[((size_t)t * kMaxGroups + g) * 5 + e];
+}
 
 // One outer iteration of every molecule in flight (tick t of the step).  The
-// weights launches of the two groups alternate (group 1 after group 0 of the
-// same tick, group 0 after group 1 of the previous tick), so that each
-// group's latency-bound per-atom solves run beside the other's weights pass.
+// weights launches of the groups take turns in a ring (group g after the
+// previous group of the same tick, the first group after the last group of
+// the previous tick), so that each group's latency-bound per-atom solves run
+// beside the other groups' weights passes.
 int enqueue_tick(hp_batch *b, int t) {
-  for (int g = 0; g < kGroups; ++g) {
+  int used[kMaxGroups], nused = 0;
+  for (int g = 0; g < kMaxGroups; ++g)
+    if (b->nslot[g] > 0) used[nused++] = g;
+  for (int u = 0; u < nused; ++u) {
+    const int g = used[u];
     const int ns = b->nslot[g];
-    if (ns == 0) continue;
-    hipStream_t s = g == 0 ? b->cs : b->cs1;
-    if (b->nslot[1 - g] > 0) {
-      if (g == 1)
-        HP_HIP(hipStreamWaitEvent(s, tick_event(b, t, 0, 2), 0));
+    hipStream_t s = b->gs[g];
+    if (nused > 1) {
+      if (u > 0)
+        HP_HIP(hipStreamWaitEvent(s, tick_event(b, t, used[u - 1], 2), 0));
       else if (t > 0)
-        HP_HIP(hipStreamWaitEvent(s, tick_event(b, t - 1, 1, 2), 0));
+        HP_HIP(hipStreamWaitEvent(s, tick_event(b, t - 1, used[nused - 1], 2), 0));
     }
     const int *slot_list = b->d_slot_list + b->slot_off[g];
     HP_HIP(hipEventRecord(tick_event(b, t, g, 0), s));
@@ -689,7 +702,7 @@ int account_ticks(hp_batch *b) {
   b->stats.ticks += b->cfg.ticks_per_step;
   if (b->cfg.use_graphs) return 0;  // events inside a replayed graph are not timed
   for (int t = 0; t < b->cfg.ticks_per_step; ++t) {
-    for (int g = 0; g < kGroups; ++g) {
+    for (int g = 0; g < kMaxGroups; ++g) {
       if (b->nslot[g] == 0) continue;
       float ms_w = 0.f, ms_i = 0.f;
       HP_HIP(hipEventElapsedTime(&ms_w, tick_event(b, t, g, 1), tick_event(b, t, g, 2)));
@@ -701,17 +714,20 @@ int account_ticks(hp_batch *b) {
   return 0;
 }
 
-// The ticks of one step and the read-back of the slot records; the second
-// group's stream forks from and joins the compute stream.
+// The ticks of one step and the read-back of the slot records; the other
+// groups' streams fork from and join the compute stream.
 int enqueue_step_work(hp_batch *b) {
+  const int G = b->cfg.groups;
   HP_HIP(hipEventRecord(b->ev_fork, b->cs));
-  HP_HIP(hipStreamWaitEvent(b->cs1, b->ev_fork, 0));
+  for (int g = 1; g < G; ++g) HP_HIP(hipStreamWaitEvent(b->gs[g], b->ev_fork, 0));
   for (int t = 0; t < b->cfg.ticks_per_step; ++t) {
     int rc = enqueue_tick(b, t);
     if (rc) return rc;
   }
-  HP_HIP(hipEventRecord(b->ev_join, b->cs1));
-  HP_HIP(hipStreamWaitEvent(b->cs, b->ev_join, 0));
+  for (int g = 1; g < G; ++g) {
+    HP_HIP(hipEventRecord(b->ev_join[g], b->gs[g]));
+    HP_HIP(hipStreamWaitEvent(b->cs, b->ev_join[g], 0));
+  }
   HP_HIP(hipMemcpyAsync(b->h_slots, b->slots, b->cfg.max_inflight * sizeof(HpSlot), hipMemcpyDeviceToHost, b->cs));
   return 0;
 }
@@ -785,9 +801,11 @@ int hp_batch_create(int device, const hp_batch_config *cfg, hp_batch **out) {
       cfg->max_inflight <= 0 || cfg->qp_capacity < 0 || cfg->max_molecule_points <= 0 || cfg->ticks_per_step <= 0)
     return fail(HP_E_ARG, "batch capacities must be positive");
   if (cfg->max_inflight > 4096) return fail(HP_E_ARG, "at most 4096 molecules in flight");
+  if (cfg->groups < 0 || cfg->groups > kMaxGroups) return fail(HP_E_ARG, "groups must be in [0, 4]");
   hp_batch *b = new hp_batch();
   b->device = device;
   b->cfg = *cfg;
+  if (b->cfg.groups == 0) b->cfg.groups = kDefaultGroups;
   b->cfg.max_molecule_points = std::min(cfg->max_molecule_points, cfg->point_capacity);
   auto bail = [&](int rc) {
     free_batch(b);
@@ -796,9 +814,13 @@ int hp_batch_create(int device, const hp_batch_config *cfg, hp_batch **out) {
   hipError_t e = hipSetDevice(device);
   if (e != hipSuccess) return bail(fail(HP_E_HIP, std::string("hipSetDevice: ") + hipGetErrorString(e)));
   if ((e = hipStreamCreateWithFlags(&b->cs, hipStreamNonBlocking)) != hipSuccess ||
-      (e = hipStreamCreateWithFlags(&b->cs1, hipStreamNonBlocking)) != hipSuccess ||
       (e = hipStreamCreateWithFlags(&b->us, hipStreamNonBlocking)) != hipSuccess)
     return bail(fail(HP_E_HIP, std::string("hipStreamCreate: ") + hipGetErrorString(e)));
+  b->gs[0] = b->cs;
+  for (int g = 1; g < b->cfg.groups; ++g)
+    if ((e = hipStreamCreateWithFlags(&b->gs[g], hipStreamNonBlocking)) != hipSuccess ||
+        (e = hipEventCreate(&b->ev_join[g])) != hipSuccess)
+      return bail(fail(HP_E_HIP, std::string("hipStreamCreate: ") + hipGetErrorString(e)));
   const int64_t NP = cfg->point_capacity, NA = cfg->atom_capacity, NS = cfg->shell_capacity,
                 NF = cfg->function_capacity, NQ = std::max<int64_t>(1, cfg->qp_capacity), MI = cfg->max_inflight,
                 MP = b->cfg.max_molecule_points;
@@ -836,10 +858,10 @@ int hp_batch_create(int device, const hp_batch_config *cfg, hp_batch **out) {
   for (auto &ev : b->up_ev)
     if ((e = hipEventCreateWithFlags(&ev, hipEventDisableTiming)) != hipSuccess)
       return bail(fail(HP_E_HIP, "hipEventCreate failed"));
-  b->tev.assign(5 * (size_t)kGroups * cfg->ticks_per_step, nullptr);
+  b->tev.assign(5 * (size_t)kMaxGroups * cfg->ticks_per_step, nullptr);
   for (auto &ev : b->tev)
     if ((e = hipEventCreate(&ev)) != hipSuccess) return bail(fail(HP_E_HIP, "hipEventCreate failed"));
-  for (hipEvent_t *ev : {&b->ev_fork, &b->ev_join, &b->ev_step[0], &b->ev_step[1]})
+  for (hipEvent_t *ev : {&b->ev_fork, &b->ev_step[0], &b->ev_step[1]})
     if ((e = hipEventCreate(ev)) != hipSuccess) return bail(fail(HP_E_HIP, "hipEventCreate failed"));
   b->last_niter.assign(MI, 0);
   for (int i = (int)MI - 1; i >= 0; --i) b->free_slots.push_back(i);
@@ -859,8 +881,8 @@ int hp_batch_create(int device, const hp_batch_config *cfg, hp_batch **out) {
 int hp_batch_destroy(hp_batch *b) {
   if (!b) return fail(HP_E_ARG, "NULL batch");
   (void)hipSetDevice(b->device);
-  (void)hipStreamSynchronize(b->cs);
-  (void)hipStreamSynchronize(b->cs1);
+  for (int g = 0; g < kMaxGroups; ++g)
+    if (b->gs[g]) (void)hipStreamSynchronize(b->gs[g]);
   (void)hipStreamSynchronize(b->us);
   free_batch(b);
   return 0;
@@ -1022,8 +1044,8 @@ int hp_host_free(void *ptr) {
 int hp_batch_get_stats(hp_batch *b, hp_batch_stats *out) {
   if (!b || !out) return fail(HP_E_ARG, "NULL argument");
   HP_HIP(hipSetDevice(b->device));
-  HP_HIP(hipStreamSynchronize(b->cs1));
-  HP_HIP(hipStreamSynchronize(b->cs));
+  for (int g = 0; g < kMaxGroups; ++g)
+    if (b->gs[g]) HP_HIP(hipStreamSynchronize(b->gs[g]));
   double d32 = 0.0, d64 = 0.0;
   if (int rc = hp::read_eval_counters(b->evals_done, &d32, &d64)) return rc;
   *out = b->stats;

@@ -154,6 +154,7 @@ class BatchConfig(ctypes.Structure):
         ("density_cutoff", _d),
         ("negative_cutoff", _d),
         ("population_cutoff", _d),
+        ("groups", ctypes.c_int32),
     ]
 
 

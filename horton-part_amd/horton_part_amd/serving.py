@@ -106,6 +106,7 @@ class PartitionServer:
         density_cutoff=DENSITY_CUTOFF,
         negative_cutoff=NEGATIVE_CUTOFF,
         population_cutoff=POPULATION_CUTOFF,
+        groups=0,
     ):
         self.lib = _native.load()
         cfg = _native.BatchConfig()
@@ -123,6 +124,7 @@ class PartitionServer:
         cfg.density_cutoff = float(density_cutoff)
         cfg.negative_cutoff = float(negative_cutoff)
         cfg.population_cutoff = float(population_cutoff)
+        cfg.groups = int(groups)
         self.config = cfg
         handle = ctypes.c_void_p()
         _native.check(self.lib.hp_batch_create(int(device), ctypes.byref(cfg), ctypes.byref(handle)))

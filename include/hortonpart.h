@@ -255,6 +255,11 @@ typedef struct {
   double density_cutoff;
   double negative_cutoff;
   double population_cutoff;
+  int32_t groups;            /* molecule groups of equal weights cost, 1..4
+                                (0 selects 2), one stream each: the groups'
+                                weights launches take turns, so that each
+                                group's per-atom solves run beside the other
+                                groups' weights passes */
 } hp_batch_config;
 
 /* Why a request left the batch (hp_batch_result.event) */

@@ -124,12 +124,12 @@ def _alone(scheme, wname, precision, cache={}):
     return cache[key]
 
 
-@pytest.mark.parametrize("graphs", [True, False])
-def test_batch_mix_equals_alone(graphs):
+@pytest.mark.parametrize("graphs,groups", [(True, 2), (False, 2), (False, 1), (True, 3), (False, 4)])
+def test_batch_mix_equals_alone(graphs, groups):
     """Requests arrive while others run and wait for one of 3 slots."""
     require_gpu()
-    _say(f"case: batch mix of {len(MIX)} requests, 3 slots, graphs={graphs}")
-    with _server(max_inflight=3, use_graphs=graphs) as srv:
+    _say(f"case: batch mix of {len(MIX)} requests, 3 slots, graphs={graphs}, groups={groups}")
+    with _server(max_inflight=3, use_graphs=graphs, groups=groups) as srv:
         tickets = {}
         done = []
         for i, req in enumerate(MIX):
