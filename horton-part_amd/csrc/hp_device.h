@@ -969,7 +969,8 @@ __global__ __launch_bounds__(512) void k_mbis_inner(const int *__restrict__ done
         Sp[k] = 3.0 * red[2 * k] / red[2 * k + 1];
       }
     }
-    const double change = have_old ? sqrt(red[2 * KM]) : 1e100;
+    // a squared change is 0 or far above 2^-767: the square root needs no rescaling
+    const double change = have_old ? fast_sqrt<double>(red[2 * KM]) : 1e100;
     if (change < thr) {
       conv = true;
       break;
