@@ -890,6 +890,8 @@ __device__ __forceinline__ void block_allsum(double *v, double *lds, int parity)
   for (int i = 0; i < NV; ++i) v[i] = t[i];
 }
 
+#define HP_INV_8PI 0.039788735772973836  // 1 / (8 pi), rounded to the nearest double
+
 template <int KM>
 __global__ __launch_bounds__(512) void k_mbis_inner(const int *__restrict__ done, DevMol m,
                                                     const double *__restrict__ rhobar, double *__restrict__ par,
@@ -940,8 +942,9 @@ __global__ __launch_bounds__(512) void k_mbis_inner(const int *__restrict__ done
     double pro = 0.0;
 #pragma unroll
     for (int k = 0; k < KM; ++k) {
-      // N S^3 / (8 pi), once per shell instead of a division per point
-      const double coef = Np[k] * (Sp[k] * Sp[k] * Sp[k]) / (8.0 * HP_PI);
+      // N S^3 / (8 pi), once per shell; the constant division as a product
+      // with its rounded reciprocal (within one rounding of the quotient)
+      const double coef = Np[k] * (Sp[k] * Sp[k] * Sp[k]) * HP_INV_8PI;
       if (k < K) {
         t[k] = coef * exp_tab(-Sp[k] * r, s_t64);
         pro += t[k];
@@ -985,8 +988,8 @@ __global__ __launch_bounds__(512) void k_mbis_inner(const int *__restrict__ done
     if (k < K) {
       const double N1 = Np[k], S1 = Sp[k];
       const double N2 = old_par[2 * (f0 + k)], S2 = old_par[2 * (f0 + k) + 1];
-      y1 += N1 * (S1 * S1 * S1) / (8.0 * HP_PI) * exp_tab(-S1 * r, s_t64);
-      y2 += N2 * (S2 * S2 * S2) / (8.0 * HP_PI) * exp_tab(-S2 * r, s_t64);
+      y1 += N1 * (S1 * S1 * S1) * HP_INV_8PI * exp_tab(-S1 * r, s_t64);
+      y2 += N2 * (S2 * S2 * S2) * HP_INV_8PI * exp_tab(-S2 * r, s_t64);
     }
   }
   const double dy = y1 - y2;
