@@ -326,6 +326,7 @@ def run_serving(args):
         ticks_per_step=args.ticks,
         use_graphs=args.graphs,
         groups=args.groups,
+        inner_budget=args.inner_budget,
     )
     if args.warmup:
         log(f"warmup: {args.warmup} request(s) served together, untimed")
@@ -392,6 +393,7 @@ def run_serving(args):
             "ticks_per_step": args.ticks,
             "graphs": args.graphs,
             "groups": args.groups or 2,
+            "inner_budget": args.inner_budget,
         },
         "p50_ms": float(np.percentile(lat, 50)) * 1e3,
         "p99_ms": float(np.percentile(lat, 99)) * 1e3,
@@ -464,6 +466,8 @@ def main():
                     choices=["serving", "water", "chain20", "chain20-gisa", "cluster200", "protein2000"])
     ap.add_argument("--ticks", type=int, default=4, help="serving: outer iterations per host step")
     ap.add_argument("--groups", type=int, default=0, help="serving: molecule groups interleaved per tick (1-4; 0: 2)")
+    ap.add_argument("--inner-budget", type=int, default=0,
+                    help="serving: MBIS inner iterations per atom and launch (0: no limit)")
     ap.add_argument("--graphs", action="store_true", help="serving: replay the ticks of a step as a hipGraph (times whole steps only)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--serve-precision", choices=("mixed", "fp64", "auto"), default="auto",

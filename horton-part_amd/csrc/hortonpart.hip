@@ -226,7 +226,7 @@ int launch_inner_ppl(hp_part *p) {
 #define HP_MBIS(KMV)                                                                                                  \
   hipLaunchKernelGGL((k_mbis_inner<KMV>), g1, b1, 0, p->stream, p->state, p->mol(), p->rhobar, p->par, p->old_par,    \
                      p->fn_off, p->inner_threshold, p->inner_maxiter, p->inner_niter, p->status, p->charges,          \
-                     p->change_part)
+                     p->change_part, 0, nullptr, nullptr)
     if (p->maxK <= 2)
       HP_MBIS(2);
     else if (p->maxK <= 3)

@@ -132,6 +132,10 @@ struct hp_batch {
   double *par_l = nullptr, *old_l = nullptr, *par_m = nullptr, *old_m = nullptr;
   double *qpP = nullptr;
   unsigned long long *evals_done = nullptr;  // [FP32, FP64] evaluations performed
+  // MBIS inner solves spread over launches: per atom the iterations done (-1:
+  // none in progress), per radial shell the previous pro-atom value
+  int *ist_it = nullptr;
+  double *ist_oldpro = nullptr;
   // slots
   HpSlot *slots = nullptr, *h_slots = nullptr, *h_slot_init = nullptr;
   std::vector<int> slot_state;
@@ -224,7 +228,7 @@ void free_batch(hp_batch *b) {
                   b->coef,       b->alpha,       b->fn_norm,     b->fn_alpha,    b->fn_order,  b->par_l,
                   b->old_l,      b->par_m,       b->old_m,       b->qpP,         b->slots,     b->st_points,
                   b->st_coords,  b->st_shell_atom, b->st_fn_off, b->st_shell_off, b->d_work,  b->d_slot_list,
-                  b->evals_done};
+                  b->evals_done,  b->ist_it,     b->ist_oldpro};
   for (void *p : bufs)
     if (p) (void)hipFree(p);
   if (b->h_slots) (void)hipHostFree(b->h_slots);
@@ -429,7 +433,7 @@ int upload(hp_batch *b, Request *r) {
   ad.has_qp = r->scheme01 == 1;
   hipLaunchKernelGGL(k_batch_admit, dim3(1), dim3(256), 0, s, ad, b->st_coords, b->st_shell_atom, b->st_shell_off,
                      b->st_fn_off, b->cx, b->cy, b->cz, b->shell_atom, b->shell_off, b->atom_shell_off, b->fn_off,
-                     b->qp_off, b->inner_niter, b->status, b->status_all, b->charges, b->change_part);
+                     b->qp_off, b->inner_niter, b->status, b->status_all, b->charges, b->change_part, b->ist_it);
   HP_LAUNCHED();
   // the slot record: idle until the host lets the molecule join the lists
   HpSlot &h = b->h_slot_init[r->slot];
@@ -659,7 +663,7 @@ int enqueue_tick(hp_batch *b, int t) {
 #define HP_BMBIS(KMV)                                                                                          \
   hipLaunchKernelGGL((k_mbis_inner<KMV>), dim3((unsigned)L.len), dim3(L.block), 0, s, nullptr, m, b->rhobar,    \
                      b->par_m, b->old_m, b->fn_off, b->cfg.inner_threshold, b->cfg.inner_maxiter, b->inner_niter, \
-                     b->status, b->charges, b->change_part)
+                     b->status, b->charges, b->change_part, b->cfg.inner_budget, b->ist_it, b->ist_oldpro)
         if (L.km == 2)
           HP_BMBIS(2);
         else if (L.km == 3)
@@ -687,7 +691,7 @@ int enqueue_tick(hp_batch *b, int t) {
     if (!inner_started) HP_HIP(hipEventRecord(tick_event(b, t, g, 2), s));
     HP_HIP(hipEventRecord(tick_event(b, t, g, 3), s));
     hipLaunchKernelGGL(k_batch_finalize, dim3(ns), dim3(256), 0, s, b->slots, slot_list, b->ent_part, b->change_part,
-                       b->status, b->status_all, b->par_m, b->old_m, b->par_l, b->old_l);
+                       b->status, b->status_all, b->par_m, b->old_m, b->par_l, b->old_l, b->ist_it);
     HP_LAUNCHED();
     HP_HIP(hipEventRecord(tick_event(b, t, g, 4), s));
   }
@@ -802,6 +806,7 @@ int hp_batch_create(int device, const hp_batch_config *cfg, hp_batch **out) {
     return fail(HP_E_ARG, "batch capacities must be positive");
   if (cfg->max_inflight > 4096) return fail(HP_E_ARG, "at most 4096 molecules in flight");
   if (cfg->groups < 0 || cfg->groups > kMaxGroups) return fail(HP_E_ARG, "groups must be in [0, 4]");
+  if (cfg->inner_budget < 0) return fail(HP_E_ARG, "inner_budget must be >= 0");
   hp_batch *b = new hp_batch();
   b->device = device;
   b->cfg = *cfg;
@@ -842,7 +847,8 @@ int hp_batch_create(int device, const hp_batch_config *cfg, hp_batch **out) {
       (rc = dalloc(&b->st_coords, 3 * (NA - 1))) || (rc = dalloc(&b->st_shell_atom, NS)) ||
       (rc = dalloc(&b->st_fn_off, NA)) || (rc = dalloc(&b->st_shell_off, NS)) ||
       (rc = dalloc(&b->d_work, b->work_cap)) || (rc = dalloc(&b->d_slot_list, MI)) ||
-      (rc = dalloc(&b->evals_done, HP_EVAL_SLOTS)))
+      (rc = dalloc(&b->evals_done, HP_EVAL_SLOTS)) || (rc = dalloc(&b->ist_it, NA)) ||
+      (rc = dalloc(&b->ist_oldpro, NS)))
     return bail(rc);
   if ((e = hipMemset(b->evals_done, 0, HP_EVAL_SLOTS * sizeof(unsigned long long))) != hipSuccess)
     return bail(fail(HP_E_HIP, std::string("hipMemset: ") + hipGetErrorString(e)));

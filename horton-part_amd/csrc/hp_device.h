@@ -76,7 +76,7 @@ struct HpSlot {
   int corr_period;     // FP32 stage: FP64 refresh of the correction every this many iterations (0: none)
   int refresh;         // this iteration refreshes the correction (set by k_batch_begin)
   int full_sum;        // weights pass: every atom on every point (no tail skipping)
-  int pad;
+  int phase;           // 0: next tick starts an outer iteration; 1: its inner solves continue
   double thr;          // threshold of the current stage
   double threshold;    // the request's threshold
   double change, entropy;
@@ -629,6 +629,7 @@ __global__ __launch_bounds__(512) void k_weights(const int *__restrict__ done, D
   if (m.work) {
     const int2 wk = m.work[blockIdx.x];
     const HpSlot &sl = m.slots[wk.y];
+    if (sl.phase) return;  // its inner solves of this outer iteration are still running
     full = sl.full_sum;
     // a molecule takes part in the launches of its current stage, and an
     // FP32-stage molecule also in the FP64 launch of a refresh iteration
@@ -898,7 +899,8 @@ __global__ __launch_bounds__(512) void k_mbis_inner(const int *__restrict__ done
                                                     const double *__restrict__ old_par, const int *__restrict__ fn_off,
                                                     double thr, int maxit, int *__restrict__ inner_niter,
                                                     int *__restrict__ status, double *__restrict__ charges,
-                                                    double *__restrict__ change_part) {
+                                                    double *__restrict__ change_part, int budget,
+                                                    int *__restrict__ ist_it, double *__restrict__ ist_oldpro) {
   constexpr int NV = 2 * KM + 1;
   __shared__ double lds[2 * 16 * NV];
   __shared__ double s_t64[64];  // 2^(j/64) for the exponential (published by the first reduction)
@@ -908,10 +910,16 @@ __global__ __launch_bounds__(512) void k_mbis_inner(const int *__restrict__ done
   __builtin_amdgcn_s_setprio(3);
   if (threadIdx.x < 64) s_t64[threadIdx.x] = hp_exp2_64[threadIdx.x];
   int a = blockIdx.x;
+  bool resume = false;
   if (m.work) {
     const int2 wk = m.work[blockIdx.x];
-    if (m.slots[wk.y].run == HP_RUN_IDLE) return;
+    const HpSlot &sl = m.slots[wk.y];
+    if (sl.run == HP_RUN_IDLE) return;
     a = wk.x;
+    if (sl.phase) {
+      if (ist_it[a] < 0) return;  // this atom's solve ended in an earlier launch
+      resume = true;
+    }
   }
   const int s0 = m.atom_shell_off[a], R = m.atom_shell_off[a + 1] - s0;
   const int f0 = fn_off[a], K = fn_off[a + 1] - f0;
@@ -933,11 +941,15 @@ __global__ __launch_bounds__(512) void k_mbis_inner(const int *__restrict__ done
     Np[k] = k < K ? par[2 * (f0 + k)] : 0.0;
     Sp[k] = k < K ? par[2 * (f0 + k) + 1] : 1.0;
   }
-  double oldpro = 0.0;
-  bool have_old = false, conv = false;
+  // a solve resumed from an earlier launch continues from its saved state:
+  // the iterate (in par), the previous pro-atom and the iterations done
+  const int it0 = resume ? ist_it[a] : 0;
+  double oldpro = (resume && valid) ? ist_oldpro[s0 + i] : 0.0;
+  bool have_old = it0 > 0, conv = false;
   int it = 0;
+  const int it_end = (budget > 0 && it0 + budget < maxit) ? it0 + budget : maxit;
   const double dc = 1e-15;  // opt_mbis_propars default density_cutoff (mbis.py:82)
-  for (it = 0; it < maxit; ++it) {
+  for (it = it0; it < it_end; ++it) {
     double t[KM];
     double pro = 0.0;
 #pragma unroll
@@ -981,6 +993,22 @@ __global__ __launch_bounds__(512) void k_mbis_inner(const int *__restrict__ done
     oldpro = pro;
     have_old = true;
   }
+  if (!conv && it < maxit) {
+    // iteration budget of this launch spent: keep the state for the next one
+    if (valid) ist_oldpro[s0 + i] = oldpro;
+    if (threadIdx.x == 0) {
+#pragma unroll
+      for (int k = 0; k < KM; ++k) {
+        if (k < K) {
+          par[2 * (f0 + k)] = Np[k];
+          par[2 * (f0 + k) + 1] = Sp[k];
+        }
+      }
+      ist_it[a] = it;
+    }
+    return;
+  }
+  if (ist_it && threadIdx.x == 0) ist_it[a] = -1;
   // change of the outer loop: new vs old pro-atom on the radial grid
   double y1 = 0.0, y2 = 0.0;
 #pragma unroll
@@ -1710,7 +1738,7 @@ static __global__ __launch_bounds__(256) void k_batch_begin(HpSlot *__restrict__
                                                      const double *__restrict__ alpha_fixed, double *__restrict__ coef,
                                                      double *__restrict__ alpha) {
   HpSlot &sl = slots[slot_list[blockIdx.x]];
-  if (sl.run == HP_RUN_IDLE) return;
+  if (sl.run == HP_RUN_IDLE || sl.phase) return;  // idle, or inner solves still running
   __syncthreads();  // every thread has read run before thread 0 writes the slot
   if (threadIdx.x == 0)
     sl.refresh = sl.run == HP_RUN_FP32 && sl.corr_period > 0 && sl.niter % sl.corr_period == 0;
@@ -1739,12 +1767,20 @@ static __global__ __launch_bounds__(256) void k_batch_finalize(HpSlot *__restric
                                                         const double *__restrict__ change_part,
                                                         const int *__restrict__ status, int *__restrict__ status_all,
                                                         double *__restrict__ par_m, const double *__restrict__ old_m,
-                                                        double *__restrict__ par_l, const double *__restrict__ old_l) {
+                                                        double *__restrict__ par_l, const double *__restrict__ old_l,
+                                                        const int *__restrict__ ist_it) {
   __shared__ double lds[32];
   __shared__ int s_rollback;
   HpSlot &sl = slots[slot_list[blockIdx.x]];
   if (sl.run == HP_RUN_IDLE) return;
   const int s0 = sl.s0, ns = sl.ns, a0 = sl.a0, na = sl.na, f0 = sl.f0, nf = sl.nf, scheme = sl.scheme;
+  // an atom whose inner solve is still running keeps the molecule in this
+  // outer iteration (phase 1: no begin, no weights pass until it ends)
+  int pend = 0;
+  for (int i = threadIdx.x; i < na; i += blockDim.x) pend |= ist_it[a0 + i] >= 0;
+  pend = __syncthreads_or(pend);
+  if (threadIdx.x == 0) sl.phase = pend ? 1 : 0;
+  if (pend) return;
   double e = 0.0, dummy = 0.0;
   for (int s = threadIdx.x; s < ns; s += blockDim.x) e += ent_part[s0 + s];
   block_sum2(e, dummy, lds);
@@ -1841,9 +1877,11 @@ static __global__ __launch_bounds__(256) void k_batch_admit(HpAdmit ad, const do
                                                      int *__restrict__ atom_shell_off, int *__restrict__ fn_off,
                                                      int64_t *__restrict__ qp_off, int *__restrict__ inner_niter,
                                                      int *__restrict__ status, int *__restrict__ status_all,
-                                                     double *__restrict__ charges, double *__restrict__ change_part) {
+                                                     double *__restrict__ charges, double *__restrict__ change_part,
+                                                     int *__restrict__ ist_it) {
   const int t = threadIdx.x;
   for (int a = t; a < ad.na; a += blockDim.x) {
+    ist_it[ad.a0 + a] = -1;  // no inner solve in progress
     cx[ad.a0 + a] = coords[3 * a];
     cy[ad.a0 + a] = coords[3 * a + 1];
     cz[ad.a0 + a] = coords[3 * a + 2];

@@ -155,6 +155,7 @@ class BatchConfig(ctypes.Structure):
         ("negative_cutoff", _d),
         ("population_cutoff", _d),
         ("groups", ctypes.c_int32),
+        ("inner_budget", ctypes.c_int32),
     ]
 
 

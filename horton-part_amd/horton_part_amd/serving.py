@@ -107,6 +107,7 @@ class PartitionServer:
         negative_cutoff=NEGATIVE_CUTOFF,
         population_cutoff=POPULATION_CUTOFF,
         groups=0,
+        inner_budget=0,
     ):
         self.lib = _native.load()
         cfg = _native.BatchConfig()
@@ -125,6 +126,7 @@ class PartitionServer:
         cfg.negative_cutoff = float(negative_cutoff)
         cfg.population_cutoff = float(population_cutoff)
         cfg.groups = int(groups)
+        cfg.inner_budget = int(inner_budget)
         self.config = cfg
         handle = ctypes.c_void_p()
         _native.check(self.lib.hp_batch_create(int(device), ctypes.byref(cfg), ctypes.byref(handle)))

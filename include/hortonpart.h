@@ -260,6 +260,12 @@ typedef struct {
                                 weights launches take turns, so that each
                                 group's per-atom solves run beside the other
                                 groups' weights passes */
+  int32_t inner_budget;      /* MBIS inner iterations per atom and launch
+                                (0: no limit).  An atom whose inner solve needs
+                                more keeps its state on the device and goes on
+                                in the next ticks, while its molecule skips the
+                                start of new outer iterations; the iterations
+                                are the same, so results do not change. */
 } hp_batch_config;
 
 /* Why a request left the batch (hp_batch_result.event) */

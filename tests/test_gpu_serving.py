@@ -124,12 +124,16 @@ def _alone(scheme, wname, precision, cache={}):
     return cache[key]
 
 
-@pytest.mark.parametrize("graphs,groups", [(True, 2), (False, 2), (False, 1), (True, 3), (False, 4)])
-def test_batch_mix_equals_alone(graphs, groups):
-    """Requests arrive while others run and wait for one of 3 slots."""
+@pytest.mark.parametrize(
+    "graphs,groups,budget",
+    [(True, 2, 0), (False, 2, 0), (False, 1, 0), (True, 3, 0), (False, 4, 0), (False, 2, 16), (True, 2, 64)],
+)
+def test_batch_mix_equals_alone(graphs, groups, budget):
+    """Requests arrive while others run and wait for one of 3 slots.  With an
+    inner budget the MBIS solves run over several ticks: same results."""
     require_gpu()
-    _say(f"case: batch mix of {len(MIX)} requests, 3 slots, graphs={graphs}, groups={groups}")
-    with _server(max_inflight=3, use_graphs=graphs, groups=groups) as srv:
+    _say(f"case: batch mix of {len(MIX)} requests, 3 slots, graphs={graphs}, groups={groups}, inner budget={budget}")
+    with _server(max_inflight=3, use_graphs=graphs, groups=groups, inner_budget=budget) as srv:
         tickets = {}
         done = []
         for i, req in enumerate(MIX):
