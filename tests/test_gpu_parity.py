@@ -342,3 +342,29 @@ def test_tail_skip_equals_full_sum(scheme, natom, seed):
     assert res[True][0] == res[False][0]
     assert dq <= 1e-12
     assert rel <= 1e-10
+
+
+def test_tail_skip_with_cutoff_radius():
+    """Tail skipping on top of the cut-off lists (a 60-atom cluster, 8-bohr
+    cut-off): against the oracle with the same cut-off, and against the full
+    sum of the candidates."""
+    require_gpu()
+    _say("case: MBIS 8-bohr cut-off with tail skipping, cluster 60 atoms seed 13, 30x26")
+    w = workloads.cluster(natom=60, seed=13, nrad=30, nang=26, device=0)
+    args = (w.coordinates, w.numbers, w.pseudo_numbers, w.grid, w.density)
+    kw = dict(maxiter=10, cutoff_radius=8.0)
+    part = MBISWPart(*args, **kw)
+    part.do_partitioning()
+    frac = _evals_done_frac(part)
+    full = MBISWPart(*args, full_sum=True, **kw)
+    full.do_partitioning()
+    frac_full = _evals_done_frac(full)
+    dq_full = np.max(np.abs(part["charges"] - full["charges"]))
+    _say(f"  evaluations performed / algorithmic: skipping {frac:.3f}, full {frac_full:.3f}; max|dq| vs full {dq_full:.2e}")
+    assert frac < frac_full
+    assert part["niter"] == full["niter"]
+    assert dq_full <= 1e-12
+    oracle = OracleMBIS(*args, **kw)
+    cache = oracle.do_partitioning()
+    _compare(part, cache, oracle)
+    full.close()
