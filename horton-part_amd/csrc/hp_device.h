@@ -893,6 +893,139 @@ __device__ __forceinline__ void block_allsum(double *v, double *lds, int parity)
 
 #define HP_INV_8PI 0.039788735772973836  // 1 / (8 pi), rounded to the nearest double
 
+// Per-block context of one atom's MBIS solve.
+struct MbisCtx {
+  const DevMol *m;
+  double *par;
+  const double *old_par;
+  const double *s_t64;
+  double *lds;
+  int *inner_niter, *status, *ist_it;
+  double *charges, *change_part, *ist_oldpro;
+  double thr, r, wr, w, rho, pop;
+  int a, f0, s0, i, maxit, budget;
+  bool valid, resume;
+};
+
+// The solve for an atom of exactly KC shells (no per-shell guards in the
+// loop; 2 KC + 1 sums per iteration).
+template <int KC>
+__device__ __forceinline__ void mbis_solve(const MbisCtx &c, int slot) {
+  constexpr int NV = 2 * KC + 1;
+  const double r = c.r, w = c.w, rho = c.rho;
+  double Np[KC], Sp[KC];
+#pragma unroll
+  for (int k = 0; k < KC; ++k) {
+    Np[k] = c.par[2 * (c.f0 + k)];
+    Sp[k] = c.par[2 * (c.f0 + k) + 1];
+  }
+  // a solve resumed from an earlier launch continues from its saved state:
+  // the iterate (in par), the previous pro-atom and the iterations done
+  const int it0 = c.resume ? c.ist_it[c.a] : 0;
+  double oldpro = (c.resume && c.valid) ? c.ist_oldpro[c.s0 + c.i] : 0.0;
+  bool have_old = it0 > 0, conv = false;
+  int it = 0;
+  const int it_end = (c.budget > 0 && it0 + c.budget < c.maxit) ? it0 + c.budget : c.maxit;
+  const double dc = 1e-15;  // opt_mbis_propars default density_cutoff (mbis.py:82)
+  for (it = it0; it < it_end; ++it) {
+    double t[KC];
+    double pro = 0.0;
+#pragma unroll
+    for (int k = 0; k < KC; ++k) {
+      // N S^3 / (8 pi), once per shell; the constant division as a product
+      // with its rounded reciprocal (within one rounding of the quotient)
+      const double coef = Np[k] * (Sp[k] * Sp[k] * Sp[k]) * HP_INV_8PI;
+      t[k] = coef * exp_tab(-Sp[k] * r, c.s_t64);
+      pro += t[k];
+    }
+    const bool sick = (rho < dc) || (pro < dc);
+    const double ratio = sick ? 0.0 : rho / pro;
+    double red[NV];
+#pragma unroll
+    for (int k = 0; k < KC; ++k) {
+      const double wt = w * (t[k] * ratio);
+      red[2 * k] = wt;
+      red[2 * k + 1] = wt * r;
+    }
+    const double e = oldpro - pro;
+    red[2 * KC] = w * e * e;
+    block_allsum<NV>(red, c.lds, slot);
+    slot ^= 1;
+#pragma unroll
+    for (int k = 0; k < KC; ++k) {
+      Np[k] = red[2 * k];
+      Sp[k] = 3.0 * red[2 * k] / red[2 * k + 1];
+    }
+    // a squared change is 0 or far above 2^-767: the square root needs no rescaling
+    const double change = have_old ? fast_sqrt<double>(red[2 * KC]) : 1e100;
+    if (change < c.thr) {
+      conv = true;
+      break;
+    }
+    oldpro = pro;
+    have_old = true;
+  }
+  if (!conv && it < c.maxit) {
+    // iteration budget of this launch spent: keep the state for the next one
+    if (c.valid) c.ist_oldpro[c.s0 + c.i] = oldpro;
+    if (threadIdx.x == 0) {
+#pragma unroll
+      for (int k = 0; k < KC; ++k) {
+        c.par[2 * (c.f0 + k)] = Np[k];
+        c.par[2 * (c.f0 + k) + 1] = Sp[k];
+      }
+      c.ist_it[c.a] = it;
+    }
+    return;
+  }
+  if (c.ist_it && threadIdx.x == 0) c.ist_it[c.a] = -1;
+  // change of the outer loop: new vs old pro-atom on the radial grid
+  double y1 = 0.0, y2 = 0.0;
+#pragma unroll
+  for (int k = 0; k < KC; ++k) {
+    const double N1 = Np[k], S1 = Sp[k];
+    const double N2 = c.old_par[2 * (c.f0 + k)], S2 = c.old_par[2 * (c.f0 + k) + 1];
+    y1 += N1 * (S1 * S1 * S1) * HP_INV_8PI * exp_tab(-S1 * r, c.s_t64);
+    y2 += N2 * (S2 * S2 * S2) * HP_INV_8PI * exp_tab(-S2 * r, c.s_t64);
+  }
+  const double dy = y1 - y2;
+  double msd = c.wr * (4.0 * HP_PI * (r * r)) * dy * dy;
+  block_allsum<1>(&msd, c.lds, slot);
+  if (threadIdx.x == 0) {
+    int st = 0;
+    double sumN = 0.0;
+#pragma unroll
+    for (int k = 0; k < KC; ++k) sumN += Np[k];
+    if (conv) {
+      // np.isclose(pop, sum, atol=1e-4): |pop - sum| <= 1e-4 + 1e-5 |sum|
+      if (!(fabs(c.pop - sumN) <= 1e-4 + 1e-5 * fabs(sumN))) st |= HP_ST_POP;
+    } else {
+      st |= HP_ST_NOCONV;
+    }
+#pragma unroll
+    for (int k = 0; k < KC; ++k) {
+      c.par[2 * (c.f0 + k)] = Np[k];
+      c.par[2 * (c.f0 + k) + 1] = Sp[k];
+    }
+    c.inner_niter[c.a] = conv ? it + 1 : c.maxit;
+    c.status[c.a] = st;
+    c.charges[c.a] = c.m->pseudo[c.a] - c.pop;
+    c.change_part[c.a] = msd;
+  }
+}
+
+// The solve for the atom's shell count K in [1, KM] (uniform per block).
+template <int KM, int KC>
+__device__ __forceinline__ void mbis_dispatch(int K, const MbisCtx &c, int slot) {
+  if constexpr (KC <= KM) {
+    if (K == KC) {
+      mbis_solve<KC>(c, slot);
+      return;
+    }
+    mbis_dispatch<KM, KC + 1>(K, c, slot);
+  }
+}
+
 template <int KM>
 __global__ __launch_bounds__(512) void k_mbis_inner(const int *__restrict__ done, DevMol m,
                                                     const double *__restrict__ rhobar, double *__restrict__ par,
@@ -922,131 +1055,40 @@ __global__ __launch_bounds__(512) void k_mbis_inner(const int *__restrict__ done
     }
   }
   const int s0 = m.atom_shell_off[a], R = m.atom_shell_off[a + 1] - s0;
-  const int f0 = fn_off[a], K = fn_off[a + 1] - f0;
+  const int f0 = fn_off[a], K = __builtin_amdgcn_readfirstlane(fn_off[a + 1] - f0);
   const int i = threadIdx.x;
   const bool valid = i < R;
-  const double r = valid ? m.shell_r[s0 + i] : 0.0;
-  const double wr = valid ? m.shell_w[s0 + i] : 0.0;
-  const double w = 4.0 * HP_PI * (r * r) * wr;
-  const double rho = valid ? rhobar[s0 + i] : 0.0;
+  MbisCtx c;
+  c.m = &m;
+  c.par = par;
+  c.old_par = old_par;
+  c.s_t64 = s_t64;
+  c.lds = lds;
+  c.inner_niter = inner_niter;
+  c.status = status;
+  c.ist_it = ist_it;
+  c.charges = charges;
+  c.change_part = change_part;
+  c.ist_oldpro = ist_oldpro;
+  c.thr = thr;
+  c.r = valid ? m.shell_r[s0 + i] : 0.0;
+  c.wr = valid ? m.shell_w[s0 + i] : 0.0;
+  c.w = 4.0 * HP_PI * (c.r * c.r) * c.wr;
+  c.rho = valid ? rhobar[s0 + i] : 0.0;
+  c.a = a;
+  c.f0 = f0;
+  c.s0 = s0;
+  c.i = i;
+  c.maxit = maxit;
+  c.budget = budget;
+  c.valid = valid;
+  c.resume = resume;
   int slot = 0;  // LDS slot of the next block reduction (alternates)
-  double pv = w * rho;
+  double pv = c.w * c.rho;
   block_allsum<1>(&pv, lds, slot);
   slot ^= 1;
-  const double pop = pv;
-
-  double Np[KM], Sp[KM];
-#pragma unroll
-  for (int k = 0; k < KM; ++k) {
-    Np[k] = k < K ? par[2 * (f0 + k)] : 0.0;
-    Sp[k] = k < K ? par[2 * (f0 + k) + 1] : 1.0;
-  }
-  // a solve resumed from an earlier launch continues from its saved state:
-  // the iterate (in par), the previous pro-atom and the iterations done
-  const int it0 = resume ? ist_it[a] : 0;
-  double oldpro = (resume && valid) ? ist_oldpro[s0 + i] : 0.0;
-  bool have_old = it0 > 0, conv = false;
-  int it = 0;
-  const int it_end = (budget > 0 && it0 + budget < maxit) ? it0 + budget : maxit;
-  const double dc = 1e-15;  // opt_mbis_propars default density_cutoff (mbis.py:82)
-  for (it = it0; it < it_end; ++it) {
-    double t[KM];
-    double pro = 0.0;
-#pragma unroll
-    for (int k = 0; k < KM; ++k) {
-      // N S^3 / (8 pi), once per shell; the constant division as a product
-      // with its rounded reciprocal (within one rounding of the quotient)
-      const double coef = Np[k] * (Sp[k] * Sp[k] * Sp[k]) * HP_INV_8PI;
-      if (k < K) {
-        t[k] = coef * exp_tab(-Sp[k] * r, s_t64);
-        pro += t[k];
-      } else {
-        t[k] = 0.0;
-      }
-    }
-    const bool sick = (rho < dc) || (pro < dc);
-    const double ratio = sick ? 0.0 : rho / pro;
-    double red[NV];
-#pragma unroll
-    for (int k = 0; k < KM; ++k) {
-      const double wt = w * (t[k] * ratio);
-      red[2 * k] = wt;
-      red[2 * k + 1] = wt * r;
-    }
-    const double e = oldpro - pro;
-    red[2 * KM] = w * e * e;
-    block_allsum<NV>(red, lds, slot);
-    slot ^= 1;
-#pragma unroll
-    for (int k = 0; k < KM; ++k) {
-      if (k < K) {
-        Np[k] = red[2 * k];
-        Sp[k] = 3.0 * red[2 * k] / red[2 * k + 1];
-      }
-    }
-    // a squared change is 0 or far above 2^-767: the square root needs no rescaling
-    const double change = have_old ? fast_sqrt<double>(red[2 * KM]) : 1e100;
-    if (change < thr) {
-      conv = true;
-      break;
-    }
-    oldpro = pro;
-    have_old = true;
-  }
-  if (!conv && it < maxit) {
-    // iteration budget of this launch spent: keep the state for the next one
-    if (valid) ist_oldpro[s0 + i] = oldpro;
-    if (threadIdx.x == 0) {
-#pragma unroll
-      for (int k = 0; k < KM; ++k) {
-        if (k < K) {
-          par[2 * (f0 + k)] = Np[k];
-          par[2 * (f0 + k) + 1] = Sp[k];
-        }
-      }
-      ist_it[a] = it;
-    }
-    return;
-  }
-  if (ist_it && threadIdx.x == 0) ist_it[a] = -1;
-  // change of the outer loop: new vs old pro-atom on the radial grid
-  double y1 = 0.0, y2 = 0.0;
-#pragma unroll
-  for (int k = 0; k < KM; ++k) {
-    if (k < K) {
-      const double N1 = Np[k], S1 = Sp[k];
-      const double N2 = old_par[2 * (f0 + k)], S2 = old_par[2 * (f0 + k) + 1];
-      y1 += N1 * (S1 * S1 * S1) * HP_INV_8PI * exp_tab(-S1 * r, s_t64);
-      y2 += N2 * (S2 * S2 * S2) * HP_INV_8PI * exp_tab(-S2 * r, s_t64);
-    }
-  }
-  const double dy = y1 - y2;
-  double msd = wr * (4.0 * HP_PI * (r * r)) * dy * dy;
-  block_allsum<1>(&msd, lds, slot);
-  if (threadIdx.x == 0) {
-    int st = 0;
-    double sumN = 0.0;
-#pragma unroll
-    for (int k = 0; k < KM; ++k)
-      if (k < K) sumN += Np[k];
-    if (conv) {
-      // np.isclose(pop, sum, atol=1e-4): |pop - sum| <= 1e-4 + 1e-5 |sum|
-      if (!(fabs(pop - sumN) <= 1e-4 + 1e-5 * fabs(sumN))) st |= HP_ST_POP;
-    } else {
-      st |= HP_ST_NOCONV;
-    }
-#pragma unroll
-    for (int k = 0; k < KM; ++k) {
-      if (k < K) {
-        par[2 * (f0 + k)] = Np[k];
-        par[2 * (f0 + k) + 1] = Sp[k];
-      }
-    }
-    inner_niter[a] = conv ? it + 1 : maxit;
-    status[a] = st;
-    charges[a] = m.pseudo[a] - pop;
-    change_part[a] = msd;
-  }
+  c.pop = pv;
+  mbis_dispatch<KM, 1>(K, c, slot);
 }
 
 // ---------------------------------------------------------------------------
