@@ -25,7 +25,6 @@ using hp::dalloc;
 using hp::fail;
 using hp::g_err;
 using hp::pick_ppl;
-using hp::weights_pt;
 
 template <typename T>
 int dupload(T **p, const T *host, size_t n, hipStream_t s) {
@@ -45,6 +44,9 @@ struct hp_part {
   int64_t npoint = 0;
   int scheme = 0, solver = 0, grid_type = 1, fnkind = HP_FN_SLATER;
   int maxL = 0, maxR = 0, maxK = 0;
+  hp::WeightsShape wshape{1, 64, 1};  // weights pass: points per thread, threads, shells per workgroup
+  int ngroup = 0;                     // weights workgroups
+  int *grp_first = nullptr;           // device: first radial shell of each weights workgroup
   double threshold = 1e-6, inner_threshold = 1e-8, dc = 1e-15, neg_cutoff = -1e-12, pop_cutoff = 1e-4;
   double rc = 0.0, tau = 1.0;
   int maxiter = 500, inner_maxiter = 2000, nlin = 20, check_mono = 0;
@@ -114,6 +116,8 @@ struct hp_part {
     m.pseudo = pseudo;
     m.work = nullptr;
     m.slots = nullptr;
+    m.grp_first = grp_first;
+    m.wgroup = wshape.g;
     return m;
   }
   DevFn fn() const {
@@ -144,7 +148,7 @@ void free_part(hp_part *p) {
                   p->fn_order, p->coef,       p->alpha,   p->qpP,       p->qp_off,      p->par,      p->old_par,
                   p->rhobar,  p->ent_part,    p->promol,  p->charges,   p->change_part, p->out,      p->mom_part,
                   p->mom_out, p->scratch_pts, p->inner_niter, p->status, p->status_all, p->hpar,    p->hch,
-                  p->hent,    p->hchg,    p->corr,      p->evals_done};
+                  p->hent,    p->hchg,    p->corr,      p->evals_done, p->grp_first};
   for (void *b : bufs)
     if (b) (void)hipFree(b);
   if (p->h_out) (void)hipHostFree(p->h_out);
@@ -166,8 +170,8 @@ void free_part(hp_part *p) {
 
 template <typename R, int FK, bool GT2, int PT, int KM>
 void launch_weights_km(hp_part *p, int bs, int tile, int corr_mode) {
-  const size_t lds = weights_lds_per_atom<R>(KM) * tile + 16;
-  hipLaunchKernelGGL((k_weights<R, FK, GT2, PT, KM>), dim3(p->nshell), dim3(bs), lds, p->stream, p->state, p->mol(),
+  const size_t lds = hp::weights_lds(weights_lds_per_atom<R>(KM) * tile, p->wshape);
+  hipLaunchKernelGGL((k_weights<R, FK, GT2, PT, KM>), dim3(p->ngroup), dim3(bs), lds, p->stream, p->state, p->mol(),
                      p->fn(), p->rhobar, p->ent_part, p->promol, p->dc, tile, p->corr, corr_mode);
 }
 
@@ -191,9 +195,9 @@ void launch_weights_pt(hp_part *p, int bs, int corr_mode) {
 
 template <typename R, int FK>
 void launch_weights_fk(hp_part *p, bool gt2, int corr_mode) {
-  // one workgroup per radial shell, each thread one or two points
-  const int pt = weights_pt(p->maxL);
-  const int bs = hp::weights_block(p->maxL);
+  // one workgroup per group of radial shells, each thread one or two points
+  const int pt = p->wshape.pt;
+  const int bs = p->wshape.bs;
   if (gt2) {
     if (pt == 1)
       launch_weights_pt<R, FK, true, 1>(p, bs, HP_CORR_NONE);
@@ -575,6 +579,7 @@ int hp_part_create(int device, const hp_molecule_desc *md, const hp_scheme_desc 
   p->maxL = maxL;
   p->maxR = maxR;
   p->maxK = maxK;
+  p->wshape = hp::weights_shape(maxL);
   p->threshold = sd->threshold;
   p->inner_threshold = sd->inner_threshold;
   p->dc = sd->density_cutoff;
@@ -640,6 +645,9 @@ int hp_part_create(int device, const hp_molecule_desc *md, const hp_scheme_desc 
     return bail(rc);
   std::vector<int> fn_off(sd->fn_offsets, sd->fn_offsets + A + 1);
   if ((rc = dupload(&p->fn_off, fn_off.data(), A + 1, s))) return bail(rc);
+  const std::vector<int> grp_first = hp::weights_groups(atom_shell_off, p->wshape.g);
+  p->ngroup = (int)grp_first.size();
+  if ((rc = dupload(&p->grp_first, grp_first.data(), grp_first.size(), s))) return bail(rc);
   if (!mbis) {
     if ((rc = dupload(&p->fn_norm, sd->fn_norm, nfn, s)) || (rc = dupload(&p->fn_alpha, sd->fn_alpha, nfn, s)) ||
         (rc = dupload(&p->fn_order, sd->fn_order, nfn, s)))

@@ -98,6 +98,11 @@ struct Request {
   int64_t n0 = 0, q0 = 0;
   int stage = HP_RUN_FP64;  // run value the host believes the slot has
   int corr_period = 0;      // FP32 stage: FP64 refresh period of the correction
+  // weights pass: workgroup shape, and the first radial shell (local) of each
+  // workgroup; both depend on the molecule alone, so that its sums are the
+  // same in any mix
+  hp::WeightsShape wshape{1, 64, 1};
+  std::vector<int> grp_first;
 };
 
 // One kernel launch of a tick.
@@ -195,6 +200,8 @@ struct hp_batch {
     m.pseudo = pseudo;
     m.work = d_work + work_off;
     m.slots = slots;
+    m.grp_first = nullptr;  // the work list names each group's first shell
+    m.wgroup = 1;           // per molecule: HpSlot::wgroup
     return m;
   }
   DevFn fn() const {
@@ -336,6 +343,10 @@ int inspect(const hp_batch *b, Request *r) {
   r->maxR = maxR;
   r->maxK = maxK;
   r->scheme01 = mbis ? 0 : 1;
+  r->wshape = hp::weights_shape(maxL);
+  std::vector<int> aso(A + 1, 0);
+  for (int a = 0; a < A; ++a) aso[a + 1] = aso[a] + nsh[a];
+  r->grp_first = hp::weights_groups(aso, r->wshape.g);
   return 0;
 }
 
@@ -462,6 +473,7 @@ int upload(hp_batch *b, Request *r) {
   bool nonneg = true;
   for (int64_t k = 0; k < r->npar; ++k) nonneg = nonneg && r->propars0[k] >= 0.0;
   h.full_sum = (r->sd.full_sum || !nonneg) ? 1 : 0;
+  h.wgroup = r->wshape.g;
   r->stage = r->precision == HP_PREC_MIXED ? HP_RUN_FP32 : HP_RUN_FP64;
   r->corr_period = h.corr_period;
   HP_HIP(hipMemcpyAsync(b->slots + r->slot, &h, sizeof(HpSlot), hipMemcpyHostToDevice, s));
@@ -487,7 +499,7 @@ void plan_group(hp_batch *b, const std::vector<int> &active, std::vector<Launch>
   std::map<std::tuple<int, int, int, int>, std::vector<int>> groups;
   for (int sl : active) {
     const Request *r = b->slot_req[sl];
-    const int pt = hp::weights_pt(r->maxL), bs = hp::weights_block(r->maxL);
+    const int pt = r->wshape.pt, bs = r->wshape.bs;
     const int fk = r->scheme01 == 0 ? HP_FN_SLATER : HP_FN_GAUSS;
     if (r->stage == HP_RUN_FP64 || r->corr_period > 0) groups[{0, fk, bs, pt}].push_back(sl);
     if (r->stage == HP_RUN_FP32) groups[{1, fk, bs, pt}].push_back(sl);
@@ -503,14 +515,15 @@ void plan_group(hp_batch *b, const std::vector<int> &active, std::vector<Launch>
     int kmax = 1, amax = 1;
     for (int sl : g.second) {
       const Request *r = b->slot_req[sl];
-      for (int s = 0; s < r->S; ++s) b->h_work[w++] = make_int2(r->s0 + s, sl);
+      for (int s : r->grp_first) b->h_work[w++] = make_int2(r->s0 + s, sl);
       kmax = std::max(kmax, r->maxK);
       amax = std::max(amax, r->A);
     }
     L.len = w - L.off;
     L.km = weights_km(kmax, L.fk);
     L.tile = L.fp64 ? weights_tile_size<double>(L.km, amax) : weights_tile_size<float>(L.km, amax);
-    L.lds = (L.fp64 ? weights_lds_per_atom<double>(L.km) : weights_lds_per_atom<float>(L.km)) * L.tile + 16;
+    L.lds = hp::weights_lds((L.fp64 ? weights_lds_per_atom<double>(L.km) : weights_lds_per_atom<float>(L.km)) * L.tile,
+                            hp::WeightsShape{L.pt, L.bs, 1});
     plan.push_back(L);
   }
   // MBIS inner solver: one launch; GISA QP: one launch per points-per-lane

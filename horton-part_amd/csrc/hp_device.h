@@ -77,6 +77,8 @@ struct HpSlot {
   int refresh;         // this iteration refreshes the correction (set by k_batch_begin)
   int full_sum;        // weights pass: every atom on every point (no tail skipping)
   int phase;           // 0: next tick starts an outer iteration; 1: its inner solves continue
+  int wgroup;          // weights pass: radial shells per workgroup
+  int pad_;
   double thr;          // threshold of the current stage
   double threshold;    // the request's threshold
   double change, entropy;
@@ -99,6 +101,11 @@ struct DevMol {
   // and natom then counts the entries of an atom work list.
   const int2 *work;
   const HpSlot *slots;
+  // Weights pass: a workgroup covers up to wgroup consecutive radial shells
+  // of one atom (batched: HpSlot::wgroup); grp_first[i] is the first shell of
+  // workgroup i of one molecule (null: one shell per workgroup, in order).
+  const int *grp_first;
+  int wgroup;
 };
 
 struct DevFn {
@@ -606,12 +613,12 @@ __device__ __forceinline__ bool tail_negligible(const DevMol &m, const DevFn &fn
 #define HP_CORR_SLOT -1
 
 template <typename R, int FK, bool GT2, int PT, int KM>
-__global__ __launch_bounds__(512) void k_weights(const int *__restrict__ done, DevMol m, DevFn fn,
+__global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(KM <= 4 ? 6 : 1))) void k_weights(const int *__restrict__ done, DevMol m, DevFn fn,
                                                  double *__restrict__ rhobar, double *__restrict__ ent_part,
                                                  double *__restrict__ promol_out, double dc, int tile,
                                                  double *__restrict__ corr, int corr_mode) {
+  // atom records of a staging round; at the end the points' terms
   extern __shared__ __align__(16) double smem_d[];
-  __shared__ double lds[32];
   __shared__ int s_own;  // list position of the shell's own atom
   __shared__ double s_t64[64];  // 2^(j/64) for the FP64 exponential
   __shared__ float s_rhi[8];    // per wave: largest point distance from the shell's atom
@@ -622,15 +629,17 @@ __global__ __launch_bounds__(512) void k_weights(const int *__restrict__ done, D
   R *s_rec = reinterpret_cast<R *>(smem_d);
   int *s_idx = reinterpret_cast<int *>(s_rec + (size_t)tile * RS);
 
-  int s = blockIdx.x;
+  int s = m.grp_first ? m.grp_first[blockIdx.x] : (int)blockIdx.x;  // first radial shell of the group
   int a_begin = 0, a_end = m.natom;  // atoms of this shell's molecule
   int mode = corr_mode;
   int full = 0;
+  int wgroup = m.wgroup;
   if (m.work) {
     const int2 wk = m.work[blockIdx.x];
     const HpSlot &sl = m.slots[wk.y];
     if (sl.phase) return;  // its inner solves of this outer iteration are still running
     full = sl.full_sum;
+    wgroup = sl.wgroup;
     // a molecule takes part in the launches of its current stage, and an
     // FP32-stage molecule also in the FP64 launch of a refresh iteration
     if (sizeof(R) == sizeof(double)) {
@@ -645,13 +654,20 @@ __global__ __launch_bounds__(512) void k_weights(const int *__restrict__ done, D
     a_end = sl.a0 + sl.na;
   }
   const int a = m.shell_atom[s];
-  const int64_t p0 = m.shell_off[s], p1 = m.shell_off[s + 1];
+  // the group: up to wgroup consecutive radial shells of atom a, their points
+  // contiguous and spread over the workgroup's point slots
+  const int gn = min(max(wgroup, 1), m.atom_shell_off[a + 1] - s);
+  const int64_t p0 = m.shell_off[s], p1 = m.shell_off[s + gn];
   int nb_n = a_end - a_begin;
   const int *nb = nullptr;
   const bool cull = fn.rc > 0.0;
   if (cull) {
+    // each shell's candidates are a prefix of the atom's list sorted by
+    // distance: the group takes the longest; the per-point cutoff test
+    // drops the extra atoms on its inner shells' points
     nb = fn.nb_list + (fn.nb_off ? fn.nb_off[a] : (int64_t)a * m.natom);
-    nb_n = fn.shell_nb[s];
+    nb_n = 0;
+    for (int g = 0; g < gn; ++g) nb_n = max(nb_n, fn.shell_nb[s + g]);
   }
   const R rc = (R)fn.rc;
   const double ax = m.cx[a], ay = m.cy[a], az = m.cz[a];
@@ -770,7 +786,11 @@ __global__ __launch_bounds__(512) void k_weights(const int *__restrict__ done, D
                                                           pz, promol, own);
     }
   }
-  double acc_avg = 0.0, acc_ent = 0.0;
+  // per point: its spherical-average and entropy terms, staged in LDS over
+  // the atom records (no longer read after this barrier)
+  const int64_t npt = p1 - p0;
+  double *s_avg = smem_d, *s_ent = smem_d + npt;
+  __syncthreads();
 #pragma unroll
   for (int j = 0; j < PT; ++j) {
     if (pidx[j] >= p1) continue;
@@ -781,10 +801,11 @@ __global__ __launch_bounds__(512) void k_weights(const int *__restrict__ done, D
     if (n_floor) pm_r += (R)n_floor * hp_floor_add<R>();
     const double pm = (double)pm_r;
     const double rho = m.rho[p];
+    double f_avg = 0.0, f_ent = 0.0;
     if (!GT2) {
       const double w = aim_weight(own[j], pm_r);
       const double f = w * rho;
-      acc_avg += m.wsph[p] * f;
+      f_avg = m.wsph[p] * f;
     } else {
       promol_out[p] = pm;
     }
@@ -792,23 +813,38 @@ __global__ __launch_bounds__(512) void k_weights(const int *__restrict__ done, D
     if (!(pm < dc || rho < dc)) {
       const double lg =
           sizeof(R) == sizeof(double) ? log(rho / pm) : (double)__logf((float)rho / (float)pm_r);
-      acc_ent += m.wmol[p] * rho * lg;
+      f_ent = m.wmol[p] * rho * lg;
+    }
+    const int q = (int)threadIdx.x + j * (int)blockDim.x;  // p - p0
+    s_avg[q] = f_avg;
+    s_ent[q] = f_ent;
+  }
+  __syncthreads();
+  // one wave per radial shell of the group: lanes add its points in a fixed
+  // stride order, then a wave all-reduce
+  for (int g = __builtin_amdgcn_readfirstlane(wv); g < gn; g += nw) {
+    const int q0 = (int)(m.shell_off[s + g] - p0), q1 = (int)(m.shell_off[s + g + 1] - p0);
+    double v[2] = {0.0, 0.0};
+    for (int q = q0 + (int)(threadIdx.x & 63); q < q1; q += 64) {
+      v[0] += s_avg[q];
+      v[1] += s_ent[q];
+    }
+    wave_allsum_n<2>(v);
+    if (lane == 0) {
+      const int sg = s + g;
+      if (GT2) {
+        ent_part[sg] = v[1];
+      } else if (mode == HP_CORR_CALIBRATE) {
+        corr[sg] = rhobar[sg] - v[0];
+      } else {
+        rhobar[sg] = mode == HP_CORR_APPLY ? v[0] + corr[sg] : v[0];
+        ent_part[sg] = v[1];
+      }
     }
   }
-  block_sum2(acc_avg, acc_ent, lds);
-  if (threadIdx.x == 0) {
-    if (fn.evals_done && p1 > p0) {
-      const int slot = (sizeof(R) == sizeof(double) ? HP_EVAL_LANES : 0) + (int)(blockIdx.x & (HP_EVAL_LANES - 1));
-      atomicAdd(fn.evals_done + (size_t)slot * HP_EVAL_STRIDE, (unsigned long long)n_fn * (unsigned long long)(p1 - p0));
-    }
-    if (GT2) {
-      ent_part[s] = acc_ent;
-    } else if (mode == HP_CORR_CALIBRATE) {
-      corr[s] = rhobar[s] - acc_avg;
-    } else {
-      rhobar[s] = mode == HP_CORR_APPLY ? acc_avg + corr[s] : acc_avg;
-      ent_part[s] = acc_ent;
-    }
+  if (threadIdx.x == 0 && fn.evals_done && p1 > p0) {
+    const int slot = (sizeof(R) == sizeof(double) ? HP_EVAL_LANES : 0) + (int)(blockIdx.x & (HP_EVAL_LANES - 1));
+    atomicAdd(fn.evals_done + (size_t)slot * HP_EVAL_STRIDE, (unsigned long long)n_fn * (unsigned long long)(p1 - p0));
   }
 }
 

@@ -5,6 +5,7 @@
 #include <hip/hip_runtime.h>
 
 #include <string>
+#include <vector>
 
 #include "hortonpart.h"
 #include "hp_device.h"
@@ -39,6 +40,49 @@ inline int weights_block(int maxL) {
   const int pt = weights_pt(maxL);
   int bs = ((maxL + pt - 1) / pt + 63) / 64 * 64;
   return bs < 64 ? 64 : (bs > 512 ? 512 : bs);
+}
+
+// Workgroup shape of the weights pass for a molecule whose largest radial
+// shell has maxL points: pt points per thread, bs threads, and up to g
+// consecutive shells of one atom per workgroup (g maxL <= pt bs).  A shell
+// rarely fills whole wavefronts (194 points take 97 of 128 lanes at two per
+// thread); g is picked for the fullest lanes, the smallest g among equals.
+#define HP_WGROUP_MAX 16
+struct WeightsShape {
+  int pt, bs, g;
+};
+inline WeightsShape weights_shape(int maxL) {
+  const int pt = weights_pt(maxL);
+  WeightsShape best{pt, weights_block(maxL), 1};
+  double best_fill = -1.0;
+  for (int g = 1; g <= HP_WGROUP_MAX; ++g) {
+    const int n = g * maxL, threads = (n + pt - 1) / pt;
+    if (threads > 512) break;
+    const int bs = threads <= 64 ? 64 : (threads + 63) / 64 * 64;
+    const double fill = (double)n / (double)(pt * bs);
+    if (fill > best_fill + 1e-9) {
+      best_fill = fill;
+      best = {pt, bs, g};
+    }
+  }
+  return best;
+}
+
+// First radial shell of every weights workgroup: groups of g consecutive
+// shells of one atom (the last group of an atom may be shorter).
+// atom_shell_off: natom+1 shell offsets per atom.
+inline std::vector<int> weights_groups(const std::vector<int> &atom_shell_off, int g) {
+  std::vector<int> first;
+  for (size_t a = 0; a + 1 < atom_shell_off.size(); ++a)
+    for (int s = atom_shell_off[a]; s < atom_shell_off[a + 1]; s += g) first.push_back(s);
+  return first;
+}
+
+// Dynamic LDS of a weights launch: the staged atom records, or at the end two
+// doubles per point of the workgroup, whichever is larger.
+inline size_t weights_lds(size_t records, const WeightsShape &w) {
+  const size_t pts = 2 * sizeof(double) * (size_t)w.pt * (size_t)w.bs;
+  return (records > pts ? records : pts) + 16;
 }
 
 // Radial points per lane of the one-wave-per-atom solvers.
