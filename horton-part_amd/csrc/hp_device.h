@@ -438,13 +438,33 @@ __device__ __forceinline__ double fast_sqrt<double>(double x) {
 template <>
 __device__ __forceinline__ float fast_sqrt<float>(float x) { return __builtin_amdgcn_sqrtf(x); }
 
-// exp of -(alpha * arg): FP64 by the table method; FP32 exp2 with alpha
-// pre-scaled by log2(e)
+// Distances of the weights pass.  FP64: v_rsq_f64, one Goldschmidt step and
+// one Newton-Raphson step (the correctly rounded sequence above stops one
+// step earlier): within about an ulp.  An ulp of r moves exp(-alpha r) by
+// alpha r 2^-53 relative, as the rounding of r itself does.
+template <typename R>
+__device__ __forceinline__ R dist_sqrt(R x);
+template <>
+__device__ __forceinline__ double dist_sqrt<double>(double x) {
+  const double y = __builtin_amdgcn_rsq(x);
+  double s = x * y, h = 0.5 * y;
+  const double e = fma(-h, s, 0.5);
+  s = fma(s, e, s);
+  h = fma(h, e, h);
+  const double d = fma(-s, s, x);
+  s = fma(d, h, s);
+  return x > 0.0 ? s : 0.0;
+}
+template <>
+__device__ __forceinline__ float dist_sqrt<float>(float x) { return __builtin_amdgcn_sqrtf(x); }
+
+// exp of -(alpha * arg) from the exponent as staged: FP64 by the table method
+// with alpha pre-scaled by 64/ln2; FP32 exp2 with alpha pre-scaled by log2(e)
 template <typename R>
 __device__ __forceinline__ R neg_exp_scaled(R al, R arg, const double *t64);
 template <>
 __device__ __forceinline__ double neg_exp_scaled<double>(double al, double arg, const double *t64) {
-  return exp_tab(-al * arg, t64);
+  return hp_exp2_64th_core(-al * arg, t64);
 }
 template <>
 __device__ __forceinline__ float neg_exp_scaled<float>(float al, float arg, const double *) {
@@ -496,7 +516,7 @@ __device__ __forceinline__ int weights_tile(const R *__restrict__ rec, const int
         dz = qz[j] - v[2];
       }
       r2[j] = dx * dx + dy * dy + dz * dz;
-      r[j] = (FK == HP_FN_GAUSS && !CULL) ? (R)0 : fast_sqrt<R>(r2[j]);
+      r[j] = (FK == HP_FN_GAUSS && !CULL) ? (R)0 : dist_sqrt<R>(r2[j]);
       pb[j] = (R)0;
     }
 #pragma unroll
@@ -526,7 +546,7 @@ __device__ __forceinline__ int weights_tile(const R *__restrict__ rec, const int
 
 // One atom record of the weights pass: position (FP64: absolute; FP32:
 // relative to the shell's atom, formed in FP64), function count, prefactors
-// and exponents (FP32: pre-scaled by log2 e).
+// and exponents (FP64: pre-scaled by 64/ln2; FP32: by log2 e).
 template <typename R, int KM>
 __device__ __forceinline__ void stage_record(R *__restrict__ q, int *__restrict__ idx, int b, const DevMol &m,
                                              const DevFn &fn, double ax, double ay, double az) {
@@ -547,7 +567,7 @@ __device__ __forceinline__ void stage_record(R *__restrict__ q, int *__restrict_
   for (int k = 0; k < KM; ++k) {
     const bool on = k < nf;
     q[4 + k] = on ? (R)fn.coef[f0 + k] : (R)0;
-    q[4 + KM + k] = on ? (R)(sizeof(R) == sizeof(double) ? fn.alpha[f0 + k] : fn.alpha[f0 + k] * log2e) : (R)0;
+    q[4 + KM + k] = on ? (R)(fn.alpha[f0 + k] * (sizeof(R) == sizeof(double) ? HP_64_OVER_LN2 : log2e)) : (R)0;
   }
 }
 
@@ -2004,7 +2024,8 @@ static __global__ __launch_bounds__(256) void k_copy16(const double2 *__restrict
 }
 
 #define HP_EXP_RATE_CHAINS 8
-// iters x 8 independent exponentials per thread, arguments in [-30, 0]
+// iters x 8 independent exponentials per thread, arguments in [-30, 0]; the
+// FP64 form is the weights pass's (argument in units of ln2/64)
 template <typename R>
 __global__ __launch_bounds__(256) void k_exp_rate(int iters, double *__restrict__ out) {
   __shared__ double s_t64[64];

@@ -48,3 +48,27 @@ HP_HD inline double hp_exp_tab_core(double x, const double *t64) {
   p *= r;
   return ldexp(fma(t, p, t), ki >> 6);  // ki >> 6 floors: m = floor(k / 64)
 }
+
+// 2^(y/64) for y <= 0: the same table method with the argument already in
+// units of ln2/64, as the weights pass forms it (y = -(alpha 64/ln2) r, the
+// exponent scaled once per function).  With k = rint(y) the reduced argument
+// f = y - k is exact, |f| <= 1/2, and e^(f ln2/64) - 1 is the degree-5 Taylor
+// polynomial in f with coefficients (ln2/64)^n / n! (truncation below 4e-17
+// relative, as above).  Arguments below -1000 64/ln2 give 0.
+HP_HD inline double hp_exp2_64th_core(double y, const double *t64) {
+  y = fmax(y, -92332.48261689366);  // -1000 * 64/ln2: keeps k in int range
+  const double k = rint(y);
+  const double f = y - k;
+  const int ki = (int)k;
+  const double t = t64[ki & 63];
+  double p = fma(f, 0x1.5d87fe78a6731p-40, 0x1.3b2ab6fba4e77p-31);
+  p = fma(p, f, 0x1.c6b08d704a0c0p-23);
+  p = fma(p, f, 0x1.ebfbdff82c58fp-15);
+  p = fma(p, f, 0x1.62e42fefa39efp-7);
+  p *= f;
+  return ldexp(fma(t, p, t), ki >> 6);  // ki >> 6 floors: m = floor(k / 64)
+}
+
+// 64 / ln2, rounded to the nearest double: exponents of the weights pass are
+// stored multiplied by it
+#define HP_64_OVER_LN2 0x1.71547652b82fep+6
