@@ -3,7 +3,7 @@
 // Kernels
 //   k_iter_begin      pro-atom function table (coefficient, exponent) from the
 //                     current parameters (mbis.py:284-289, basis.py:161-171)
-//   k_weights<R,F,G>  fused pass over one radial shell per workgroup:
+//   k_weights<R,F,G>  fused pass over a group of radial shells per workgroup:
 //                     pro-atoms of every (or every neighbouring) atom,
 //                     promolecule with the +1e-100 floor, own-atom weight
 //                     clipped to [0,1], spherical average of w*rho and the
@@ -573,11 +573,11 @@ __device__ __forceinline__ void stage_record(R *__restrict__ q, int *__restrict_
 
 // Bounded tails.  With non-negative coefficients (MBIS populations, GISA
 // simplex coefficients) every pro-atom is a decreasing function of the
-// distance.  For the points of one radial shell of atom a, at distances up to
-// rhi from R_a, atom b then contributes at most
+// distance.  For the points of one workgroup (radial shells of atom a), at
+// distances up to rhi from R_a, atom b then contributes at most
 //   U_b = sum_k c_bk f_k(x_b),   x_b = max(0, |R_b - R_a| - rhi),
 // and the promolecule is at least L = sum_k c_ak f_k(rhi) + (floors).  An
-// atom with U_b < eps L moves the promolecule of every point of the shell by
+// atom with U_b < eps L moves the promolecule of every point of the group by
 // less than eps relative, so the pass may skip it (its 1e-100 floor is still
 // added).  The test runs in the log2 domain in FP32 (hardware log2), with
 //   log2 U_b <= log2 nf + max_k (log2 c_bk - alpha_bk log2(e) g(x_b)),
