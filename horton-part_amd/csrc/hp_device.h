@@ -483,8 +483,10 @@ __device__ __forceinline__ R fn_arg(R r, R r2, const DevFn &fn, int b, int k) {
 // The atoms of one staged tile on the thread's first NP point slots.  The
 // distances come first; then the functions, two at a time, each uniform
 // branch holding 2 x NP independent exponentials.  Per point the terms are
-// added in function order.  Returns the number of functions of the tile's
-// atoms (uniform).
+// added in function order.  With the cut-off, an atom beyond it from every
+// point of the wavefront is not evaluated (it adds its floors only, as the
+// evaluated and zeroed terms would).  Returns the number of functions
+// evaluated (uniform in the wavefront).
 template <typename R, int FK, int PT, int NP, int KM, bool CULL>
 __device__ __forceinline__ int weights_tile(const R *__restrict__ rec, const int *__restrict__ s_idx,
                                              const double *__restrict__ t64, int tn, int own_i,
@@ -499,7 +501,6 @@ __device__ __forceinline__ int weights_tile(const R *__restrict__ rec, const int
 #pragma unroll
     for (int u = 0; u < RS; ++u) v[u] = q[u];
     const int nf = __builtin_amdgcn_readfirstlane((int)v[3]);
-    nfn += nf;
     const bool is_own = i == own_i;
     const int b = FK == HP_FN_GENERAL ? s_idx[i] : 0;
     R r[NP], r2[NP], pb[NP];
@@ -519,18 +520,28 @@ __device__ __forceinline__ int weights_tile(const R *__restrict__ rec, const int
       r[j] = (FK == HP_FN_GAUSS && !CULL) ? (R)0 : dist_sqrt<R>(r2[j]);
       pb[j] = (R)0;
     }
+    bool eval = true;
+    if (CULL) {
+      bool in = false;
 #pragma unroll
-    for (int k = 0; k < KM; k += 2) {
-      if (k + 1 < nf) {
+      for (int j = 0; j < NP; ++j) in = in || r[j] <= rc;
+      eval = __any(in);
+    }
+    if (eval) {
+      nfn += nf;
 #pragma unroll
-        for (int j = 0; j < NP; ++j) {
-          pb[j] += v[4 + k] * neg_exp_scaled<R>(v[4 + KM + k], fn_arg<R, FK>(r[j], r2[j], fn, b, k), t64);
-          pb[j] += v[5 + k] * neg_exp_scaled<R>(v[5 + KM + k], fn_arg<R, FK>(r[j], r2[j], fn, b, k + 1), t64);
+      for (int k = 0; k < KM; k += 2) {
+        if (k + 1 < nf) {
+#pragma unroll
+          for (int j = 0; j < NP; ++j) {
+            pb[j] += v[4 + k] * neg_exp_scaled<R>(v[4 + KM + k], fn_arg<R, FK>(r[j], r2[j], fn, b, k), t64);
+            pb[j] += v[5 + k] * neg_exp_scaled<R>(v[5 + KM + k], fn_arg<R, FK>(r[j], r2[j], fn, b, k + 1), t64);
+          }
+        } else if (k < nf) {
+#pragma unroll
+          for (int j = 0; j < NP; ++j)
+            pb[j] += v[4 + k] * neg_exp_scaled<R>(v[4 + KM + k], fn_arg<R, FK>(r[j], r2[j], fn, b, k), t64);
         }
-      } else if (k < nf) {
-#pragma unroll
-        for (int j = 0; j < NP; ++j)
-          pb[j] += v[4 + k] * neg_exp_scaled<R>(v[4 + KM + k], fn_arg<R, FK>(r[j], r2[j], fn, b, k), t64);
       }
     }
 #pragma unroll
@@ -633,7 +644,7 @@ __device__ __forceinline__ bool tail_negligible(const DevMol &m, const DevFn &fn
 #define HP_CORR_SLOT -1
 
 template <typename R, int FK, bool GT2, int PT, int KM>
-__global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(KM <= 4 ? 6 : 1))) void k_weights(const int *__restrict__ done, DevMol m, DevFn fn,
+__global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(KM <= 2 ? 6 : (KM <= 4 ? 5 : 1)))) void k_weights(const int *__restrict__ done, DevMol m, DevFn fn,
                                                  double *__restrict__ rhobar, double *__restrict__ ent_part,
                                                  double *__restrict__ promol_out, double dc, int tile,
                                                  double *__restrict__ corr, int corr_mode) {
@@ -734,7 +745,7 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(KM <= 4 ? 6
   const bool tail = tail_req;
   float tail_rhi = 0.0f, tail_log2_thr = 0.0f;
   int n_cut = 0;  // candidates skipped by the tail bound
-  int n_fn = 0;  // functions of the staged atoms (each evaluated on every point of the shell)
+  int n_fn = 0;  // functions evaluated on every point of this wavefront (uniform in it)
   // the tail path stages one candidate per thread and round
   const int step = tail_req ? min(tile, (int)blockDim.x) : tile;
   for (int t0 = 0, round = 0; t0 < nb_n; t0 += step, ++round) {
@@ -862,9 +873,15 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(KM <= 4 ? 6
       }
     }
   }
-  if (threadIdx.x == 0 && fn.evals_done && p1 > p0) {
-    const int slot = (sizeof(R) == sizeof(double) ? HP_EVAL_LANES : 0) + (int)(blockIdx.x & (HP_EVAL_LANES - 1));
-    atomicAdd(fn.evals_done + (size_t)slot * HP_EVAL_STRIDE, (unsigned long long)n_fn * (unsigned long long)(p1 - p0));
+  if (fn.evals_done) {
+    // per wavefront: its evaluated functions times its points
+    unsigned long long npts_w = 0;
+#pragma unroll
+    for (int j = 0; j < PT; ++j) npts_w += (unsigned long long)__popcll(__ballot(pidx[j] < p1));
+    if (lane == 0 && npts_w) {
+      const int slot = (sizeof(R) == sizeof(double) ? HP_EVAL_LANES : 0) + (int)((blockIdx.x * 8u + wv) & (HP_EVAL_LANES - 1));
+      atomicAdd(fn.evals_done + (size_t)slot * HP_EVAL_STRIDE, (unsigned long long)n_fn * npts_w);
+    }
   }
 }
 
