@@ -28,6 +28,12 @@ import time
 os.environ.setdefault("OMP_NUM_THREADS", "1")
 os.environ.setdefault("OPENBLAS_NUM_THREADS", "1")
 os.environ.setdefault("MKL_NUM_THREADS", "1")
+# The batch engine overlaps one stream per molecule group, and HIP gives each
+# stream a hardware queue of its own only while the process has queues to
+# spare (GPU_MAX_HW_QUEUES, 4 when unset): ask for 8 before HIP starts, which
+# lets the engine run three groups (hortonpart.h, hp_batch_config.groups).
+if int(os.environ.get("GPU_MAX_HW_QUEUES", "4")) < 8:
+    os.environ["GPU_MAX_HW_QUEUES"] = "8"
 
 ROOT = os.path.dirname(os.path.abspath(__file__))
 for _p in (os.path.join(ROOT, "horton-part_amd"), ROOT):
@@ -392,7 +398,8 @@ def run_serving(args):
             "points": int(sum(it["mol"]["npoint"] for it in timed)),
             "ticks_per_step": args.ticks,
             "graphs": args.graphs,
-            "groups": args.groups or 2,
+            "groups": args.groups or (3 if int(os.environ["GPU_MAX_HW_QUEUES"]) >= 8 else 2),
+            "hw_queues": int(os.environ["GPU_MAX_HW_QUEUES"]),
             "inner_budget": args.inner_budget,
         },
         "p50_ms": float(np.percentile(lat, 50)) * 1e3,
@@ -465,7 +472,7 @@ def main():
     ap.add_argument("--workload", default="serving",
                     choices=["serving", "water", "chain20", "chain20-gisa", "cluster200", "protein2000"])
     ap.add_argument("--ticks", type=int, default=4, help="serving: outer iterations per host step")
-    ap.add_argument("--groups", type=int, default=0, help="serving: molecule groups interleaved per tick (1-4; 0: 2)")
+    ap.add_argument("--groups", type=int, default=0, help="serving: molecule groups interleaved per tick (1-4; 0: 3 with >= 8 hardware queues, else 2)")
     ap.add_argument("--inner-budget", type=int, default=0,
                     help="serving: MBIS inner iterations per atom and launch (0: no limit)")
     ap.add_argument("--graphs", action="store_true", help="serving: replay the ticks of a step as a hipGraph (times whole steps only)")

@@ -21,6 +21,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 #include <deque>
 #include <map>
@@ -80,7 +81,17 @@ class RangeAlloc {
 enum SlotState { SLOT_FREE = 0, SLOT_UPLOADING, SLOT_ACTIVE };
 
 constexpr int kMaxGroups = 4;  // groups of molecules whose ticks interleave (one stream each)
-constexpr int kDefaultGroups = 2;
+
+// Default group count.  Each group's stream needs a hardware queue of its
+// own for the groups to overlap; streams beyond the process's queues share
+// one and run in turn.  HIP gives a process GPU_MAX_HW_QUEUES queues (4 when
+// unset), and the engine also uses an upload stream, so three groups need
+// at least 8 queues; two groups otherwise.
+int default_groups() {
+  const char *e = std::getenv("GPU_MAX_HW_QUEUES");
+  const int q = e ? std::atoi(e) : 4;
+  return q >= 8 ? 3 : 2;
+}
 
 struct Request {
   int64_t ticket = 0;
@@ -823,7 +834,7 @@ int hp_batch_create(int device, const hp_batch_config *cfg, hp_batch **out) {
   hp_batch *b = new hp_batch();
   b->device = device;
   b->cfg = *cfg;
-  if (b->cfg.groups == 0) b->cfg.groups = kDefaultGroups;
+  if (b->cfg.groups == 0) b->cfg.groups = default_groups();
   b->cfg.max_molecule_points = std::min(cfg->max_molecule_points, cfg->point_capacity);
   auto bail = [&](int rc) {
     free_batch(b);

@@ -255,11 +255,14 @@ typedef struct {
   double density_cutoff;
   double negative_cutoff;
   double population_cutoff;
-  int32_t groups;            /* molecule groups of equal weights cost, 1..4
-                                (0 selects 2), one stream each: the groups'
-                                weights launches take turns, so that each
-                                group's per-atom solves run beside the other
-                                groups' weights passes */
+  int32_t groups;            /* molecule groups of equal weights cost, 1..4,
+                                one stream each: the groups' weights launches
+                                take turns, so that each group's per-atom
+                                solves run beside the other groups' weights
+                                passes.  0 selects 3 when the process has at
+                                least 8 hardware queues (GPU_MAX_HW_QUEUES,
+                                read when HIP starts), else 2: streams beyond
+                                the queues share one and run in turn */
   int32_t inner_budget;      /* MBIS inner iterations per atom and launch
                                 (0: no limit).  An atom whose inner solve needs
                                 more keeps its state on the device and goes on
