@@ -1000,6 +1000,14 @@ __device__ __forceinline__ void mbis_solve(const MbisCtx &c, int slot) {
   int it = 0;
   const int it_end = (c.budget > 0 && it0 + c.budget < c.maxit) ? it0 + c.budget : c.maxit;
   const double dc = 1e-15;  // opt_mbis_propars default density_cutoff (mbis.py:82)
+  // The stop test sqrt(S) < thr (S = the summed squared change) decided
+  // from S where the bounds thr^2 (1 -/+ 2^-50) settle it (they cover the
+  // roundings of thr^2 and of the square root), so the square root runs only
+  // in that narrow band: the same decision, off the loop's critical path.
+  const double thr2 = c.thr * c.thr;
+  // (a threshold <= 0 never stops: no bound passes)
+  const double thr2_lo = c.thr > 0.0 ? thr2 * (1.0 - 0x1p-50) : -1.0;
+  const double thr2_hi = c.thr > 0.0 ? thr2 * (1.0 + 0x1p-50) : -1.0;
   for (it = it0; it < it_end; ++it) {
     double t[KC];
     double pro = 0.0;
@@ -1029,11 +1037,14 @@ __device__ __forceinline__ void mbis_solve(const MbisCtx &c, int slot) {
       Np[k] = red[2 * k];
       Sp[k] = 3.0 * red[2 * k] / red[2 * k + 1];
     }
-    // a squared change is 0 or far above 2^-767: the square root needs no rescaling
-    const double change = have_old ? fast_sqrt<double>(red[2 * KC]) : 1e100;
-    if (change < c.thr) {
-      conv = true;
-      break;
+    if (have_old) {
+      const double S = red[2 * KC];
+      // a squared change is 0 or far above 2^-767: the square root needs no rescaling
+      const bool stop = S < thr2_lo || (S <= thr2_hi && fast_sqrt<double>(S) < c.thr);
+      if (stop) {
+        conv = true;
+        break;
+      }
     }
     oldpro = pro;
     have_old = true;
