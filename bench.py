@@ -471,7 +471,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=None)
     ap.add_argument("--workload", default="serving",
                     choices=["serving", "water", "chain20", "chain20-gisa", "cluster200", "protein2000"])
-    ap.add_argument("--ticks", type=int, default=4, help="serving: outer iterations per host step")
+    ap.add_argument("--ticks", type=int, default=8, help="serving: outer iterations per host step")
     ap.add_argument("--groups", type=int, default=0, help="serving: molecule groups interleaved per tick (1-4; 0: 3 with >= 8 hardware queues, else 2)")
     ap.add_argument("--inner-budget", type=int, default=0,
                     help="serving: MBIS inner iterations per atom and launch (0: no limit)")

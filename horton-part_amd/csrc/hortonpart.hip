@@ -225,18 +225,28 @@ template <int PPL>
 int launch_inner_ppl(hp_part *p) {
   const dim3 grid((p->natom + 3) / 4), block(256);
   if (p->scheme == HP_SCHEME_MBIS) {
-    // one workgroup per atom, one radial point per lane
-    const dim3 g1(p->natom), b1(((p->maxR + 63) / 64) * 64);
-#define HP_MBIS(KMV)                                                                                                  \
-  hipLaunchKernelGGL((k_mbis_inner<KMV>), g1, b1, 0, p->stream, p->state, p->mol(), p->rhobar, p->par, p->old_par,    \
-                     p->fn_off, p->inner_threshold, p->inner_maxiter, p->inner_niter, p->status, p->charges,          \
-                     p->change_part, 0, nullptr, nullptr)
+    // one wavefront per atom with two radial points per lane (up to 128
+    // points), else one workgroup per atom with one point per lane
+    const int ppl = hp::mbis_ppl(p->maxR);
+    const dim3 g1(p->natom), b1(ppl > 1 ? 64 : ((p->maxR + 63) / 64) * 64);
+#define HP_MBIS(KMV, PPLV)                                                                                            \
+  hipLaunchKernelGGL((k_mbis_inner<KMV, PPLV>), g1, b1, 0, p->stream, p->state, p->mol(), p->rhobar, p->par,          \
+                     p->old_par, p->fn_off, p->inner_threshold, p->inner_maxiter, p->inner_niter, p->status,          \
+                     p->charges, p->change_part, 0, nullptr, nullptr)
+#define HP_MBIS_K(KMV) \
+  do {                    \
+    if (ppl == 2)         \
+      HP_MBIS(KMV, 2);    \
+    else                  \
+      HP_MBIS(KMV, 1);    \
+  } while (0)
     if (p->maxK <= 2)
-      HP_MBIS(2);
+      HP_MBIS_K(2);
     else if (p->maxK <= 3)
-      HP_MBIS(3);
+      HP_MBIS_K(3);
     else
-      HP_MBIS(7);
+      HP_MBIS_K(7);
+#undef HP_MBIS_K
 #undef HP_MBIS
   } else if (p->solver == HP_SOLVER_QP) {
     hipLaunchKernelGGL((k_gisa_qp<PPL>), grid, block, 0, p->stream, p->state, p->mol(), p->rhobar, p->par, p->old_par, p->fn_off,

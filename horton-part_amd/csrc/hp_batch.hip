@@ -537,22 +537,24 @@ void plan_group(hp_batch *b, const std::vector<int> &active, std::vector<Launch>
                             hp::WeightsShape{L.pt, L.bs, 1});
     plan.push_back(L);
   }
-  // MBIS inner solver: one launch; GISA QP: one launch per points-per-lane
-  {
+  // MBIS inner solver: one launch per layout (a molecule's own, as alone);
+  // GISA QP: one launch per points-per-lane
+  for (int ppl : {2, 1}) {
     Launch L{};
     L.kind = 1;
     L.off = w;
+    L.ppl = ppl;
     int kmax = 1, rmax = 1;
     for (int sl : active) {
       const Request *r = b->slot_req[sl];
-      if (r->scheme01 != 0) continue;
+      if (r->scheme01 != 0 || hp::mbis_ppl(r->maxR) != ppl) continue;
       for (int a = 0; a < r->A; ++a) b->h_work[w++] = make_int2(r->a0 + a, sl);
       kmax = std::max(kmax, r->maxK);
       rmax = std::max(rmax, r->maxR);
     }
     L.len = w - L.off;
     L.km = kmax <= 2 ? 2 : (kmax <= 3 ? 3 : 7);
-    L.block = ((rmax + 63) / 64) * 64;
+    L.block = ppl > 1 ? 64 : ((rmax + 63) / 64) * 64;
     if (L.len) plan.push_back(L);
   }
   for (int ppl : {2, 4, 8}) {
@@ -684,16 +686,24 @@ int enqueue_tick(hp_batch *b, int t) {
           launch_w_r<float>(b, L, s);
       } else if (L.kind == 1) {
         const DevMol m = b->mol(L.off, (int)L.len);
-#define HP_BMBIS(KMV)                                                                                          \
-  hipLaunchKernelGGL((k_mbis_inner<KMV>), dim3((unsigned)L.len), dim3(L.block), 0, s, nullptr, m, b->rhobar,    \
-                     b->par_m, b->old_m, b->fn_off, b->cfg.inner_threshold, b->cfg.inner_maxiter, b->inner_niter, \
+#define HP_BMBIS(KMV, PPLV)                                                                                         \
+  hipLaunchKernelGGL((k_mbis_inner<KMV, PPLV>), dim3((unsigned)L.len), dim3(L.block), 0, s, nullptr, m, b->rhobar,  \
+                     b->par_m, b->old_m, b->fn_off, b->cfg.inner_threshold, b->cfg.inner_maxiter, b->inner_niter,     \
                      b->status, b->charges, b->change_part, b->cfg.inner_budget, b->ist_it, b->ist_oldpro)
+#define HP_BMBIS_K(KMV) \
+  do {                     \
+    if (L.ppl == 2)        \
+      HP_BMBIS(KMV, 2);    \
+    else                   \
+      HP_BMBIS(KMV, 1);    \
+  } while (0)
         if (L.km == 2)
-          HP_BMBIS(2);
+          HP_BMBIS_K(2);
         else if (L.km == 3)
-          HP_BMBIS(3);
+          HP_BMBIS_K(3);
         else
-          HP_BMBIS(7);
+          HP_BMBIS_K(7);
+#undef HP_BMBIS_K
 #undef HP_BMBIS
       } else {
         const DevMol m = b->mol(L.off, (int)L.len);

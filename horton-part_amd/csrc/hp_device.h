@@ -975,17 +975,29 @@ struct MbisCtx {
   double *lds;
   int *inner_niter, *status, *ist_it;
   double *charges, *change_part, *ist_oldpro;
-  double thr, r, wr, w, rho, pop;
+  double thr, pop;
+  // this lane's radial points i + 64 j, j < PPL (the solver's template)
+  double r[2], wr[2], w[2], rho[2];
+  bool valid[2];
   int a, f0, s0, i, maxit, budget;
-  bool valid, resume;
+  bool resume;
 };
 
+// All-reduce of the solver: across the workgroup (one radial point per lane,
+// several wavefronts) or inside the one wavefront (two points per lane).
+template <int NV, int PPL>
+__device__ __forceinline__ void solver_allsum(double *v, double *lds, int parity) {
+  if constexpr (PPL == 1)
+    block_allsum<NV>(v, lds, parity);
+  else
+    wave_allsum_n<NV>(v);
+}
+
 // The solve for an atom of exactly KC shells (no per-shell guards in the
-// loop; 2 KC + 1 sums per iteration).
-template <int KC>
+// loop; 2 KC + 1 sums per iteration), PPL radial points per lane.
+template <int KC, int PPL>
 __device__ __forceinline__ void mbis_solve(const MbisCtx &c, int slot) {
   constexpr int NV = 2 * KC + 1;
-  const double r = c.r, w = c.w, rho = c.rho;
   double Np[KC], Sp[KC];
 #pragma unroll
   for (int k = 0; k < KC; ++k) {
@@ -995,7 +1007,9 @@ __device__ __forceinline__ void mbis_solve(const MbisCtx &c, int slot) {
   // a solve resumed from an earlier launch continues from its saved state:
   // the iterate (in par), the previous pro-atom and the iterations done
   const int it0 = c.resume ? c.ist_it[c.a] : 0;
-  double oldpro = (c.resume && c.valid) ? c.ist_oldpro[c.s0 + c.i] : 0.0;
+  double oldpro[PPL];
+#pragma unroll
+  for (int j = 0; j < PPL; ++j) oldpro[j] = (c.resume && c.valid[j]) ? c.ist_oldpro[c.s0 + c.i + 64 * j] : 0.0;
   bool have_old = it0 > 0, conv = false;
   int it = 0;
   const int it_end = (c.budget > 0 && it0 + c.budget < c.maxit) ? it0 + c.budget : c.maxit;
@@ -1009,28 +1023,33 @@ __device__ __forceinline__ void mbis_solve(const MbisCtx &c, int slot) {
   const double thr2_lo = c.thr > 0.0 ? thr2 * (1.0 - 0x1p-50) : -1.0;
   const double thr2_hi = c.thr > 0.0 ? thr2 * (1.0 + 0x1p-50) : -1.0;
   for (it = it0; it < it_end; ++it) {
-    double t[KC];
-    double pro = 0.0;
-#pragma unroll
-    for (int k = 0; k < KC; ++k) {
-      // N S^3 / (8 pi), once per shell; the constant division as a product
-      // with its rounded reciprocal (within one rounding of the quotient)
-      const double coef = Np[k] * (Sp[k] * Sp[k] * Sp[k]) * HP_INV_8PI;
-      t[k] = coef * exp_tab(-Sp[k] * r, c.s_t64);
-      pro += t[k];
-    }
-    const bool sick = (rho < dc) || (pro < dc);
-    const double ratio = sick ? 0.0 : rho / pro;
     double red[NV];
+    double pro[PPL];
 #pragma unroll
-    for (int k = 0; k < KC; ++k) {
-      const double wt = w * (t[k] * ratio);
-      red[2 * k] = wt;
-      red[2 * k + 1] = wt * r;
+    for (int j = 0; j < PPL; ++j) {
+      const double r = c.r[j], w = c.w[j], rho = c.rho[j];
+      double t[KC];
+      pro[j] = 0.0;
+#pragma unroll
+      for (int k = 0; k < KC; ++k) {
+        // N S^3 / (8 pi), once per shell; the constant division as a product
+        // with its rounded reciprocal (within one rounding of the quotient)
+        const double coef = Np[k] * (Sp[k] * Sp[k] * Sp[k]) * HP_INV_8PI;
+        t[k] = coef * exp_tab(-Sp[k] * r, c.s_t64);
+        pro[j] += t[k];
+      }
+      const bool sick = (rho < dc) || (pro[j] < dc);
+      const double ratio = sick ? 0.0 : rho / pro[j];
+#pragma unroll
+      for (int k = 0; k < KC; ++k) {
+        const double wt = w * (t[k] * ratio);
+        red[2 * k] = j ? red[2 * k] + wt : wt;
+        red[2 * k + 1] = j ? red[2 * k + 1] + wt * r : wt * r;
+      }
+      const double e = oldpro[j] - pro[j];
+      red[2 * KC] = j ? red[2 * KC] + w * e * e : w * e * e;
     }
-    const double e = oldpro - pro;
-    red[2 * KC] = w * e * e;
-    block_allsum<NV>(red, c.lds, slot);
+    solver_allsum<NV, PPL>(red, c.lds, slot);
     slot ^= 1;
 #pragma unroll
     for (int k = 0; k < KC; ++k) {
@@ -1046,12 +1065,15 @@ __device__ __forceinline__ void mbis_solve(const MbisCtx &c, int slot) {
         break;
       }
     }
-    oldpro = pro;
+#pragma unroll
+    for (int j = 0; j < PPL; ++j) oldpro[j] = pro[j];
     have_old = true;
   }
   if (!conv && it < c.maxit) {
     // iteration budget of this launch spent: keep the state for the next one
-    if (c.valid) c.ist_oldpro[c.s0 + c.i] = oldpro;
+#pragma unroll
+    for (int j = 0; j < PPL; ++j)
+      if (c.valid[j]) c.ist_oldpro[c.s0 + c.i + 64 * j] = oldpro[j];
     if (threadIdx.x == 0) {
 #pragma unroll
       for (int k = 0; k < KC; ++k) {
@@ -1064,17 +1086,22 @@ __device__ __forceinline__ void mbis_solve(const MbisCtx &c, int slot) {
   }
   if (c.ist_it && threadIdx.x == 0) c.ist_it[c.a] = -1;
   // change of the outer loop: new vs old pro-atom on the radial grid
-  double y1 = 0.0, y2 = 0.0;
+  double msd = 0.0;
 #pragma unroll
-  for (int k = 0; k < KC; ++k) {
-    const double N1 = Np[k], S1 = Sp[k];
-    const double N2 = c.old_par[2 * (c.f0 + k)], S2 = c.old_par[2 * (c.f0 + k) + 1];
-    y1 += N1 * (S1 * S1 * S1) * HP_INV_8PI * exp_tab(-S1 * r, c.s_t64);
-    y2 += N2 * (S2 * S2 * S2) * HP_INV_8PI * exp_tab(-S2 * r, c.s_t64);
+  for (int j = 0; j < PPL; ++j) {
+    const double r = c.r[j];
+    double y1 = 0.0, y2 = 0.0;
+#pragma unroll
+    for (int k = 0; k < KC; ++k) {
+      const double N1 = Np[k], S1 = Sp[k];
+      const double N2 = c.old_par[2 * (c.f0 + k)], S2 = c.old_par[2 * (c.f0 + k) + 1];
+      y1 += N1 * (S1 * S1 * S1) * HP_INV_8PI * exp_tab(-S1 * r, c.s_t64);
+      y2 += N2 * (S2 * S2 * S2) * HP_INV_8PI * exp_tab(-S2 * r, c.s_t64);
+    }
+    const double dy = y1 - y2;
+    msd += c.wr[j] * (4.0 * HP_PI * (r * r)) * dy * dy;
   }
-  const double dy = y1 - y2;
-  double msd = c.wr * (4.0 * HP_PI * (r * r)) * dy * dy;
-  block_allsum<1>(&msd, c.lds, slot);
+  solver_allsum<1, PPL>(&msd, c.lds, slot);
   if (threadIdx.x == 0) {
     int st = 0;
     double sumN = 0.0;
@@ -1099,18 +1126,21 @@ __device__ __forceinline__ void mbis_solve(const MbisCtx &c, int slot) {
 }
 
 // The solve for the atom's shell count K in [1, KM] (uniform per block).
-template <int KM, int KC>
+template <int KM, int KC, int PPL>
 __device__ __forceinline__ void mbis_dispatch(int K, const MbisCtx &c, int slot) {
   if constexpr (KC <= KM) {
     if (K == KC) {
-      mbis_solve<KC>(c, slot);
+      mbis_solve<KC, PPL>(c, slot);
       return;
     }
-    mbis_dispatch<KM, KC + 1>(K, c, slot);
+    mbis_dispatch<KM, KC + 1, PPL>(K, c, slot);
   }
 }
 
-template <int KM>
+// PPL = 1: one workgroup per atom, one radial point per lane (ceil(R/64)
+// wavefronts).  PPL = 2: one wavefront per atom, radial points i and i + 64
+// on lane i (R <= 128): its sums need no LDS exchange and no barrier.
+template <int KM, int PPL>
 __global__ __launch_bounds__(512) void k_mbis_inner(const int *__restrict__ done, DevMol m,
                                                     const double *__restrict__ rhobar, double *__restrict__ par,
                                                     const double *__restrict__ old_par, const int *__restrict__ fn_off,
@@ -1120,7 +1150,7 @@ __global__ __launch_bounds__(512) void k_mbis_inner(const int *__restrict__ done
                                                     int *__restrict__ ist_it, double *__restrict__ ist_oldpro) {
   constexpr int NV = 2 * KM + 1;
   __shared__ double lds[2 * 16 * NV];
-  __shared__ double s_t64[64];  // 2^(j/64) for the exponential (published by the first reduction)
+  __shared__ double s_t64[64];  // 2^(j/64) for the exponential
   if (done && *done) return;
   // a dependent chain of short steps: its waves issue first on their SIMDs,
   // beside the throughput-bound weights pass of other molecules
@@ -1138,10 +1168,10 @@ __global__ __launch_bounds__(512) void k_mbis_inner(const int *__restrict__ done
       resume = true;
     }
   }
+  __syncthreads();  // the table (uniform exits above: the whole block returns)
   const int s0 = m.atom_shell_off[a], R = m.atom_shell_off[a + 1] - s0;
   const int f0 = fn_off[a], K = __builtin_amdgcn_readfirstlane(fn_off[a + 1] - f0);
   const int i = threadIdx.x;
-  const bool valid = i < R;
   MbisCtx c;
   c.m = &m;
   c.par = par;
@@ -1155,24 +1185,30 @@ __global__ __launch_bounds__(512) void k_mbis_inner(const int *__restrict__ done
   c.change_part = change_part;
   c.ist_oldpro = ist_oldpro;
   c.thr = thr;
-  c.r = valid ? m.shell_r[s0 + i] : 0.0;
-  c.wr = valid ? m.shell_w[s0 + i] : 0.0;
-  c.w = 4.0 * HP_PI * (c.r * c.r) * c.wr;
-  c.rho = valid ? rhobar[s0 + i] : 0.0;
+  double pv = 0.0;
+#pragma unroll
+  for (int j = 0; j < 2; ++j) {
+    const int q = i + 64 * j;
+    const bool valid = j < PPL && q < R;
+    c.valid[j] = valid;
+    c.r[j] = valid ? m.shell_r[s0 + q] : 0.0;
+    c.wr[j] = valid ? m.shell_w[s0 + q] : 0.0;
+    c.w[j] = 4.0 * HP_PI * (c.r[j] * c.r[j]) * c.wr[j];
+    c.rho[j] = valid ? rhobar[s0 + q] : 0.0;
+    if (j < PPL) pv = j ? pv + c.w[j] * c.rho[j] : c.w[j] * c.rho[j];
+  }
   c.a = a;
   c.f0 = f0;
   c.s0 = s0;
   c.i = i;
   c.maxit = maxit;
   c.budget = budget;
-  c.valid = valid;
   c.resume = resume;
   int slot = 0;  // LDS slot of the next block reduction (alternates)
-  double pv = c.w * c.rho;
-  block_allsum<1>(&pv, lds, slot);
+  solver_allsum<1, PPL>(&pv, lds, slot);
   slot ^= 1;
   c.pop = pv;
-  mbis_dispatch<KM, 1>(K, c, slot);
+  mbis_dispatch<KM, 1, PPL>(K, c, slot);
 }
 
 // ---------------------------------------------------------------------------

@@ -85,6 +85,14 @@ inline size_t weights_lds(size_t records, const WeightsShape &w) {
   return (records > pts ? records : pts) + 16;
 }
 
+// MBIS solver layout of a molecule whose atoms have at most maxR radial
+// points: 2 for one wavefront per atom with two points per lane (up to 128
+// points; measured faster than two wavefronts and a barrier per iteration),
+// else 1 for one workgroup per atom with one point per lane (three points per
+// lane measured no faster at 150).  It depends on the molecule alone, so that
+// its sums are the same alone and in any batch.
+inline int mbis_ppl(int maxR) { return maxR <= 128 ? 2 : 1; }
+
 // Radial points per lane of the one-wave-per-atom solvers.
 inline int pick_ppl(int R) {
   if (R <= 128) return 2;
