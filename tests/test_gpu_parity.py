@@ -93,6 +93,21 @@ def test_mbis_synthetic_water_degenerate_shells():
     _compare(part, cache, oracle, qtol=1e-10, ptol=1e-8)
 
 
+def test_mbis_workgroup_layout():
+    """Atoms of more than 128 radial points: the MBIS solver runs one
+    workgroup per atom (one point per lane, several wavefronts) instead of
+    one wavefront per atom."""
+    require_gpu()
+    _say("case: MBIS chain 4 atoms seed 5, 150x26 (solver: one workgroup per atom)")
+    w = workloads.chain(natom=4, seed=5, nrad=150, nang=26)
+    kw = dict(maxiter=60)
+    part = MBISWPart(w.coordinates, w.numbers, w.pseudo_numbers, w.grid, w.density, **kw)
+    part.do_partitioning()
+    oracle = OracleMBIS(w.coordinates, w.numbers, w.pseudo_numbers, w.grid, w.density, **kw)
+    cache = oracle.do_partitioning()
+    _compare(part, cache, oracle)
+
+
 def test_mbis_chain():
     require_gpu()
     _say("case: MBIS chain 6 atoms seed 7, 50x110")

@@ -49,6 +49,9 @@ def _workload(name):
             _W[name] = workloads.chain(natom=8, seed=12, nrad=36, nang=26)
         elif name == "chain3":
             _W[name] = workloads.chain(natom=3, seed=13, nrad=40, nang=38)
+        elif name == "chain4r":
+            # more than 128 radial points: the MBIS solver's workgroup layout
+            _W[name] = workloads.chain(natom=4, seed=14, nrad=140, nang=26)
         else:
             raise KeyError(name)
     return _W[name]
@@ -111,6 +114,7 @@ MIX = [
     ("gisa", "chain8", "mixed"),
     ("mbis", "chain3", "mixed"),
     ("gisa", "chain3", "fp64"),
+    ("mbis", "chain4r", "fp64"),
 ]
 
 
