@@ -649,10 +649,12 @@ inline hipEvent_t &tick_event(hp_batch *b, int t, int g, int e) {
 }
 
 // One outer iteration of every molecule in flight (tick t of the step).  The
-// weights launches of the groups take turns in a ring (group g after the
-// previous group of the same tick, the first group after the last group of
-// the previous tick), so that each group's latency-bound per-atom solves run
-// beside the other groups' weights passes.
+// weights launches of the groups take turns within the tick (group g after
+// the previous group of the same tick), so that each group's latency-bound
+// per-atom solves run beside the other groups' weights passes.  The first
+// group of a tick does not wait for the last group of the previous one: a
+// group held up by a long inner solve then delays only the groups after it
+// in its own tick.
 int enqueue_tick(hp_batch *b, int t) {
   int used[kMaxGroups], nused = 0;
   for (int g = 0; g < kMaxGroups; ++g)
@@ -661,12 +663,7 @@ int enqueue_tick(hp_batch *b, int t) {
     const int g = used[u];
     const int ns = b->nslot[g];
     hipStream_t s = b->gs[g];
-    if (nused > 1) {
-      if (u > 0)
-        HP_HIP(hipStreamWaitEvent(s, tick_event(b, t, used[u - 1], 2), 0));
-      else if (t > 0)
-        HP_HIP(hipStreamWaitEvent(s, tick_event(b, t - 1, used[nused - 1], 2), 0));
-    }
+    if (u > 0) HP_HIP(hipStreamWaitEvent(s, tick_event(b, t, used[u - 1], 2), 0));
     const int *slot_list = b->d_slot_list + b->slot_off[g];
     HP_HIP(hipEventRecord(tick_event(b, t, g, 0), s));
     hipLaunchKernelGGL(k_batch_begin, dim3(ns), dim3(256), 0, s, b->slots, slot_list, b->par_m, b->old_m, b->par_l,
