@@ -1,13 +1,19 @@
 """Exponential basis functions of the GISA and LISA pro-atoms.
 
-Restates ``core/basis.py``: ``evaluate_function`` (85-186) computes
-``population * n alpha^(3/n) / (4 pi Gamma(3/n)) * exp(-alpha r^n)`` and its
-radial derivative; ``ExpBasisFuncHelper`` (241-327) holds per-element orders,
-exponents and initial values loaded from JSON/YAML tables (``load_params``
-42-82).  The tables shipped in ``data/`` are the reference's ``gauss.json``
-and ``slater.json`` parameter sets (physical constants of the method).
-The device kernels evaluate the same functions; this module supplies their
-parameters (``norm`` is the prefactor at unit population).
+The public names are those of the reference's ``core/basis.py``
+(``evaluate_function`` 85-186, ``load_params`` 42-82, ``BasisFuncHelper`` and
+``AnalyticBasisFuncHelper`` 189-238, ``ExpBasisFuncHelper`` 241-327), with the
+same arguments, return shapes and exceptions, because they are the interface
+a caller of the scheme classes sees.
+
+A shell is ``c * g(n, alpha) * exp(-alpha r^n)`` with the normalisation
+``g(n, alpha) = n alpha^(3/n) / (4 pi Gamma(3/n))`` (``4 pi int r^2 g e^{-alpha
+r^n} dr = 1``).  One routine, ``shell_norm``, computes ``g`` for the host
+evaluation and for the device tables, so the kernels and the host agree on
+the prefactor bit for bit.  The radial derivative returned with
+``nderiv=1`` is ``-n r^(n-1) f`` (no factor alpha), which is what the
+reference returns.  The tables in ``data/`` are the reference's
+``gauss.json`` and ``slater.json`` parameter sets.
 """
 
 import json
@@ -17,68 +23,76 @@ import numpy as np
 
 from .utils import DATA_PATH
 
-__all__ = ["evaluate_function", "load_params", "BasisFuncHelper", "AnalyticBasisFuncHelper", "ExpBasisFuncHelper"]
-
-
-def load_params(filename, extension="json"):
-    """Read ``{Z: [orders, exponents, initials]}`` from a JSON or YAML file."""
-    assert extension.lower() in ["json", "yaml"], "Format must be 'json' or 'yaml'."
-    with open(filename) as f:
-        if extension.lower() == "json":
-            data = json.load(f)
-        else:
-            import yaml
-
-            data = yaml.safe_load(f)
-    orders, exps, inits = {}, {}, {}
-    for number, values in data.items():
-        number = int(number)
-        orders[number] = np.asarray(values[0])
-        exps[number] = np.asarray(values[1])
-        inits[number] = np.asarray(values[2])
-    return orders, exps, inits
-
-
-def evaluate_function(n, population, alpha, r, nderiv=0, axis=None):
-    """Normalised ``exp(-alpha r^n)`` shells (vectorised over n/alpha arrays)."""
-    if np.any(np.asarray(n) <= 0):
-        raise ValueError("n must be positive")
-    if np.any(np.asarray(alpha) < 0):
-        raise ValueError("alpha must be non-negative")
-    if not isinstance(r, np.ndarray):
-        raise ValueError("r must be a numpy array")
-    from scipy.special import gamma
-
-    prefactor = population * n * alpha ** (3 / n) / (4 * np.pi * gamma(3 / n))
-    vec = not np.isscalar(n)
-    if vec:
-        assert not (np.isscalar(alpha) or np.isscalar(population))
-        prefactor = prefactor[:, np.newaxis]
-        alpha = alpha[:, np.newaxis]
-        r = r[np.newaxis, :]
-        n = n[:, np.newaxis]
-    f = prefactor * np.exp(-alpha * r**n)
-    if nderiv > 0:
-        # the reference's derivative has no factor alpha (core/basis.py:175)
-        df = -n * r ** (n - 1) * f
-    if vec and axis is not None:
-        f = np.sum(f, axis=axis)
-        if nderiv > 0:
-            df = np.sum(df, axis=axis)
-    if nderiv == 0:
-        return f
-    if nderiv == 1:
-        return f, df
-    raise NotImplementedError
+__all__ = ["evaluate_function", "load_params", "shell_norm", "BasisFuncHelper", "AnalyticBasisFuncHelper", "ExpBasisFuncHelper"]
 
 
 def shell_norm(order, alpha):
-    """Prefactor of ``evaluate_function`` at population 1 (same expression)."""
-    return 1.0 * order * alpha ** (3 / order) / (4 * np.pi * math.gamma(3 / order))
+    """``n alpha^(3/n) / (4 pi Gamma(3/n))`` for scalars or arrays of equal shape."""
+    n = np.asarray(order, dtype=float)
+    a = np.asarray(alpha, dtype=float)
+    gam = np.vectorize(math.gamma, otypes=[float])(3.0 / n)
+    g = n * a ** (3.0 / n) / (4.0 * np.pi * gam)
+    return float(g) if g.ndim == 0 else g
+
+
+def _check_args(n, alpha, r):
+    if (np.asarray(n) <= 0).any():
+        raise ValueError("n must be positive")
+    if (np.asarray(alpha) < 0).any():
+        raise ValueError("alpha must be non-negative")
+    if not isinstance(r, np.ndarray):
+        raise ValueError("r must be a numpy array")
+
+
+def evaluate_function(n, population, alpha, r, nderiv=0, axis=None):
+    """Shells ``population * g(n, alpha) exp(-alpha r^n)`` on the radii ``r``.
+
+    Scalar ``n`` gives an array shaped like ``r``.  Arrays ``n``, ``alpha``
+    and ``population`` (one entry per shell) give one row per shell, summed
+    over ``axis`` when it is given.  ``nderiv=1`` also returns the radial
+    derivative ``-n r^(n-1) f``; any other order raises NotImplementedError.
+    """
+    _check_args(n, alpha, r)
+    if nderiv not in (0, 1):
+        raise NotImplementedError
+    per_shell = not np.isscalar(n)
+    if per_shell:
+        # shells along the first axis, radii along the second
+        col = lambda v: np.asarray(v, dtype=float).reshape(-1, 1)  # noqa: E731
+        n_, a_, c_, rr = col(n), col(alpha), col(population), np.asarray(r)[None, :]
+    else:
+        n_, a_, c_, rr = n, alpha, population, r
+    rn = rr**n_
+    f = c_ * shell_norm(n_, a_) * np.exp(-a_ * rn)
+    out = [f]
+    if nderiv == 1:
+        out.append(-n_ * rr ** (n_ - 1) * f)
+    if per_shell and axis is not None:
+        out = [np.sum(v, axis=axis) for v in out]
+    return out[0] if nderiv == 0 else tuple(out)
+
+
+def load_params(filename, extension="json"):
+    """``(orders, exponents, initials)`` dictionaries keyed by atomic number
+    from a file mapping ``Z -> [orders, exponents, initials]`` (JSON or YAML)."""
+    kind = extension.lower()
+    assert kind in ("json", "yaml"), "Format must be 'json' or 'yaml'."
+    with open(filename) as handle:
+        if kind == "json":
+            table = json.load(handle)
+        else:
+            import yaml
+
+            table = yaml.safe_load(handle)
+    columns = ({}, {}, {})
+    for key, rows in table.items():
+        for column, row in zip(columns, rows):
+            column[int(key)] = np.asarray(row)
+    return columns
 
 
 class BasisFuncHelper:
-    """Base of basis function helpers."""
+    """Per-element basis description: initial values and shell densities."""
 
     def __init__(self, initials, *args, **kwargs):
         self._initials = initials
@@ -91,29 +105,33 @@ class BasisFuncHelper:
         raise NotImplementedError
 
     def get_initial(self, number, ishell=None):
-        return np.asarray(self.initials[number]) if ishell is None else self.initials[number][ishell]
+        values = self.initials[number]
+        return np.asarray(values) if ishell is None else values[ishell]
 
     def compute_proshell_dens(self, number, ishell, population, points, nderiv=0):
         raise NotImplementedError
 
     def compute_proatom_dens(self, number, populations, points, nderiv=0):
-        y = d = 0.0
-        for i in range(self.get_nshell(number)):
-            res = self.compute_proshell_dens(number, i, populations[i], points, nderiv)
-            if nderiv == 0:
-                y += res
-            else:
-                y += res[0]
-                d += res[1]
-        return y if nderiv == 0 else (y, d)
+        """Sum of the element's shells with the given populations (and the
+        sum of their derivatives for ``nderiv=1``)."""
+        if nderiv not in (0, 1):
+            raise RuntimeError("The argument `nderiv` should only be 0 or 1.")
+        total = [0.0, 0.0]
+        for ishell in range(self.get_nshell(number)):
+            res = self.compute_proshell_dens(number, ishell, populations[ishell], points, nderiv)
+            parts = (res,) if nderiv == 0 else res
+            for j, v in enumerate(parts):
+                total[j] = total[j] + v
+        return total[0] if nderiv == 0 else (total[0], total[1])
 
 
 class AnalyticBasisFuncHelper(BasisFuncHelper):
-    """Analytic basis function helper."""
+    """Helpers whose shells are closed-form functions of the radius."""
 
 
 class ExpBasisFuncHelper(AnalyticBasisFuncHelper):
-    """Per-element ``exp(-alpha r^n)`` basis (orders, exponents, initials)."""
+    """Shells ``exp(-alpha r^n)`` with per-element orders, exponents and
+    initial populations."""
 
     def __init__(self, exponents_orders, exponents=None, initials=None):
         super().__init__(initials)
@@ -129,52 +147,51 @@ class ExpBasisFuncHelper(AnalyticBasisFuncHelper):
         return self._exponents
 
     def get_nshell(self, number):
-        return len(self.exponents[number])
+        return len(self._exponents[number])
 
     def get_order(self, number, ishell=None):
-        return self.orders[number] if ishell is None else self.orders[number][ishell]
+        values = self._orders[number]
+        return values if ishell is None else values[ishell]
 
     def get_exponent(self, number, ishell=None):
-        return np.asarray(self.exponents[number]) if ishell is None else self.exponents[number][ishell]
+        values = self._exponents[number]
+        return np.asarray(values) if ishell is None else values[ishell]
 
     def compute_proshell_dens(self, number, ishell, population, points, nderiv=0):
-        return evaluate_function(
-            self.get_order(number, ishell), population, self.get_exponent(number, ishell), points, nderiv
-        )
+        n = self.get_order(number, ishell)
+        alpha = self.get_exponent(number, ishell)
+        return evaluate_function(n, population, alpha, points, nderiv)
 
     def device_tables(self, numbers):
-        """Per-atom function offsets and per-function (alpha, order, norm)."""
-        offsets = [0]
-        alpha, order, norm = [], [], []
-        for z in numbers:
-            z = int(z)
-            if z not in self.exponents:
-                raise KeyError(f"no basis functions for element {z}")
-            for k in range(self.get_nshell(z)):
-                a = float(self.get_exponent(z, k))
-                n = float(self.get_order(z, k))
-                if n <= 0:
-                    raise ValueError("n must be positive")
-                if a < 0:
-                    raise ValueError("alpha must be non-negative")
-                alpha.append(a)
-                order.append(n)
-                norm.append(shell_norm(n, a))
-            offsets.append(offsets[-1] + self.get_nshell(z))
-        return (np.array(offsets, np.int32), np.array(alpha), np.array(order), np.array(norm))
+        """The per-atom tables the device kernels take: function offsets
+        (``natom + 1``) and per function its exponent, order and ``g``."""
+        missing = sorted({int(z) for z in numbers} - {int(z) for z in self._exponents})
+        if missing:
+            raise KeyError(f"no basis functions for element {missing[0]}")
+        alpha = np.concatenate([np.asarray(self._exponents[int(z)], float).ravel() for z in numbers])
+        order = np.concatenate([np.asarray(self._orders[int(z)], float).ravel() for z in numbers])
+        counts = [self.get_nshell(int(z)) for z in numbers]
+        _check_args(order, alpha, np.zeros(0))
+        offsets = np.zeros(len(counts) + 1, np.int32)
+        np.cumsum(counts, out=offsets[1:])
+        norm = np.atleast_1d(shell_norm(order, alpha)) if order.size else np.zeros(0)
+        return offsets, alpha, order, norm
 
     @classmethod
     def from_function_type(cls, func_type="gauss"):
-        assert func_type in ["gauss", "slater"]
+        """The shipped ``gauss`` or ``slater`` parameter set."""
+        assert func_type in ("gauss", "slater")
         return cls.from_file(DATA_PATH / f"{func_type}.json")
 
     @classmethod
     def from_file(cls, filename):
-        ext = "yaml" if str(filename).endswith(".yaml") else "json"
-        orders, exponents, initials = load_params(filename, extension=ext)
-        for number, exps in exponents.items():
-            if number not in initials:
-                initials[number] = np.ones_like(exps) / len(exps)
+        """Parameters from a JSON or YAML table; elements without initial
+        values start from equal populations."""
+        kind = "yaml" if str(filename).endswith(".yaml") else "json"
+        orders, exponents, initials = load_params(filename, extension=kind)
+        for number in exponents.keys() - initials.keys():
+            k = len(exponents[number])
+            initials[number] = np.full(k, 1.0 / k)
         return cls(orders, exponents, initials)
 
     from_yaml = from_file
@@ -182,12 +199,12 @@ class ExpBasisFuncHelper(AnalyticBasisFuncHelper):
 
     @classmethod
     def uniform(cls, numbers, exponents, order=1.0, initials=None):
-        """Same ``exponents`` (and order) for every element in ``numbers``."""
-        exponents = np.asarray(exponents, float)
-        ords = {int(z): np.full(exponents.size, float(order)) for z in numbers}
-        exps = {int(z): exponents.copy() for z in numbers}
-        if initials is None:
-            inits = {int(z): np.ones(exponents.size) / exponents.size for z in numbers}
-        else:
-            inits = {int(z): np.asarray(initials, float) for z in numbers}
-        return cls(ords, exps, inits)
+        """The same exponents (and order) for every element in ``numbers``."""
+        exps = np.asarray(exponents, float)
+        start = np.full(exps.size, 1.0 / exps.size) if initials is None else np.asarray(initials, float)
+        elements = sorted({int(z) for z in numbers})
+        return cls(
+            {z: np.full(exps.size, float(order)) for z in elements},
+            {z: exps.copy() for z in elements},
+            {z: start.copy() for z in elements},
+        )

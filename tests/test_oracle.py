@@ -272,3 +272,146 @@ def test_golden_vectors():
         assert got["niter"] == exp["niter"], name
         np.testing.assert_allclose(got["charges"], exp["charges"], atol=1e-10, err_msg=name)
         np.testing.assert_allclose(got["propars"], exp["propars"], rtol=1e-9, atol=1e-12, err_msg=name)
+
+
+# --- moments and entropy by closed forms ---------------------------------------
+
+
+def _one_atom_oracle(shift, alpha=2.0, pop=1.5, nrad=150, nang=302, z=3):
+    """An oracle object for one atom at C, its whole density in the AIM (weights
+    1) and equal to a Slater density pop a^3/(8 pi) exp(-a |p - C - shift|)."""
+    from horton_part_amd.grid import BeckeWeights, MolGrid
+
+    center = np.array([[0.3, -0.2, 0.1]])
+    numbers = np.array([z], dtype=np.int64)
+    grid = MolGrid.from_size(numbers, center, nang, exp_radial_grid(nrad), BeckeWeights())
+    r = np.linalg.norm(grid.points - center[0] - shift, axis=1)
+    rho = pop * alpha**3 / (8 * np.pi) * np.exp(-alpha * r)
+    o = ho.OracleMBIS(center, numbers, None, grid, rho)
+    o.cache["at_weights_0"] = np.ones(grid.size)
+    return o
+
+
+def _solid_harmonics_textbook(s):
+    """Racah-normalised real regular solid harmonics at the point s, written out
+    (c_l0, c_l1, s_l1, c_l2, s_l2, ...; no Condon-Shortley phase)."""
+    x, y, z = s
+    r2 = x * x + y * y + z * z
+    return np.array([
+        1.0,
+        z, x, y,
+        (3 * z * z - r2) / 2, np.sqrt(3) * x * z, np.sqrt(3) * y * z, np.sqrt(3) / 2 * (x * x - y * y), np.sqrt(3) * x * y,
+        z * (5 * z * z - 3 * r2) / 2, np.sqrt(3 / 8) * x * (5 * z * z - r2), np.sqrt(3 / 8) * y * (5 * z * z - r2),
+        np.sqrt(15) / 2 * z * (x * x - y * y), np.sqrt(15) * x * y * z,
+        np.sqrt(5 / 8) * x * (x * x - 3 * y * y), np.sqrt(5 / 8) * y * (3 * x * x - y * y),
+    ])
+
+
+def test_oracle_moments_closed_forms():
+    """``OracleWPart.do_moments`` (core/base.py:329-402) on a displaced Slater
+    density N a^3/(8 pi) e^{-a|u|}, u = p - C - s, whose moments about C are
+    known in closed form (<r^2> = 12/a^2 about its own centre):
+
+    * cartesian: <x> = s_x, <x^2> = s_x^2 + <r^2>/3, <xy> = s_x s_y,
+      <x^3> = s_x^3 + s_x <r^2>, <x^2 y> = s_x^2 s_y + s_y <r^2>/3, <xyz> = s_x s_y s_z,
+      times -N (electron sign), plus Z on the monopole;
+    * pure: the solid harmonics are harmonic polynomials, so their average over
+      a spherical density centred at s is their value at s: -N R_lm(s);
+    * radial: N, N(<r^2> + |s|^2) for n = 0, 2; N 3/a and N 60/a^3 for n = 1, 3
+      at s = 0 (no sign flip)."""
+    a, N, Z = 2.0, 1.5, 3
+    s = np.array([0.05, -0.03, 0.04])
+    o = _one_atom_oracle(s, alpha=a, pop=N, z=Z)
+    cart, pure, rad = o.do_moments()
+    m2 = 12 / a**2
+    sx, sy, sz = s
+    second = {(2, 0, 0): sx * sx + m2 / 3, (1, 1, 0): sx * sy, (1, 0, 1): sx * sz,
+              (0, 2, 0): sy * sy + m2 / 3, (0, 1, 1): sy * sz, (0, 0, 2): sz * sz + m2 / 3}
+
+    def third(p):
+        out = np.prod([s[i] ** p[i] for i in range(3)])
+        for i in range(3):  # one factor pairs with the variance m2/3 of an axis
+            if p[i] >= 1:
+                q = list(p)
+                q[i] -= 2 if p[i] >= 2 else 0
+                if p[i] >= 2:
+                    out += (3 if p[i] == 3 else 1) * m2 / 3 * np.prod([s[j] ** q[j] for j in range(3)])
+        return out
+
+    # HORTON's cartesian order, written out
+    powers = [(0, 0, 0), (1, 0, 0), (0, 1, 0), (0, 0, 1),
+              (2, 0, 0), (1, 1, 0), (1, 0, 1), (0, 2, 0), (0, 1, 1), (0, 0, 2),
+              (3, 0, 0), (2, 1, 0), (2, 0, 1), (1, 2, 0), (1, 1, 1), (1, 0, 2), (0, 3, 0), (0, 2, 1), (0, 1, 2), (0, 0, 3)]
+    expect = []
+    for p in powers:
+        l = sum(p)
+        if l == 0:
+            expect.append(1.0)
+        elif l == 1:
+            expect.append(s[int(np.argmax(p))])
+        elif l == 2:
+            expect.append(second[p])
+        else:
+            expect.append(third(p))
+    expect = -N * np.array(expect)
+    expect[0] += Z
+    np.testing.assert_allclose(cart[0], expect, rtol=1e-7, atol=1e-9)
+    pure_expect = -N * _solid_harmonics_textbook(s)
+    pure_expect[0] += Z
+    np.testing.assert_allclose(pure[0], pure_expect, rtol=1e-7, atol=1e-9)
+    assert rad[0, 0] == pytest.approx(N, rel=1e-8)
+    assert rad[0, 2] == pytest.approx(N * (m2 + s @ s), rel=1e-7)
+    o0 = _one_atom_oracle(np.zeros(3), alpha=a, pop=N, z=Z)
+    _, _, rad0 = o0.do_moments()
+    np.testing.assert_allclose(rad0[0], N * np.array([1.0, 3 / a, m2, 60 / a**3]), rtol=1e-7)
+
+
+def test_oracle_moments_fail_on_planted_convention_errors():
+    """The closed forms above separate the conventions: a Condon-Shortley
+    phase (-1)^m or a swapped (cos, sin) order in the pure moments moves them
+    away from -N R_lm(s)."""
+    s = np.array([0.05, -0.03, 0.04])
+    good = _solid_harmonics_textbook(s)
+    d = s[None, :]
+    from oracle.horton_oracle import real_solid_harmonics
+
+    np.testing.assert_allclose(real_solid_harmonics(3, d)[:, 0], good, rtol=1e-12, atol=1e-15)
+    phase = good.copy()
+    phase[[2, 3, 10, 11, 14, 15]] *= -1  # (-1)^m on the m = 1, 3 pairs
+    swapped = good.copy()
+    swapped[[2, 3]] = swapped[[3, 2]]
+    for bad in (phase, swapped):
+        assert np.abs(bad - real_solid_harmonics(3, d)[:, 0]).max() > 1e-3
+
+
+def test_oracle_entropy_closed_forms():
+    """``compute_entropy`` (core/stockholder.py:145-151) is
+    sum_p w^mol_p rho ln(rho/rho0) over the molecular grid (Becke weights
+    included) with points where rho or rho0 < 1e-15 left out:
+
+    * rho = rho0 gives 0;
+    * rho = c rho0 gives c ln(c) int rho0, here c ln(c) N for a normalised
+      Slater density on a two-atom molecular grid;
+    * points below the density cut-off contribute nothing."""
+    from horton_part_amd.grid import BeckeWeights, MolGrid
+
+    coords = np.array([[0.0, 0.0, 0.0], [2.2, 0.3, -0.4]])
+    numbers = np.array([6, 8], dtype=np.int64)
+    grid = MolGrid.from_size(numbers, coords, 194, exp_radial_grid(120), BeckeWeights())
+    N, a = 2.5, 1.7
+    rho0 = N * a**3 / (8 * np.pi) * np.exp(-a * np.linalg.norm(grid.points - coords[1], axis=1))
+    assert grid.integrate(rho0) == pytest.approx(N, rel=1e-8)
+    for c in (1.0, 0.7, 1.9):
+        o = ho.OracleMBIS(coords, numbers, None, grid, c * rho0)
+        o.cache["promoldens"] = rho0
+        expect = 0.0 if c == 1.0 else c * np.log(c) * N
+        assert o.compute_entropy() == pytest.approx(expect, rel=1e-7, abs=1e-12)
+    # the sick mask: the density set below the cut-off on half the points
+    rho = 1.9 * rho0
+    cut = np.arange(grid.size) % 2 == 0
+    rho[cut] = 1e-16
+    o = ho.OracleMBIS(coords, numbers, None, grid, rho)
+    o.cache["promoldens"] = rho0
+    keep = ~cut & (rho0 >= 1e-15)
+    expect = np.sum(grid.weights[keep] * rho[keep] * np.log(1.9))
+    assert o.compute_entropy() == pytest.approx(expect, rel=1e-12)
