@@ -43,6 +43,8 @@ struct hp_part {
   int natom = 0, nshell = 0, nfn = 0, npar = 0;
   int64_t npoint = 0;
   int scheme = 0, solver = 0, grid_type = 1, fnkind = HP_FN_SLATER;
+  int only_molgrid = 0;  // grid_type 3: moments on the molecular grid (the loop runs as grid_type 2)
+  int weights_ready = 0; // a weights pass has run since the parameters were first set
   int maxL = 0, maxR = 0, maxK = 0;
   hp::WeightsShape wshape{1, 64, 1};  // weights pass: points per thread, threads, shells per workgroup
   int ngroup = 0;                     // weights workgroups
@@ -512,8 +514,8 @@ int hp_part_create(int device, const hp_molecule_desc *md, const hp_scheme_desc 
       !md->shell_rweight)
     return fail(HP_E_ARG, "molecule description has a NULL array");
   if (sd->scheme < 0 || sd->scheme > 2) return fail(HP_E_ARG, "unknown scheme");
-  if (sd->grid_type != 1 && sd->grid_type != 2) return fail(HP_E_UNSUPP, "the device path implements grid types 1 and 2");
-  if (sd->grid_type == 2 && sd->check_mono) return fail(HP_E_UNSUPP, "check_mono is defined on radial grids (grid_type 1)");
+  if (sd->grid_type < 1 || sd->grid_type > 3) return fail(HP_E_ARG, "grid_type must be 1, 2 or 3");
+  if (sd->grid_type >= 2 && sd->check_mono) return fail(HP_E_UNSUPP, "check_mono is defined on radial grids (grid_type 1)");
   if (!sd->fn_offsets) return fail(HP_E_ARG, "fn_offsets is NULL");
   const bool mbis = sd->scheme == HP_SCHEME_MBIS;
   if (mbis && sd->solver != HP_SOLVER_MBIS) return fail(HP_E_ARG, "MBIS uses HP_SOLVER_MBIS");
@@ -559,8 +561,8 @@ int hp_part_create(int device, const hp_molecule_desc *md, const hp_scheme_desc 
     maxK = std::max(maxK, K);
   }
   if (mbis && maxK > 7) return fail(HP_E_UNSUPP, "MBIS supports at most 7 shells per atom");
-  if (sd->grid_type == 2 && maxK > HP_GT2_KM) return fail(HP_E_UNSUPP, "grid_type 2 supports at most 8 functions per atom");
-  if (sd->grid_type == 2 && sd->solver == HP_SOLVER_MNEWTON && sd->newton_linspace > HP_GT2_NLIN)
+  if (sd->grid_type >= 2 && maxK > HP_GT2_KM) return fail(HP_E_UNSUPP, "grid_type 2 supports at most 8 functions per atom");
+  if (sd->grid_type >= 2 && sd->solver == HP_SOLVER_MNEWTON && sd->newton_linspace > HP_GT2_NLIN)
     return fail(HP_E_UNSUPP, "grid_type 2 line search supports at most 32 candidates");
   if (maxK > HP_KMAX) return fail(HP_E_UNSUPP, "too many functions per atom");
   int fnkind = HP_FN_SLATER;
@@ -584,7 +586,10 @@ int hp_part_create(int device, const hp_molecule_desc *md, const hp_scheme_desc 
   p->npar = mbis ? 2 * nfn : nfn;
   p->scheme = sd->scheme;
   p->solver = sd->solver;
-  p->grid_type = sd->grid_type;
+  // grid_type 3 runs the molecular-grid loop of grid_type 2 (core/base.py:490-499:
+  // both set on_molgrid); it differs in where the moments integrate
+  p->grid_type = sd->grid_type == 3 ? 2 : sd->grid_type;
+  p->only_molgrid = sd->grid_type == 3;
   p->fnkind = fnkind;
   p->maxL = maxL;
   p->maxR = maxR;
@@ -776,6 +781,37 @@ int hp_part_set_propars(hp_part *p, const double *propars, int64_t n) {
   HP_HIP(hipMemcpyAsync(p->old_par, propars, sizeof(double) * n, hipMemcpyHostToDevice, p->stream));
   HP_HIP(hipStreamSynchronize(p->stream));
   p->niter = 0;
+  p->weights_ready = 0;
+  return 0;
+}
+
+int hp_part_weights(hp_part *p, double *rhobar, double *entropy) {
+  if (!p) return fail(HP_E_ARG, "NULL part");
+  if (rhobar && p->grid_type != 1) return fail(HP_E_UNSUPP, "spherical averages exist for grid_type 1 only");
+  HP_HIP(hipSetDevice(p->device));
+  hipStream_t s = p->stream;
+  // the kernels skip their work while the device stop flag is set
+  HP_HIP(hipMemsetAsync(p->state, 0, sizeof(int), s));
+  const int n = std::max(p->npar, p->nfn);
+  hipLaunchKernelGGL(k_iter_begin, dim3((n + 255) / 256), dim3(256), 0, s, p->state, p->scheme == HP_SCHEME_MBIS ? 0 : 1,
+                     p->npar, p->nfn, p->par, p->old_par, p->fn_norm, p->fn_alpha, p->coef, p->alpha);
+  HP_LAUNCHED();
+  HP_HIP(hipEventRecord(p->ev[0], s));
+  launch_weights<double>(p, p->grid_type == 2);
+  HP_LAUNCHED();
+  HP_HIP(hipEventRecord(p->ev[1], s));
+  hipLaunchKernelGGL(k_sum_shells, dim3(1), dim3(256), 0, s, p->nshell, p->ent_part, p->out + 1);
+  HP_LAUNCHED();
+  if (rhobar) HP_HIP(hipMemcpyAsync(rhobar, p->rhobar, sizeof(double) * p->nshell, hipMemcpyDeviceToHost, s));
+  HP_HIP(hipMemcpyAsync(p->h_out + 1, p->out + 1, sizeof(double), hipMemcpyDeviceToHost, s));
+  HP_HIP(hipStreamSynchronize(s));
+  float ms = 0.f;
+  HP_HIP(hipEventElapsedTime(&ms, p->ev[0], p->ev[1]));
+  p->k_ms[HP_KERNEL_WEIGHTS] += ms;
+  p->k_n[HP_KERNEL_WEIGHTS] += 1;
+  p->last_entropy = p->h_out[1];
+  if (entropy) *entropy = p->h_out[1];
+  p->weights_ready = 1;
   return 0;
 }
 
@@ -901,7 +937,7 @@ static int point_weights(hp_part *p, int atom, int64_t p0, int64_t n, double *ou
 int hp_part_at_weights(hp_part *p, int atom, double *out, int64_t n) {
   if (!p || !out) return fail(HP_E_ARG, "NULL argument");
   if (atom < 0 || atom >= p->natom) return fail(HP_E_ARG, "atom index out of range");
-  if (p->niter == 0) return fail(HP_E_ARG, "no weights yet: run the partitioning first");
+  if (p->niter == 0 && !p->weights_ready) return fail(HP_E_ARG, "no weights yet: run the partitioning first");
   int64_t p0 = p->h_shell_off[p->h_atom_shell_off[atom]];
   int64_t p1 = p->h_shell_off[p->h_atom_shell_off[atom + 1]];
   if (p->grid_type == 2) {  // weights on the whole molecular grid
@@ -915,28 +951,46 @@ int hp_part_at_weights(hp_part *p, int atom, double *out, int64_t n) {
 int hp_part_promoldens(hp_part *p, double *out, int64_t n) {
   if (!p || !out) return fail(HP_E_ARG, "NULL argument");
   if (n != p->npoint) return fail(HP_E_ARG, "output length must equal npoint");
-  if (p->niter == 0) return fail(HP_E_ARG, "no promolecule yet: run the partitioning first");
+  if (p->niter == 0 && !p->weights_ready) return fail(HP_E_ARG, "no promolecule yet: run the partitioning first");
   return point_weights(p, -1, 0, n, out);
 }
 
 int hp_part_moments(hp_part *p, int lmax, double *cart, double *pure, double *radial) {
   if (!p) return fail(HP_E_ARG, "NULL part");
   if (lmax != 3) return fail(HP_E_UNSUPP, "the device moments are implemented for lmax = 3");
-  if (p->niter == 0) return fail(HP_E_ARG, "run the partitioning first");
+  if (p->niter == 0 && !p->weights_ready) return fail(HP_E_ARG, "run the partitioning first");
   HP_HIP(hipSetDevice(p->device));
   int rc = 0;
-  if (!p->mom_part && ((rc = dalloc(&p->mom_part, (size_t)p->nshell * HP_NMOM)) ||
+  // grid_type 3: every atom's AIM density on the whole molecular grid with the
+  // molecular weights (core/base.py:352-402 with local = False); otherwise on
+  // the atom's own grid with the atom-grid weights
+  const int nchunk = p->only_molgrid ? (int)((p->npoint + HP_MOM_CHUNK - 1) / HP_MOM_CHUNK) : 0;
+  const size_t npart = p->only_molgrid ? (size_t)p->natom * nchunk : (size_t)p->nshell;
+  if (!p->mom_part && ((rc = dalloc(&p->mom_part, npart * HP_NMOM)) ||
                        (rc = dalloc(&p->mom_out, (size_t)p->natom * HP_NMOM))))
     return rc;
   DevFn f = p->fn();
-  if (p->fnkind == HP_FN_SLATER)
-    hipLaunchKernelGGL((k_moments<HP_FN_SLATER>), dim3(p->nshell), dim3(256), 0, p->stream, p->mol(), f, p->mom_part);
-  else if (p->fnkind == HP_FN_GAUSS)
-    hipLaunchKernelGGL((k_moments<HP_FN_GAUSS>), dim3(p->nshell), dim3(256), 0, p->stream, p->mol(), f, p->mom_part);
-  else
-    hipLaunchKernelGGL((k_moments<HP_FN_GENERAL>), dim3(p->nshell), dim3(256), 0, p->stream, p->mol(), f, p->mom_part);
-  HP_LAUNCHED();
-  hipLaunchKernelGGL(k_moments_reduce, dim3(p->natom), dim3(64), 0, p->stream, p->mol(), p->mom_part, p->mom_out);
+  if (p->only_molgrid) {
+    const dim3 g((unsigned)nchunk, (unsigned)p->natom);
+    if (p->fnkind == HP_FN_SLATER)
+      hipLaunchKernelGGL((k_moments_mol<HP_FN_SLATER>), g, dim3(256), 0, p->stream, p->mol(), f, p->promol, p->mom_part);
+    else if (p->fnkind == HP_FN_GAUSS)
+      hipLaunchKernelGGL((k_moments_mol<HP_FN_GAUSS>), g, dim3(256), 0, p->stream, p->mol(), f, p->promol, p->mom_part);
+    else
+      hipLaunchKernelGGL((k_moments_mol<HP_FN_GENERAL>), g, dim3(256), 0, p->stream, p->mol(), f, p->promol, p->mom_part);
+    HP_LAUNCHED();
+    hipLaunchKernelGGL(k_moments_reduce_mol, dim3(p->natom), dim3(64), 0, p->stream, nchunk, p->mom_part, p->mom_out);
+  } else {
+    if (p->fnkind == HP_FN_SLATER)
+      hipLaunchKernelGGL((k_moments<HP_FN_SLATER>), dim3(p->nshell), dim3(256), 0, p->stream, p->mol(), f, p->mom_part);
+    else if (p->fnkind == HP_FN_GAUSS)
+      hipLaunchKernelGGL((k_moments<HP_FN_GAUSS>), dim3(p->nshell), dim3(256), 0, p->stream, p->mol(), f, p->mom_part);
+    else
+      hipLaunchKernelGGL((k_moments<HP_FN_GENERAL>), dim3(p->nshell), dim3(256), 0, p->stream, p->mol(), f,
+                         p->mom_part);
+    HP_LAUNCHED();
+    hipLaunchKernelGGL(k_moments_reduce, dim3(p->natom), dim3(64), 0, p->stream, p->mol(), p->mom_part, p->mom_out);
+  }
   HP_LAUNCHED();
   std::vector<double> h((size_t)p->natom * HP_NMOM);
   std::vector<double> z(p->natom);
@@ -994,34 +1048,69 @@ int hp_part_get_work(hp_part *p, hp_part_work *out) {
 // ---- grid utilities ----------------------------------------------------------
 
 namespace {
-struct GridUpload {
-  double *cx = nullptr, *cy = nullptr, *cz = nullptr, *px = nullptr, *py = nullptr, *pz = nullptr, *out = nullptr;
-  ~GridUpload() {
-    void *b[] = {cx, cy, cz, px, py, pz, out};
-    for (void *q : b)
-      if (q) (void)hipFree(q);
-  }
+// Device buffers and a stream of the calling host thread for the grid
+// utilities: they grow to the largest molecule seen and are kept until the
+// process ends, so that building a request's inputs while a batch serves
+// allocates and frees nothing (hipFree waits for the whole device) and waits
+// only for its own stream.
+struct GridScratch {
+  int device = -1;
+  hipStream_t stream = nullptr;
+  double *buf[8] = {};
+  size_t cap[8] = {};
 };
 
-int upload_points(GridUpload &g, int natom, const double *coords, int64_t npoint, const double *points) {
-  std::vector<double> ax(natom), ay(natom), az(natom);
-  for (int a = 0; a < natom; ++a) {
-    ax[a] = coords[3 * a];
-    ay[a] = coords[3 * a + 1];
-    az[a] = coords[3 * a + 2];
+thread_local GridScratch t_grid;
+
+int grid_scratch(int device, GridScratch **out) {
+  GridScratch &g = t_grid;
+  if (g.device != device) {
+    // a thread that changes devices leaves its buffers of the previous one
+    g = GridScratch();
+    HP_HIP(hipSetDevice(device));
+    HP_HIP(hipStreamCreateWithFlags(&g.stream, hipStreamNonBlocking));
+    g.device = device;
+  } else {
+    HP_HIP(hipSetDevice(device));
   }
-  std::vector<double> gx(npoint), gy(npoint), gz(npoint);
-  for (int64_t i = 0; i < npoint; ++i) {
-    gx[i] = points[3 * i];
-    gy[i] = points[3 * i + 1];
-    gz[i] = points[3 * i + 2];
+  *out = &g;
+  return 0;
+}
+
+// Buffer k of at least n bytes.
+int grid_buf(GridScratch *g, int k, size_t n, void **out) {
+  if (g->cap[k] < n) {
+    const size_t want = std::max(n, g->cap[k] + g->cap[k] / 2);
+    if (g->buf[k]) HP_HIP(hipFree(g->buf[k]));
+    g->buf[k] = nullptr;
+    g->cap[k] = 0;
+    HP_HIP(hipMalloc((void **)&g->buf[k], want));
+    g->cap[k] = want;
   }
+  *out = g->buf[k];
+  return 0;
+}
+
+// Atom centres and points as SoA device columns on the thread's stream
+// (buffers 0-1: centres, points).
+int upload_points(GridScratch *g, int natom, const double *coords, int64_t npoint, const double *points, double **c,
+                  double **p) {
+  std::vector<double> a3(3 * (size_t)natom);
+  for (int a = 0; a < natom; ++a)
+    for (int d = 0; d < 3; ++d) a3[(size_t)d * natom + a] = coords[3 * a + d];
+  std::vector<double> p3(3 * (size_t)npoint);
+  for (int64_t i = 0; i < npoint; ++i)
+    for (int d = 0; d < 3; ++d) p3[(size_t)d * npoint + i] = points[3 * i + d];
+  void *dc = nullptr, *dp = nullptr;
   int rc = 0;
-  if ((rc = dupload(&g.cx, ax.data(), natom, nullptr)) || (rc = dupload(&g.cy, ay.data(), natom, nullptr)) ||
-      (rc = dupload(&g.cz, az.data(), natom, nullptr)) || (rc = dupload(&g.px, gx.data(), npoint, nullptr)) ||
-      (rc = dupload(&g.py, gy.data(), npoint, nullptr)) || (rc = dupload(&g.pz, gz.data(), npoint, nullptr)) ||
-      (rc = dalloc(&g.out, npoint)))
+  if ((rc = grid_buf(g, 0, a3.size() * sizeof(double), &dc)) || (rc = grid_buf(g, 1, p3.size() * sizeof(double), &dp)))
     return rc;
+  HP_HIP(hipMemcpyAsync(dc, a3.data(), a3.size() * sizeof(double), hipMemcpyHostToDevice, g->stream));
+  HP_HIP(hipMemcpyAsync(dp, p3.data(), p3.size() * sizeof(double), hipMemcpyHostToDevice, g->stream));
+  // the host staging vectors go out of scope: wait for the copies
+  HP_HIP(hipStreamSynchronize(g->stream));
+  *c = (double *)dc;
+  *p = (double *)dp;
   return 0;
 }
 }  // namespace
@@ -1037,7 +1126,8 @@ int hp_becke_weights(int device, int natom, const double *coordinates, const dou
   for (int a = 0; a < natom; ++a)
     if (atom_offsets[a + 1] < atom_offsets[a]) return fail(HP_E_ARG, "atom_offsets must be non-decreasing");
   // size adjustment a_ij from the radius ratio chi = R_i / R_j, |a_ij| <= 1/2
-  std::vector<double> adj((size_t)natom * natom, 0.0), invr((size_t)natom * natom, 0.0);
+  std::vector<double> tab(2 * (size_t)natom * natom, 0.0);
+  double *adj = tab.data(), *invr = tab.data() + (size_t)natom * natom;
   for (int i = 0; i < natom; ++i)
     for (int j = 0; j < natom; ++j) {
       if (i == j) continue;
@@ -1053,25 +1143,25 @@ int hp_becke_weights(int device, int natom, const double *coordinates, const dou
       if (d == 0.0) return fail(HP_E_ARG, "two atoms at the same position");
       invr[(size_t)i * natom + j] = 1.0 / d;
     }
-  HP_HIP(hipSetDevice(device));
-  GridUpload g;
-  double *dadj = nullptr, *dinvr = nullptr;
-  int64_t *doff = nullptr;
-  int rc = upload_points(g, natom, coordinates, npoint, points);
-  if (!rc) rc = dupload(&dadj, adj.data(), adj.size(), nullptr);
-  if (!rc) rc = dupload(&dinvr, invr.data(), invr.size(), nullptr);
-  if (!rc) rc = dupload(&doff, atom_offsets, natom + 1, nullptr);
-  if (!rc) {
-    hipLaunchKernelGGL(k_becke, dim3((unsigned)((npoint + 255) / 256)), dim3(256), 0, nullptr, natom, g.cx, g.cy, g.cz,
-                       dadj, dinvr, npoint, g.px, g.py, g.pz, doff, 3, g.out);
-    hipError_t e = hipGetLastError();
-    if (e == hipSuccess) e = hipMemcpy(out, g.out, sizeof(double) * npoint, hipMemcpyDeviceToHost);
-    if (e != hipSuccess) rc = fail(HP_E_HIP, std::string("becke: ") + hipGetErrorString(e));
-  }
-  if (dadj) (void)hipFree(dadj);
-  if (dinvr) (void)hipFree(dinvr);
-  if (doff) (void)hipFree(doff);
-  return rc;
+  GridScratch *g = nullptr;
+  double *c = nullptr, *p = nullptr;
+  void *dtab = nullptr, *doff = nullptr, *dout = nullptr;
+  int rc = grid_scratch(device, &g);
+  if (!rc) rc = upload_points(g, natom, coordinates, npoint, points, &c, &p);
+  if (!rc) rc = grid_buf(g, 2, tab.size() * sizeof(double), &dtab);
+  if (!rc) rc = grid_buf(g, 3, (natom + 1) * sizeof(int64_t), &doff);
+  if (!rc) rc = grid_buf(g, 4, npoint * sizeof(double), &dout);
+  if (rc) return rc;
+  HP_HIP(hipMemcpyAsync(dtab, tab.data(), tab.size() * sizeof(double), hipMemcpyHostToDevice, g->stream));
+  HP_HIP(hipMemcpyAsync(doff, atom_offsets, (natom + 1) * sizeof(int64_t), hipMemcpyHostToDevice, g->stream));
+  const double *dadj = (const double *)dtab, *dinvr = (const double *)dtab + (size_t)natom * natom;
+  hipLaunchKernelGGL(k_becke, dim3((unsigned)((npoint + 255) / 256)), dim3(256), 0, g->stream, natom, c, c + natom,
+                     c + 2 * natom, dadj, dinvr, npoint, p, p + npoint, p + 2 * npoint, (const int64_t *)doff, 3,
+                     (double *)dout);
+  HP_LAUNCHED();
+  HP_HIP(hipMemcpyAsync(out, dout, sizeof(double) * npoint, hipMemcpyDeviceToHost, g->stream));
+  HP_HIP(hipStreamSynchronize(g->stream));
+  return 0;
 }
 
 int hp_slater_density(int device, int natom, const double *coordinates, const int32_t *fn_offsets,
@@ -1081,26 +1171,29 @@ int hp_slater_density(int device, int natom, const double *coordinates, const in
     return fail(HP_E_ARG, "bad argument");
   const int nfn = fn_offsets[natom];
   if (fn_offsets[0] != 0 || nfn <= 0) return fail(HP_E_ARG, "bad fn_offsets");
-  HP_HIP(hipSetDevice(device));
-  GridUpload g;
-  int *doff = nullptr;
-  double *dcoef = nullptr, *dalpha = nullptr;
-  std::vector<int> off(fn_offsets, fn_offsets + natom + 1);
-  int rc = upload_points(g, natom, coordinates, npoint, points);
-  if (!rc) rc = dupload(&doff, off.data(), natom + 1, nullptr);
-  if (!rc) rc = dupload(&dcoef, coef, nfn, nullptr);
-  if (!rc) rc = dupload(&dalpha, alpha, nfn, nullptr);
-  if (!rc) {
-    hipLaunchKernelGGL(k_slater_sum, dim3((unsigned)((npoint + 255) / 256)), dim3(256), 0, nullptr, natom, g.cx, g.cy,
-                       g.cz, doff, dcoef, dalpha, npoint, g.px, g.py, g.pz, g.out);
-    hipError_t e = hipGetLastError();
-    if (e == hipSuccess) e = hipMemcpy(out, g.out, sizeof(double) * npoint, hipMemcpyDeviceToHost);
-    if (e != hipSuccess) rc = fail(HP_E_HIP, std::string("slater density: ") + hipGetErrorString(e));
-  }
-  if (doff) (void)hipFree(doff);
-  if (dcoef) (void)hipFree(dcoef);
-  if (dalpha) (void)hipFree(dalpha);
-  return rc;
+  for (int a = 0; a < natom; ++a)
+    if (fn_offsets[a + 1] < fn_offsets[a]) return fail(HP_E_ARG, "fn_offsets must be non-decreasing");
+  std::vector<double> tab(2 * (size_t)nfn);
+  std::copy(coef, coef + nfn, tab.begin());
+  std::copy(alpha, alpha + nfn, tab.begin() + nfn);
+  GridScratch *g = nullptr;
+  double *c = nullptr, *p = nullptr;
+  void *dtab = nullptr, *doff = nullptr, *dout = nullptr;
+  int rc = grid_scratch(device, &g);
+  if (!rc) rc = upload_points(g, natom, coordinates, npoint, points, &c, &p);
+  if (!rc) rc = grid_buf(g, 5, tab.size() * sizeof(double), &dtab);
+  if (!rc) rc = grid_buf(g, 6, (natom + 1) * sizeof(int32_t), &doff);
+  if (!rc) rc = grid_buf(g, 4, npoint * sizeof(double), &dout);
+  if (rc) return rc;
+  HP_HIP(hipMemcpyAsync(dtab, tab.data(), tab.size() * sizeof(double), hipMemcpyHostToDevice, g->stream));
+  HP_HIP(hipMemcpyAsync(doff, fn_offsets, (natom + 1) * sizeof(int32_t), hipMemcpyHostToDevice, g->stream));
+  hipLaunchKernelGGL(k_slater_sum, dim3((unsigned)((npoint + 255) / 256)), dim3(256), 0, g->stream, natom, c, c + natom,
+                     c + 2 * natom, (const int *)doff, (const double *)dtab, (const double *)dtab + nfn, npoint, p,
+                     p + npoint, p + 2 * npoint, (double *)dout);
+  HP_LAUNCHED();
+  HP_HIP(hipMemcpyAsync(out, dout, sizeof(double) * npoint, hipMemcpyDeviceToHost, g->stream));
+  HP_HIP(hipStreamSynchronize(g->stream));
+  return 0;
 }
 
 int hp_stream_copy(int device, int64_t nbytes, int reps, double *gbps, double *ms_per_copy) {

@@ -501,8 +501,10 @@ int upload(hp_batch *b, Request *r) {
 double weights_cost(const Request *r) { return (double)r->N * (double)r->F * (r->stage == HP_RUN_FP64 ? 2.3 : 1.0); }
 
 // Launches of one group of molecules; their work entries go to h_work[w...].
-void plan_group(hp_batch *b, const std::vector<int> &active, std::vector<Launch> &plan, int64_t &w) {
+// Returns false, with nothing written past work_cap, if the entries do not fit.
+bool plan_group(hp_batch *b, const std::vector<int> &active, std::vector<Launch> &plan, int64_t &w) {
   plan.clear();
+  auto room = [&](int64_t n) { return w + n <= b->work_cap; };
   // weights launches: one per (precision, function kind, workgroup shape),
   // the FP64 ones first: an FP32-stage molecule with the defect correction
   // is in both, and on its refresh iterations its FP32 pass reads the
@@ -526,6 +528,7 @@ void plan_group(hp_batch *b, const std::vector<int> &active, std::vector<Launch>
     int kmax = 1, amax = 1;
     for (int sl : g.second) {
       const Request *r = b->slot_req[sl];
+      if (!room((int64_t)r->grp_first.size())) return false;
       for (int s : r->grp_first) b->h_work[w++] = make_int2(r->s0 + s, sl);
       kmax = std::max(kmax, r->maxK);
       amax = std::max(amax, r->A);
@@ -548,6 +551,7 @@ void plan_group(hp_batch *b, const std::vector<int> &active, std::vector<Launch>
     for (int sl : active) {
       const Request *r = b->slot_req[sl];
       if (r->scheme01 != 0 || hp::mbis_ppl(r->maxR) != ppl) continue;
+      if (!room(r->A)) return false;
       for (int a = 0; a < r->A; ++a) b->h_work[w++] = make_int2(r->a0 + a, sl);
       kmax = std::max(kmax, r->maxK);
       rmax = std::max(rmax, r->maxR);
@@ -565,11 +569,13 @@ void plan_group(hp_batch *b, const std::vector<int> &active, std::vector<Launch>
     for (int sl : active) {
       const Request *r = b->slot_req[sl];
       if (r->scheme01 != 1 || hp::pick_ppl(r->maxR) != ppl) continue;
+      if (!room(r->A)) return false;
       for (int a = 0; a < r->A; ++a) b->h_work[w++] = make_int2(r->a0 + a, sl);
     }
     L.len = w - L.off;
     if (L.len) plan.push_back(L);
   }
+  return true;
 }
 
 // Split the molecules in flight into cfg.groups groups of about equal weights
@@ -599,10 +605,9 @@ int build_plan(hp_batch *b) {
     b->slot_off[g] = ns;
     b->nslot[g] = (int)grp[g].size();
     for (int sl : grp[g]) b->h_slot_list[ns++] = sl;
-    plan_group(b, grp[g], b->plan[g], w);
+    if (!plan_group(b, grp[g], b->plan[g], w)) return fail(HP_E_ARG, "work list overflow");
   }
   b->nslot_list = ns;
-  if (w > b->work_cap) return fail(HP_E_ARG, "work list overflow");
   if (w) HP_HIP(hipMemcpyAsync(b->d_work, b->h_work, w * sizeof(int2), hipMemcpyHostToDevice, b->cs));
   if (ns) HP_HIP(hipMemcpyAsync(b->d_slot_list, b->h_slot_list, ns * sizeof(int), hipMemcpyHostToDevice, b->cs));
   return 0;
@@ -860,7 +865,10 @@ int hp_batch_create(int device, const hp_batch_config *cfg, hp_batch **out) {
   const int64_t NP = cfg->point_capacity, NA = cfg->atom_capacity, NS = cfg->shell_capacity,
                 NF = cfg->function_capacity, NQ = std::max<int64_t>(1, cfg->qp_capacity), MI = cfg->max_inflight,
                 MP = b->cfg.max_molecule_points;
-  b->work_cap = NS + NA;
+  // work entries: per molecule its weights workgroups (at most one per radial
+  // shell, twice for an FP32-stage molecule with the FP64 correction, which
+  // is in both weights launches) and one per atom for its solver
+  b->work_cap = 2 * NS + NA;
   int rc = 0;
   if ((rc = dalloc(&b->px, NP)) || (rc = dalloc(&b->py, NP)) || (rc = dalloc(&b->pz, NP)) ||
       (rc = dalloc(&b->rho, NP)) || (rc = dalloc(&b->wmol, NP)) || (rc = dalloc(&b->wsph, NP)) ||

@@ -1752,6 +1752,61 @@ static __global__ __launch_bounds__(256) void k_finalize(int *__restrict__ state
 // moments (lmax <= 3): one workgroup per radial shell, 40 partial sums
 // ---------------------------------------------------------------------------
 #define HP_NMOM 40
+#define HP_MOM_CHUNK 4096  // molecular-grid moments: points per workgroup
+
+// Adds wf times the 40 moment integrands of the offset (x, y, z) from the
+// centre: 20 cartesian powers (HORTON order), 16 real regular solid harmonics
+// (Racah normalisation; c_l0, c_l1, s_l1, ...), radial powers r^0..r^3.
+__device__ __forceinline__ void add_moment_terms(double wf, double x, double y, double z, double *acc) {
+  const double xx = x * x, yy = y * y, zz = z * z, r2 = xx + yy + zz;
+  const double c[20] = {1.0,        x,          y,          z,          xx,         x * y,      x * z,
+                        yy,         y * z,      zz,         xx * x,     xx * y,     xx * z,     x * yy,
+                        x * y * z,  x * zz,     yy * y,     yy * z,     y * zz,     zz * z};
+  for (int i = 0; i < 20; ++i) acc[i] += wf * c[i];
+  const double s3 = 1.7320508075688772, s15 = 3.872983346207417;
+  const double s38 = 0.6123724356957945, s58 = 0.7905694150420949;
+  const double pu[16] = {1.0,
+                         z,
+                         x,
+                         y,
+                         0.5 * (3.0 * zz - r2),
+                         s3 * x * z,
+                         s3 * y * z,
+                         0.5 * s3 * (xx - yy),
+                         s3 * x * y,
+                         0.5 * z * (5.0 * zz - 3.0 * r2),
+                         s38 * x * (5.0 * zz - r2),
+                         s38 * y * (5.0 * zz - r2),
+                         0.5 * s15 * z * (xx - yy),
+                         s15 * x * y * z,
+                         s58 * x * (xx - 3.0 * yy),
+                         s58 * y * (3.0 * xx - yy)};
+  for (int i = 0; i < 16; ++i) acc[20 + i] += wf * pu[i];
+  const double r = sqrt(r2);
+  acc[36] += wf;
+  acc[37] += wf * r;
+  acc[38] += wf * r2;
+  acc[39] += wf * r2 * r;
+}
+
+// Sums the 40 accumulators of a 256-thread workgroup (waves in order) into out.
+__device__ __forceinline__ void block_moments_store(double *acc, double *lds, double *__restrict__ out) {
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6, nw = blockDim.x >> 6;
+  for (int i = 0; i < HP_NMOM; ++i) {
+    double v = acc[i];
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_down(v, o, 64);
+    if (lane == 0) lds[i * 4 + wv] = v;
+  }
+  __syncthreads();
+  if (threadIdx.x < HP_NMOM) {
+    double v = 0.0;
+    for (int w = 0; w < nw; ++w) v += lds[threadIdx.x * 4 + w];
+    out[threadIdx.x] = v;
+  }
+}
+
+// moments (lmax <= 3) on atom grids: one workgroup per radial shell, the
+// shell atom's AIM density with the atom-grid weights
 template <int FK>
 __global__ __launch_bounds__(256) void k_moments(DevMol m, DevFn fn, double *__restrict__ part) {
   __shared__ double lds[HP_NMOM * 4];
@@ -1766,53 +1821,54 @@ __global__ __launch_bounds__(256) void k_moments(DevMol m, DevFn fn, double *__r
     double promol, own;
     promol_at<double, FK>(m, f2, a, m.natom, nullptr, m.px[p], m.py[p], m.pz[p], promol, own);
     const double w = aim_weight(own, promol);
-    const double aim = m.rho[p] * w;
-    const double wf = m.wat[p] * aim;
+    const double wf = m.wat[p] * (m.rho[p] * w);
+    add_moment_terms(wf, m.px[p] - m.cx[a], m.py[p] - m.cy[a], m.pz[p] - m.cz[a], acc);
+  }
+  block_moments_store(acc, lds, part + (int64_t)s * HP_NMOM);
+}
+
+// moments on the molecular grid (grid_type 3): workgroup (chunk, atom) over
+// HP_MOM_CHUNK points, the atom's AIM density from the stored promolecule
+// with the molecular weights
+template <int FK>
+__global__ __launch_bounds__(256) void k_moments_mol(DevMol m, DevFn fn, const double *__restrict__ promol,
+                                                     double *__restrict__ part) {
+  __shared__ double lds[HP_NMOM * 4];
+  const int a = blockIdx.y;
+  const int64_t p0 = (int64_t)blockIdx.x * HP_MOM_CHUNK;
+  const int64_t p1 = p0 + HP_MOM_CHUNK < m.npoint ? p0 + HP_MOM_CHUNK : m.npoint;
+  double acc[HP_NMOM];
+  for (int i = 0; i < HP_NMOM; ++i) acc[i] = 0.0;
+  const bool cull = fn.rc > 0.0;
+  for (int64_t p = p0 + threadIdx.x; p < p1; p += blockDim.x) {
     const double x = m.px[p] - m.cx[a], y = m.py[p] - m.cy[a], z = m.pz[p] - m.cz[a];
-    const double xx = x * x, yy = y * y, zz = z * z, r2 = xx + yy + zz;
-    // cartesian, HORTON order
-    const double c[20] = {1.0,        x,          y,          z,          xx,         x * y,      x * z,
-                          yy,         y * z,      zz,         xx * x,     xx * y,     xx * z,     x * yy,
-                          x * y * z,  x * zz,     yy * y,     yy * z,     y * zz,     zz * z};
-    for (int i = 0; i < 20; ++i) acc[i] += wf * c[i];
-    // pure: real regular solid harmonics (Racah), c_l0, c_l1, s_l1, ...
-    const double s3 = 1.7320508075688772, s15 = 3.872983346207417;
-    const double s38 = 0.6123724356957945, s58 = 0.7905694150420949;
-    const double pu[16] = {1.0,
-                           z,
-                           x,
-                           y,
-                           0.5 * (3.0 * zz - r2),
-                           s3 * x * z,
-                           s3 * y * z,
-                           0.5 * s3 * (xx - yy),
-                           s3 * x * y,
-                           0.5 * z * (5.0 * zz - 3.0 * r2),
-                           s38 * x * (5.0 * zz - r2),
-                           s38 * y * (5.0 * zz - r2),
-                           0.5 * s15 * z * (xx - yy),
-                           s15 * x * y * z,
-                           s58 * x * (xx - 3.0 * yy),
-                           s58 * y * (3.0 * xx - yy)};
-    for (int i = 0; i < 16; ++i) acc[20 + i] += wf * pu[i];
+    const double r2 = x * x + y * y + z * z;
     const double r = sqrt(r2);
-    acc[36] += wf;
-    acc[37] += wf * r;
-    acc[38] += wf * r2;
-    acc[39] += wf * r2 * r;
+    const double own = (!cull || r <= fn.rc) ? proatom_value<double, FK>(fn, a, r, r2) : 0.0;
+    const double wf = m.wmol[p] * (m.rho[p] * aim_weight(own, promol[p]));
+    add_moment_terms(wf, x, y, z, acc);
   }
-  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6, nw = blockDim.x >> 6;
-  for (int i = 0; i < HP_NMOM; ++i) {
-    double v = acc[i];
-    for (int o = 32; o > 0; o >>= 1) v += __shfl_down(v, o, 64);
-    if (lane == 0) lds[i * 4 + wv] = v;
-  }
-  __syncthreads();
-  if (threadIdx.x < HP_NMOM) {
-    double v = 0.0;
-    for (int w = 0; w < nw; ++w) v += lds[threadIdx.x * 4 + w];
-    part[(int64_t)s * HP_NMOM + threadIdx.x] = v;
-  }
+  block_moments_store(acc, lds, part + ((int64_t)a * gridDim.x + blockIdx.x) * HP_NMOM);
+}
+
+static __global__ void k_moments_reduce_mol(int nchunk, const double *__restrict__ part, double *__restrict__ out) {
+  const int a = blockIdx.x;
+  const int i = threadIdx.x;
+  if (i >= HP_NMOM) return;
+  double v = 0.0;
+  for (int c = 0; c < nchunk; ++c) v += part[((int64_t)a * nchunk + c) * HP_NMOM + i];
+  out[(int64_t)a * HP_NMOM + i] = v;
+}
+
+// Sum of the per-shell partials (one workgroup of 256 threads): the entropy
+// of a weights pass, in the order k_finalize sums it.
+static __global__ __launch_bounds__(256) void k_sum_shells(int nshell, const double *__restrict__ part,
+                                                           double *__restrict__ out) {
+  __shared__ double lds[32];
+  double e = 0.0, dummy = 0.0;
+  for (int s = threadIdx.x; s < nshell; s += blockDim.x) e += part[s];
+  block_sum2(e, dummy, lds);
+  if (threadIdx.x == 0) *out = e;
 }
 
 static __global__ void k_moments_reduce(DevMol m, const double *__restrict__ part, double *__restrict__ out) {

@@ -44,6 +44,7 @@ EXPORTED_SYMBOLS = [
     "hp_part_create",
     "hp_part_destroy",
     "hp_part_set_propars",
+    "hp_part_weights",
     "hp_part_run",
     "hp_part_results",
     "hp_part_history",
@@ -226,6 +227,7 @@ def load():
     lib.hp_part_create.argtypes = [ctypes.c_int, _P(MoleculeDesc), _P(SchemeDesc), _P(ctypes.c_void_p)]
     lib.hp_part_destroy.argtypes = [ctypes.c_void_p]
     lib.hp_part_set_propars.argtypes = [ctypes.c_void_p, _pd, ctypes.c_int64]
+    lib.hp_part_weights.argtypes = [ctypes.c_void_p, _pd, _pd]
     lib.hp_part_run.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, _pi32]
     lib.hp_part_results.argtypes = [ctypes.c_void_p, _pd, _pd, _pd, _pd]
     lib.hp_part_history.argtypes = [ctypes.c_void_p, _pd, _pd, _pd, _pd]
@@ -349,6 +351,14 @@ class DevicePartition:
     def set_propars(self, propars):
         propars = np.ascontiguousarray(propars, dtype=float)
         check(self.lib.hp_part_set_propars(self.handle, ptr(propars), propars.size))
+
+    def weights(self, spherical=True):
+        """One FP64 weights pass at the current parameters (``hp_part_weights``):
+        (spherical averages per radial shell or None, entropy)."""
+        rhobar = np.empty(self.nshell) if spherical else None
+        entropy = ctypes.c_double()
+        check(self.lib.hp_part_weights(self.handle, ptr(rhobar) if spherical else None, ctypes.byref(entropy)))
+        return rhobar, entropy.value
 
     def run(self, precision="fp64", n_polish=1):
         n = ctypes.c_int32(0)

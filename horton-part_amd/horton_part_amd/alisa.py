@@ -4,11 +4,12 @@ API of ``alisa.py`` in the reference: ``setup_bs_helper`` (1130-1152) and
 ``LinearISAWPart`` (1155-1335).  Device solvers: ``"sc"`` (solver_sc,
 193-291), ``"newton"`` (exact Newton, 543-610) and ``"m-newton"``
 (line-searched Newton, 613-680), all through ``_solver_general_newton``
-(753-938) semantics.  The reference's API default ``"cvxopt"`` is an
-interior-point method from a third-party package that is not restated here;
-this class defaults to ``"sc"`` (the reference CLI default,
-``data/part-dens.yaml:9``) and raises ``NotImplementedError`` for the other
-host-only solvers.
+(753-938) semantics.  A solver callable (1323-1335) runs in the caller on
+each atom's problem, with the weights pass on the device.  The reference's
+API default ``"cvxopt"`` (1219) is an interior-point method of a third-party
+package that is not restated here: it stays the default, so that a caller
+who omits ``solver=`` gets a ``NotImplementedError`` that says so instead of
+another algorithm's numbers.
 """
 
 import numpy as np
@@ -58,7 +59,7 @@ class LinearISAWPart(GaussianISAWPart):
         maxiter=500,
         inner_threshold=1e-8,
         radius_cutoff=np.inf,
-        solver="sc",
+        solver="cvxopt",
         solver_options=None,
         basis_func="gauss",
         basis_type="analytic",
@@ -90,15 +91,39 @@ class LinearISAWPart(GaussianISAWPart):
     def bs_helper(self):
         return setup_bs_helper(self)
 
+    def _solve_atom_on_host(self, iatom, rho, points, weights, propars):
+        """LinearISAWPart._opt_propars with a callable (alisa.py:1304-1335)."""
+        return self._solver(
+            self._host_bs_funcs(iatom),
+            rho,
+            propars,
+            points,
+            weights,
+            self._inner_threshold,
+            self.logger,
+            density_cutoff=self.density_cutoff,
+            negative_cutoff=self.negative_cutoff,
+            population_cutoff=self.population_cutoff,
+            **self._solver_options,
+        )
+
     def _solver_code(self):
         if callable(self._solver):
-            raise NotImplementedError("custom solver callables run on the host; use one of %s" % sorted(DEVICE_SOLVERS))
+            return 2
+        if self._solver == "cvxopt":
+            raise NotImplementedError(
+                "solver 'cvxopt' (the reference's default, an interior-point method of the cvxopt package) is not "
+                f"provided: pass solver= one of {sorted(DEVICE_SOLVERS)} for the device solvers, or a callable"
+            )
         if self._solver not in DEVICE_SOLVERS:
             raise NotImplementedError(f"solver {self._solver!r} is not on the device; use one of {sorted(DEVICE_SOLVERS)}")
         return DEVICE_SOLVERS[self._solver]
 
     def _scheme_arrays(self):
         d = super()._scheme_arrays()
+        if callable(self._solver):
+            d["inner_maxiter"] = 1
+            return d
         opts = dict(self._solver_options)
         solver = d["solver"]
         if solver == 2:

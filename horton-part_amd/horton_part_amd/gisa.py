@@ -108,9 +108,33 @@ class GaussianISAWPart(AbstractISAWPart):
     def _init_propars(self):
         return init_propars(self)
 
+    def _host_solver(self):
+        """A solver callable (gisa.py:321-331) runs in the caller; the device
+        supplies each outer iteration's weights pass."""
+        return self._solver if callable(self._solver) else None
+
+    def _host_bs_funcs(self, iatom):
+        """``bs_funcs_{iatom}`` (gisa.py:91-106): the atom's shells at unit
+        population on its radial grid (grid_type 1) or molecular-grid distances."""
+        key = f"bs_funcs_{iatom}"
+        if key not in self.cache:
+            number = self.numbers[iatom]
+            r = self._host_basis_radii(iatom)
+            rows = [self.bs_helper.compute_proshell_dens(number, k, 1.0, r) for k in range(self.bs_helper.get_nshell(number))]
+            self.cache.dump(key, np.array(rows))
+        return self.cache.load(key)
+
+    def _solve_atom_on_host(self, iatom, rho, points, weights, propars):
+        """GaussianISAWPart._opt_propars with a callable (gisa.py:320-331)."""
+        helper = self.bs_helper
+        alphas = helper.get_exponent(self.numbers[iatom]) if isinstance(helper, ExpBasisFuncHelper) else None
+        return self._solver(self._host_bs_funcs(iatom), rho, propars, points, weights, alphas, self._inner_threshold, **self._solver_options)
+
     def _solver_code(self):
         if callable(self._solver):
-            raise NotImplementedError("custom solver callables run on the host; the device path solves the QP itself")
+            # the device runs only the weights passes; a linear-coefficient
+            # scheme sums every atom (a callable may return any sign)
+            return 2
         if self._solver not in ("quadprog",):
             raise NotImplementedError(f"solver {self._solver!r}: the device path solves the QP (quadprog's optimum)")
         return 1
@@ -125,5 +149,7 @@ class GaussianISAWPart(AbstractISAWPart):
                 raise NotImplementedError("the GISA QP uses Gaussian overlaps and needs n = 2 functions")
             blocks = [gaussian_overlap_matrix(alpha[fn_off[a] : fn_off[a + 1]]).ravel() for a in range(self.natom)]
             d["qp_matrix"] = np.ascontiguousarray(np.concatenate(blocks))
+            d["inner_maxiter"] = 1
+        if callable(self._solver):
             d["inner_maxiter"] = 1
         return d

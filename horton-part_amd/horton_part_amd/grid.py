@@ -200,17 +200,26 @@ class AtomGrid:
         self._degrees = np.array([LEBEDEV_NPOINTS[int(s)] for s in self._sizes])
         self._center = np.zeros(3) if center is None else np.asarray(center, dtype=float)
         self._indices = np.concatenate([[0], np.cumsum(self._sizes)])
-        r = rgrid.points
-        w = rgrid.weights
-        pts, wts, sph = [], [], []
-        for i, n in enumerate(self._sizes):
-            u, wl = lebedev_rule(int(n))
-            pts.append(self._center + r[i] * u)
-            wts.append(w[i] * r[i] ** 2 * 4.0 * np.pi * wl)
-            sph.append(wl)
-        self._points = np.concatenate(pts)
-        self._weights = np.concatenate(wts)
-        self._sph_weights = np.concatenate(sph)
+        r = np.asarray(rgrid.points, dtype=float)
+        w = np.asarray(rgrid.weights, dtype=float)
+        # per shell: w_i 4 pi r_i^2 (the radial factor of the product weight)
+        shell_factor = w * r**2 * 4.0 * np.pi
+        if (self._sizes == self._sizes[0]).all():
+            # one Lebedev rule for every shell: all shells at once
+            u, wl = lebedev_rule(int(self._sizes[0]))
+            self._points = (self._center + r[:, None, None] * u[None, :, :]).reshape(-1, 3)
+            self._weights = (shell_factor[:, None] * wl[None, :]).ravel()
+            self._sph_weights = np.tile(wl, nrad)
+        else:
+            pts, wts, sph = [], [], []
+            for i, n in enumerate(self._sizes):
+                u, wl = lebedev_rule(int(n))
+                pts.append(self._center + r[i] * u)
+                wts.append(shell_factor[i] * wl)
+                sph.append(wl)
+            self._points = np.concatenate(pts)
+            self._weights = np.concatenate(wts)
+            self._sph_weights = np.concatenate(sph)
 
     @property
     def rgrid(self):

@@ -320,27 +320,28 @@ class AbstractISAWPart(WPart):
     # -- device description ---------------------------------------------------
     def _molecule_arrays(self):
         grid = self.grid
-        if grid.atgrids is None:
-            raise NotImplementedError("the device path needs the atom grids (grid_type 1)")
+        if getattr(grid, "atgrids", None) is None:
+            if not self.only_use_molgrid:
+                raise ValueError("Atomic grids are discarded from molecular grid object, but are needed for local integrations.")
+            return self._molgrid_block_arrays()
         indices = np.asarray(grid.indices, dtype=np.int64)
-        shell_atom, shell_off, shell_r, shell_w, sph, atw = [], [0], [], [], [], []
+        shell_atom, shell_off, shell_r, shell_w, sph, atw = [], [np.zeros(1, np.int64)], [], [], [], []
         for a, atgrid in enumerate(grid.atgrids):
             if atgrid.size != indices[a + 1] - indices[a]:
                 raise ValueError("molecular grid is not the concatenation of its atom grids")
             r = atgrid.rgrid.points
-            w = atgrid.rgrid.weights
             aidx = np.asarray(atgrid.indices, dtype=np.int64)
-            for i in range(len(r)):
-                shell_atom.append(a)
-                shell_off.append(indices[a] + aidx[i + 1])
+            shell_atom.append(np.full(len(r), a, np.int32))
+            shell_off.append(indices[a] + aidx[1:])
             shell_r.append(r)
-            shell_w.append(w)
+            shell_w.append(atgrid.rgrid.weights)
             if hasattr(atgrid, "sph_weights"):
                 sph.append(np.asarray(atgrid.sph_weights, float))
             else:
-                per_shell = 4 * np.pi * r**2 * w
+                per_shell = 4 * np.pi * r**2 * atgrid.rgrid.weights
                 sph.append(atgrid.weights / np.repeat(per_shell, np.diff(aidx)))
             atw.append(np.asarray(atgrid.weights, float))
+        shell_atom = np.concatenate(shell_atom)
         points = np.ascontiguousarray(grid.points, dtype=float)
         return {
             "natom": self.natom,
@@ -354,10 +355,47 @@ class AbstractISAWPart(WPart):
             "atom_weights": np.ascontiguousarray(np.concatenate(atw)),
             "sph_weights": np.ascontiguousarray(np.concatenate(sph)),
             "nshell": len(shell_atom),
-            "shell_atom": np.asarray(shell_atom, dtype=np.int32),
-            "shell_offsets": np.asarray(shell_off, dtype=np.int64),
+            "shell_atom": shell_atom,
+            "shell_offsets": np.concatenate(shell_off),
             "shell_radius": np.ascontiguousarray(np.concatenate(shell_r)),
             "shell_rweight": np.ascontiguousarray(np.concatenate(shell_w)),
+        }
+
+    def _molgrid_block_arrays(self):
+        """Device description of a molecular grid without atom grids
+        (grid_type 3): the points in consecutive blocks of at most 1024, the
+        blocks split among the atoms in order.  On the molecular grid a block
+        plays the part of a radial shell only as a unit of work: the loop sums
+        over all points with the molecular weights, so the split changes no
+        result."""
+        grid = self.grid
+        points = np.ascontiguousarray(grid.points, dtype=float)
+        npoint, natom = points.shape[0], self.natom
+        size = max(256, -(-npoint // (512 * natom)))
+        if size > 1024:
+            raise NotImplementedError("molecular grids of more than 512K points per atom need atom grids on the device")
+        nblock = max(natom, -(-npoint // size))
+        if nblock > npoint:
+            raise ValueError("the molecular grid has fewer points than atoms")
+        offsets = (np.arange(nblock + 1, dtype=np.int64) * npoint) // nblock
+        owner = (np.arange(nblock, dtype=np.int64) * natom) // nblock
+        weights = np.ascontiguousarray(grid.weights, dtype=float)
+        return {
+            "natom": natom,
+            "coordinates": np.ascontiguousarray(self.coordinates, dtype=float),
+            "numbers": np.ascontiguousarray(self.numbers, dtype=np.int64),
+            "pseudo_numbers": np.ascontiguousarray(self.pseudo_numbers, dtype=float),
+            "npoint": int(npoint),
+            "points": points,
+            "density": np.ascontiguousarray(self._moldens, dtype=float),
+            "mol_weights": weights,
+            "atom_weights": weights,
+            "sph_weights": np.zeros(npoint),
+            "nshell": int(nblock),
+            "shell_atom": owner.astype(np.int32),
+            "shell_offsets": offsets,
+            "shell_radius": np.zeros(nblock),
+            "shell_rweight": np.zeros(nblock),
         }
 
     def _scheme_arrays(self):
@@ -387,8 +425,6 @@ class AbstractISAWPart(WPart):
 
     def _ensure_device(self):
         if self._dev is None:
-            if self.grid_type not in (1, 2):
-                raise NotImplementedError("the device path implements grid types 1 and 2")
             self._dev = _native.DevicePartition(self._molecule_arrays(), self._scheme_arrays(), device=self._device)
         return self._dev
 
@@ -420,10 +456,89 @@ class AbstractISAWPart(WPart):
         if rc != 0:
             _native.check(rc)
 
+    # -- per-atom solvers that run in the caller ------------------------------
+    def _host_solver(self):
+        """The caller's per-atom solver (a callable), or None when the device
+        solvers run the whole loop (subclasses)."""
+        return None
+
+    def _solve_atom_on_host(self, iatom, rho, points, weights, propars):
+        """New parameters of atom ``iatom`` from the caller's solver (subclasses)."""
+        raise NotImplementedError
+
+    def _host_basis_radii(self, iatom):
+        """Radii on which atom ``iatom``'s per-atom problem lives: its radial
+        grid (grid_type 1) or its distances to every molecular point."""
+        return self.radial_distances[iatom] if self.on_molgrid else self.get_grid(iatom).rgrid.points
+
+    def _run_with_host_solver(self, dev):
+        """The outer loop of core/iterstock.py:159-193 with the weights pass on
+        the device (``hp_part_weights``) and each atom's update
+        (gisa.py:251-318, alisa.py:1304-1335) by the caller's solver: the
+        device supplies the spherical averages (grid_type 1) or the AIM weights
+        (grid_type 2 and 3) of the pass, the solver returns the parameters."""
+        propars = self._init_propars()
+        self.cache.dump("propars", propars, tags="o")
+        charges = np.zeros(self.natom)
+        self.logger.info("Iteration       Change      Entropy")
+        shell0 = None
+        if not self.on_molgrid:
+            shell0 = np.concatenate([[0], np.cumsum([ag.rgrid.size for ag in self.grid.atgrids])])
+        counter = 0
+        while True:
+            counter += 1
+            old = propars.copy()
+            dev.set_propars(propars)
+            rhobar, entropy = dev.weights(spherical=not self.on_molgrid)
+            for iatom in range(self.natom):
+                sl = slice(self._ranges[iatom], self._ranges[iatom + 1])
+                if self.on_molgrid:
+                    rho = dev.at_weights(iatom, int(self.grid.size)) * self._moldens
+                    points, weights = self.grid.points, self.grid.weights
+                else:
+                    rgrid = self.get_grid(iatom).rgrid
+                    rho = rhobar[shell0[iatom] : shell0[iatom + 1]].copy()
+                    points = rgrid.points
+                    weights = 4 * np.pi * points**2 * rgrid.weights
+                    self.cache.dump(f"radial_points_{iatom}", points, tags="o")
+                    self.cache.dump(f"spherical_average_{iatom}", rho, tags="o")
+                    self.cache.dump(f"radial_weights_{iatom}", rgrid.weights, tags="o")
+                propars[sl] = self._solve_atom_on_host(iatom, rho, points, weights, propars[sl].copy())
+                charges[iatom] = self.pseudo_numbers[iatom] - np.einsum("i,i", weights, rho)
+            self.history_propars.append(propars.copy())
+            self.history_charges.append(charges.copy())
+            self.history_entropies.append(entropy)
+            change = self.compute_change(propars, old)
+            self.history_changes.append(change)
+            self.logger.info("%9i   %10.5e   %10.5e" % (counter, change, entropy))
+            if change < self._threshold or counter >= self._maxiter:
+                break
+        self.cache.dump("charges", charges, tags="o")
+        self.cache.dump("niter", counter, tags="o")
+        self.cache.dump("change", change, tags="o")
+        self._inner_niter = None
+        self._register_lazy(dev, rhobar=rhobar)
+        self._finalize_propars()
+
+    def compute_change(self, propars1, propars2):
+        """Root-mean-square change of the pro-atoms (core/iterstock.py:32-45)."""
+        msd = 0.0
+        for iatom in range(self.natom):
+            delta = self.get_proatom_rho(iatom, propars1)[0] - self.get_proatom_rho(iatom, propars2)[0]
+            if self.on_molgrid:
+                msd += self.grid.integrate(delta, delta)
+            else:
+                rgrid = self.get_grid(iatom).rgrid
+                msd += rgrid.integrate(4 * np.pi * rgrid.points**2, delta, delta)
+        return np.sqrt(msd)
+
     @just_once
     def do_partitioning(self):
         """Outer fixed-point loop (core/iterstock.py:159-193) on the GPU."""
         dev = self._ensure_device()
+        if self._host_solver() is not None:
+            self._run_with_host_solver(dev)
+            return
         propars = self._init_propars()
         dev.set_propars(propars)
         self.logger.info("Iteration       Change      Entropy")
@@ -448,7 +563,7 @@ class AbstractISAWPart(WPart):
         self._register_lazy(dev)
         self._finalize_propars()
 
-    def _register_lazy(self, dev):
+    def _register_lazy(self, dev, rhobar=None):
         grid = self.grid
         for a in range(self.natom):
             size = int(grid.indices[a + 1] - grid.indices[a]) if not self.on_molgrid else int(grid.size)
@@ -456,7 +571,8 @@ class AbstractISAWPart(WPart):
         self.cache.add_lazy("promoldens", dev.promoldens)
         if self.on_molgrid:
             return  # spherical averages exist on atom grids only (mbis.py:176-198)
-        rhobar = dev.spherical_averages()
+        if rhobar is None:
+            rhobar = dev.spherical_averages()
         s = 0
         for a, atgrid in enumerate(grid.atgrids):
             nr = atgrid.rgrid.size

@@ -85,7 +85,8 @@ typedef struct {
 typedef struct {
   int scheme;    /* HP_SCHEME_* */
   int solver;    /* HP_SOLVER_* */
-  int grid_type; /* 1: atom grids (default); 2: molecular grid */
+  int grid_type; /* 1: atom grids (default); 2: molecular grid; 3: molecular
+                    grid only (the loop of 2; moments on the molecular grid) */
   const int32_t *fn_offsets; /* natom+1: functions of atom a are [off[a], off[a+1]) */
   const double *fn_alpha;    /* GISA/LISA: exponent of each function */
   const double *fn_order;    /* GISA/LISA: power n of r in exp(-alpha r^n) */
@@ -148,6 +149,18 @@ int hp_part_destroy(hp_part *part);
  * gisa.py:68-88).  Length: 2*sum(K_a) for MBIS, sum(K_a) otherwise. */
 int hp_part_set_propars(hp_part *part, const double *propars, int64_t n);
 
+/* One update_at_weights pass (core/stockholder.py:352-384) at the current
+ * parameters, in FP64, for partitionings whose per-atom solver runs in the
+ * caller: the solver callables of GaussianISAWPart._opt_propars
+ * (gisa.py:320-341) and LinearISAWPart._opt_propars (alisa.py:1304-1335).
+ * rhobar (grid_type 1, nshell values; may be NULL) receives the spherical
+ * averages of the AIM densities at the radial knots, the
+ * "spherical_average_{a}" entries of gisa.py:281-299; entropy (may be NULL)
+ * the entropy of the promolecule (stockholder.py:145-151).  Afterwards
+ * hp_part_at_weights and hp_part_promoldens return this pass's weights
+ * and promolecule until the next pass or run. */
+int hp_part_weights(hp_part *part, double *rhobar, double *entropy);
+
 /* Run the outer fixed-point loop of AbstractISAWPart.do_partitioning
  * (core/iterstock.py:159-193): per iteration update_at_weights
  * (core/stockholder.py:352-384), the per-atom updates (mbis.py:167-203,
@@ -190,7 +203,10 @@ int hp_part_promoldens(hp_part *part, double *out, int64_t n);
 
 /* AIM multipole moments (core/base.py:329-402) for lmax <= 3 in HORTON order:
  * cartesian natom*20, pure natom*16, radial natom*4 (pseudo_numbers added to
- * the monopoles, electron sign applied as the reference does). */
+ * the monopoles, electron sign applied as the reference does).  Grid types
+ * 1 and 2 integrate each atom's AIM density on its own atom grid with the
+ * atom-grid weights; grid_type 3 on the whole molecular grid with the
+ * molecular weights (local = False, core/base.py:455-459). */
 int hp_part_moments(hp_part *part, int lmax, double *cartesian, double *pure, double *radial);
 
 /* Timing of the pro-atom/weights kernel over the last hp_part_run, measured

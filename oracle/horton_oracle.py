@@ -418,9 +418,10 @@ class OracleWPart:
         self.threshold = threshold
         self.inner_threshold = inner_threshold if inner_threshold < threshold else threshold  # iterstock.py:101
         self.maxiter = maxiter
-        assert grid_type in (1, 2)
+        assert grid_type in (1, 2, 3)
         self.grid_type = grid_type
-        self.on_molgrid = grid_type == 2  # base.py:490-499
+        self.on_molgrid = grid_type in (2, 3)  # base.py:490-499
+        self.only_molgrid = grid_type == 3  # local = False: moments on the molecular grid
         self.density_cutoff = density_cutoff
         self.negative_cutoff = negative_cutoff
         self.population_cutoff = population_cutoff
@@ -562,13 +563,18 @@ class OracleWPart:
         pure = np.zeros((self.natom, (lmax + 1) ** 2))
         rad = np.zeros((self.natom, lmax + 1))
         for i in range(self.natom):
-            atgrid = self.grid.atgrids[i]
             w = self.cache[f"at_weights_{i}"]
-            if w.shape == self.grid.weights.shape and self.on_molgrid:
-                w = self.to_atomic_grid(i, w)
-            aim = self.to_atomic_grid(i, self.moldens) * w
-            d = atgrid.points - self.coordinates[i]
-            wf = atgrid.weights * aim
+            if self.only_molgrid:
+                # local = False (base.py:455-459): the whole molecular grid
+                points, weights, aim = self.grid.points, self.grid.weights, self.moldens * w
+            else:
+                atgrid = self.grid.atgrids[i]
+                if w.shape == self.grid.weights.shape and self.on_molgrid:
+                    w = self.to_atomic_grid(i, w)
+                points, weights = atgrid.points, atgrid.weights
+                aim = self.to_atomic_grid(i, self.moldens) * w
+            d = points - self.coordinates[i]
+            wf = weights * aim
             cart[i] = -np.array([np.sum(wf * d[:, 0] ** a * d[:, 1] ** b * d[:, 2] ** c) for a, b, c in powers])
             cart[i, 0] += self.pseudo_numbers[i]
             pure[i] = -(real_solid_harmonics(lmax, d) @ wf)

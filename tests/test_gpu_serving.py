@@ -202,7 +202,7 @@ def test_batch_rejects_what_it_cannot_run():
     with _server(atom_capacity=6) as srv:
         # LISA is not a batch scheme
         w = _workload("water")
-        lisa = LinearISAWPart(w.coordinates, w.numbers, w.pseudo_numbers, w.grid, w.density, threshold=1e-6)
+        lisa = LinearISAWPart(w.coordinates, w.numbers, w.pseudo_numbers, w.grid, w.density, threshold=1e-6, solver="sc")
         with pytest.raises(_native.HPError) as e:
             srv.submit(lisa)
         assert e.value.code == _native.HP_E_UNSUPP

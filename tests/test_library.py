@@ -83,8 +83,9 @@ def _desc(**over):
     [
         (dict(shell_atom=np.array([1, 0, 0], np.int32)), _native.HP_E_ARG),
         (dict(shell_offsets=np.array([0, 4, 8, 11], np.int64)), _native.HP_E_ARG),
-        (dict(grid_type=3), _native.HP_E_UNSUPP),
+        (dict(grid_type=4), _native.HP_E_ARG),
         (dict(grid_type=2, check_mono=1), _native.HP_E_UNSUPP),
+        (dict(grid_type=3, check_mono=1), _native.HP_E_UNSUPP),
         (dict(solver=1), _native.HP_E_ARG),
         (dict(fn_offsets=np.array([0, 8, 9], np.int32)), _native.HP_E_UNSUPP),
         (dict(maxiter=0), _native.HP_E_ARG),
