@@ -54,6 +54,7 @@ struct hp_part {
   int maxiter = 500, inner_maxiter = 2000, nlin = 20, check_mono = 0;
   std::vector<int64_t> h_shell_off;
   std::vector<int> h_atom_shell_off;
+  std::vector<double> h_order;  // NLIS: the fixed power n of each function
   // device: molecule
   double *cx = nullptr, *cy = nullptr, *cz = nullptr, *px = nullptr, *py = nullptr, *pz = nullptr;
   double *rho = nullptr, *wmol = nullptr, *wat = nullptr, *wsph = nullptr, *shell_r = nullptr, *shell_w = nullptr,
@@ -140,6 +141,12 @@ struct hp_part {
 };
 
 namespace {
+
+// k_iter_begin's parameter layout: 0 MBIS (N, S), 1 linear coefficients,
+// 2 NLIS/GMBIS (N, S, n)
+int table_kind(const hp_part *p) {
+  return p->scheme == HP_SCHEME_MBIS ? 0 : (p->scheme == HP_SCHEME_NLIS ? 2 : 1);
+}
 
 void free_part(hp_part *p) {
   if (!p) return;
@@ -250,6 +257,11 @@ int launch_inner_ppl(hp_part *p) {
       HP_MBIS_K(7);
 #undef HP_MBIS_K
 #undef HP_MBIS
+  } else if (p->scheme == HP_SCHEME_NLIS) {
+    const dim3 g1(p->natom), b1(((p->maxR + 63) / 64) * 64);
+    hipLaunchKernelGGL(k_nlis_inner, g1, b1, 0, p->stream, p->state, p->mol(), p->rhobar, p->par, p->old_par,
+                       p->fn_off, p->fn_norm, p->inner_threshold, p->inner_maxiter, p->inner_niter, p->status,
+                       p->charges, p->change_part);
   } else if (p->solver == HP_SOLVER_QP) {
     hipLaunchKernelGGL((k_gisa_qp<PPL>), grid, block, 0, p->stream, p->state, p->mol(), p->rhobar, p->par, p->old_par, p->fn_off,
                        p->fn_norm, p->fn_alpha, p->fn_order, p->qpP, p->qp_off, p->pop_cutoff, p->neg_cutoff,
@@ -290,7 +302,7 @@ int enqueue_iteration(hp_part *p, int precision, double thr, int min_stop, int m
                       int rollback) {
   hipStream_t s = p->stream;
   const int n = std::max(p->npar, p->nfn);
-  hipLaunchKernelGGL(k_iter_begin, dim3((n + 255) / 256), dim3(256), 0, s, p->state, p->scheme == HP_SCHEME_MBIS ? 0 : 1,
+  hipLaunchKernelGGL(k_iter_begin, dim3((n + 255) / 256), dim3(256), 0, s, p->state, table_kind(p),
                      p->npar, p->nfn, p->par, p->old_par, p->fn_norm, p->fn_alpha, p->coef, p->alpha);
   HP_LAUNCHED();
   HP_HIP(hipEventRecord(p->slot_ev[slot][0], s));
@@ -373,7 +385,7 @@ namespace {
 int gt2_iteration(hp_part *p, int precision, double thr, int min_stop, int max_iter) {
   hipStream_t s = p->stream;
   const int n = std::max(p->npar, p->nfn);
-  hipLaunchKernelGGL(k_iter_begin, dim3((n + 255) / 256), dim3(256), 0, s, p->state, p->scheme == HP_SCHEME_MBIS ? 0 : 1,
+  hipLaunchKernelGGL(k_iter_begin, dim3((n + 255) / 256), dim3(256), 0, s, p->state, table_kind(p),
                      p->npar, p->nfn, p->par, p->old_par, p->fn_norm, p->fn_alpha, p->coef, p->alpha);
   HP_LAUNCHED();
   HP_HIP(hipEventRecord(p->ev[0], s));
@@ -513,18 +525,22 @@ int hp_part_create(int device, const hp_molecule_desc *md, const hp_scheme_desc 
       !md->atom_weights || !md->sph_weights || !md->shell_atom || !md->shell_offsets || !md->shell_radius ||
       !md->shell_rweight)
     return fail(HP_E_ARG, "molecule description has a NULL array");
-  if (sd->scheme < 0 || sd->scheme > 2) return fail(HP_E_ARG, "unknown scheme");
+  if (sd->scheme < 0 || sd->scheme > 3) return fail(HP_E_ARG, "unknown scheme");
   if (sd->grid_type < 1 || sd->grid_type > 3) return fail(HP_E_ARG, "grid_type must be 1, 2 or 3");
   if (sd->grid_type >= 2 && sd->check_mono) return fail(HP_E_UNSUPP, "check_mono is defined on radial grids (grid_type 1)");
   if (!sd->fn_offsets) return fail(HP_E_ARG, "fn_offsets is NULL");
   const bool mbis = sd->scheme == HP_SCHEME_MBIS;
   if (mbis && sd->solver != HP_SOLVER_MBIS) return fail(HP_E_ARG, "MBIS uses HP_SOLVER_MBIS");
   if (sd->scheme == HP_SCHEME_GISA && sd->solver != HP_SOLVER_QP) return fail(HP_E_ARG, "GISA uses HP_SOLVER_QP");
+  const bool nlis = sd->scheme == HP_SCHEME_NLIS;
+  if (nlis && sd->solver != HP_SOLVER_NLIS) return fail(HP_E_ARG, "NLIS uses HP_SOLVER_NLIS");
+  if (nlis && !sd->fn_order) return fail(HP_E_ARG, "fn_order is NULL");
+  if (nlis && sd->grid_type != 1) return fail(HP_E_UNSUPP, "NLIS/GMBIS on the device runs on atom grids (grid_type 1)");
   if (sd->scheme == HP_SCHEME_LISA &&
       !(sd->solver == HP_SOLVER_SC || sd->solver == HP_SOLVER_NEWTON || sd->solver == HP_SOLVER_MNEWTON))
     return fail(HP_E_UNSUPP, "LISA solver not available on the device");
   if (sd->solver == HP_SOLVER_QP && !sd->qp_matrix) return fail(HP_E_ARG, "qp_matrix is NULL");
-  if (!mbis && (!sd->fn_alpha || !sd->fn_order || !sd->fn_norm)) return fail(HP_E_ARG, "basis arrays are NULL");
+  if (!mbis && !nlis && (!sd->fn_alpha || !sd->fn_order || !sd->fn_norm)) return fail(HP_E_ARG, "basis arrays are NULL");
   if (sd->maxiter < 1 || sd->inner_maxiter < 1) return fail(HP_E_ARG, "iteration caps must be positive");
   if (sd->solver == HP_SOLVER_MNEWTON && sd->newton_linspace < 2) return fail(HP_E_ARG, "linspace size < 2");
 
@@ -561,6 +577,7 @@ int hp_part_create(int device, const hp_molecule_desc *md, const hp_scheme_desc 
     maxK = std::max(maxK, K);
   }
   if (mbis && maxK > 7) return fail(HP_E_UNSUPP, "MBIS supports at most 7 shells per atom");
+  if (nlis && maxK > HP_NLIS_KM) return fail(HP_E_UNSUPP, "NLIS supports at most 8 shells per atom");
   if (sd->grid_type >= 2 && maxK > HP_GT2_KM) return fail(HP_E_UNSUPP, "grid_type 2 supports at most 8 functions per atom");
   if (sd->grid_type >= 2 && sd->solver == HP_SOLVER_MNEWTON && sd->newton_linspace > HP_GT2_NLIN)
     return fail(HP_E_UNSUPP, "grid_type 2 line search supports at most 32 candidates");
@@ -569,7 +586,7 @@ int hp_part_create(int device, const hp_molecule_desc *md, const hp_scheme_desc 
   if (!mbis) {
     bool all1 = true, all2 = true;
     for (int f = 0; f < nfn; ++f) {
-      if (!(sd->fn_order[f] > 0.0) || !(sd->fn_alpha[f] >= 0.0))
+      if (!(sd->fn_order[f] > 0.0) || !(nlis || sd->fn_alpha[f] >= 0.0))
         return fail(HP_E_ARG, "basis orders must be positive and exponents non-negative");
       all1 = all1 && sd->fn_order[f] == 1.0;
       all2 = all2 && sd->fn_order[f] == 2.0;
@@ -583,7 +600,7 @@ int hp_part_create(int device, const hp_molecule_desc *md, const hp_scheme_desc 
   p->nshell = S;
   p->npoint = N;
   p->nfn = nfn;
-  p->npar = mbis ? 2 * nfn : nfn;
+  p->npar = mbis ? 2 * nfn : (nlis ? 3 * nfn : nfn);
   p->scheme = sd->scheme;
   p->solver = sd->solver;
   // grid_type 3 runs the molecular-grid loop of grid_type 2 (core/base.py:490-499:
@@ -663,7 +680,14 @@ int hp_part_create(int device, const hp_molecule_desc *md, const hp_scheme_desc 
   const std::vector<int> grp_first = hp::weights_groups(atom_shell_off, p->wshape.g);
   p->ngroup = (int)grp_first.size();
   if ((rc = dupload(&p->grp_first, grp_first.data(), grp_first.size(), s))) return bail(rc);
-  if (!mbis) {
+  if (nlis) {
+    // n / (4 pi Gamma(3/n)) per function (n fixed: the solver updates N and S only)
+    std::vector<double> g(nfn);
+    for (int f = 0; f < nfn; ++f) g[f] = sd->fn_order[f] / (4.0 * M_PI * std::tgamma(3.0 / sd->fn_order[f]));
+    p->h_order.assign(sd->fn_order, sd->fn_order + nfn);
+    if ((rc = dupload(&p->fn_norm, g.data(), nfn, s)) || (rc = dupload(&p->fn_order, sd->fn_order, nfn, s)))
+      return bail(rc);
+  } else if (!mbis) {
     if ((rc = dupload(&p->fn_norm, sd->fn_norm, nfn, s)) || (rc = dupload(&p->fn_alpha, sd->fn_alpha, nfn, s)) ||
         (rc = dupload(&p->fn_order, sd->fn_order, nfn, s)))
       return bail(rc);
@@ -773,6 +797,9 @@ int hp_part_destroy(hp_part *part) {
 int hp_part_set_propars(hp_part *p, const double *propars, int64_t n) {
   if (!p || !propars) return fail(HP_E_ARG, "NULL argument");
   if (n != p->npar) return fail(HP_E_ARG, "propars length mismatch");
+  if (p->scheme == HP_SCHEME_NLIS)
+    for (int f = 0; f < p->nfn; ++f)
+      if (propars[3 * f + 2] != p->h_order[f]) return fail(HP_E_ARG, "NLIS powers n differ from the scheme's fn_order");
   bool nonneg = true;
   for (int64_t k = 0; k < n; ++k) nonneg = nonneg && propars[k] >= 0.0;
   p->params_nonneg = nonneg ? 1 : 0;
@@ -793,7 +820,7 @@ int hp_part_weights(hp_part *p, double *rhobar, double *entropy) {
   // the kernels skip their work while the device stop flag is set
   HP_HIP(hipMemsetAsync(p->state, 0, sizeof(int), s));
   const int n = std::max(p->npar, p->nfn);
-  hipLaunchKernelGGL(k_iter_begin, dim3((n + 255) / 256), dim3(256), 0, s, p->state, p->scheme == HP_SCHEME_MBIS ? 0 : 1,
+  hipLaunchKernelGGL(k_iter_begin, dim3((n + 255) / 256), dim3(256), 0, s, p->state, table_kind(p),
                      p->npar, p->nfn, p->par, p->old_par, p->fn_norm, p->fn_alpha, p->coef, p->alpha);
   HP_LAUNCHED();
   HP_HIP(hipEventRecord(p->ev[0], s));

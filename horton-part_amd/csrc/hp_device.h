@@ -319,6 +319,11 @@ static __global__ void k_iter_begin(const int *__restrict__ done, int scheme, in
       const double N = par[2 * i], S = par[2 * i + 1];
       coef[i] = N * (S * S * S) / (8.0 * HP_PI);
       alpha[i] = S;
+    } else if (scheme == 2) {
+      // NLIS/GMBIS (N, S, n): N n S^(3/n) / (4 pi Gamma(3/n)), norm = n / (4 pi Gamma(3/n))
+      const double N = par[3 * i], S = par[3 * i + 1], n = par[3 * i + 2];
+      coef[i] = N * norm[i] * pow(S, 3.0 / n);
+      alpha[i] = S;
     } else {
       coef[i] = par[i] * norm[i];
       alpha[i] = alpha_fixed[i];
@@ -1209,6 +1214,145 @@ __global__ __launch_bounds__(512) void k_mbis_inner(const int *__restrict__ done
   slot ^= 1;
   c.pop = pv;
   mbis_dispatch<KM, 1, PPL>(K, c, slot);
+}
+
+// ---------------------------------------------------------------------------
+// NLIS / GMBIS inner fixed point (nlis.py:99-195): shells N g(n, S)
+// exp(-S r^n), g(n, S) = n S^(3/n) / (4 pi Gamma(3/n)), parameters (N, S, n)
+// per shell with n fixed.  Per iteration N <- sum w t ratio N and
+// S <- 3 / (n sum w t ratio r^n) (1e-5 when that moment is 0 to 1e-8), t the
+// unit-population shell.  One workgroup per atom, one radial point per lane;
+// gnorm[f] = n / (4 pi Gamma(3/n)) per function (fixed with n).
+// ---------------------------------------------------------------------------
+template <int KC>
+__device__ __forceinline__ void nlis_solve(const DevMol &m, int a, int f0, int s0, int R, double *par,
+                                           const double *old_par, const double *gnorm, const double *rhobar,
+                                           double thr, int maxit, int *inner_niter, int *status, double *charges,
+                                           double *change_part, double *lds) {
+  constexpr int NV = 2 * KC + 1;
+  const int i = threadIdx.x;
+  const bool valid = i < R;
+  const double r = valid ? m.shell_r[s0 + i] : 0.0;
+  const double wr = valid ? m.shell_w[s0 + i] : 0.0;
+  const double w = 4.0 * HP_PI * (r * r) * wr;
+  const double rho = valid ? rhobar[s0 + i] : 0.0;
+  double Np[KC], Sp[KC], np_[KC], rn[KC], g[KC], e3[KC];
+#pragma unroll
+  for (int k = 0; k < KC; ++k) {
+    Np[k] = par[3 * (f0 + k)];
+    Sp[k] = par[3 * (f0 + k) + 1];
+    np_[k] = par[3 * (f0 + k) + 2];
+    rn[k] = pow(r, np_[k]);
+    g[k] = gnorm[f0 + k];
+    e3[k] = 3.0 / np_[k];
+  }
+  int slot = 0;
+  double pv = w * rho;
+  block_allsum<1>(&pv, lds, slot);
+  slot ^= 1;
+  const double pop = pv;
+  const double dc = 1e-15;  // opt_nlis_propars default density_cutoff (nlis.py:100)
+  double oldpro = 0.0;
+  bool have_old = false, conv = false;
+  int it = 0;
+  for (it = 0; it < maxit; ++it) {
+    double red[NV];
+    double t[KC];
+    double pro = 0.0;
+#pragma unroll
+    for (int k = 0; k < KC; ++k) {
+      t[k] = g[k] * pow(Sp[k], e3[k]) * exp(-Sp[k] * rn[k]);
+      pro += t[k] * Np[k];
+    }
+    const bool sick = (rho < dc) || (pro < dc);
+    const double ratio = sick ? 0.0 : rho / pro;
+#pragma unroll
+    for (int k = 0; k < KC; ++k) {
+      const double tr = t[k] * ratio;
+      red[2 * k] = w * (tr * Np[k]);
+      red[2 * k + 1] = w * tr * rn[k];
+    }
+    const double e = oldpro - pro;
+    red[2 * KC] = w * e * e;
+    block_allsum<NV>(red, lds, slot);
+    slot ^= 1;
+#pragma unroll
+    for (int k = 0; k < KC; ++k) {
+      Np[k] = red[2 * k];
+      const double m1 = red[2 * k + 1];
+      // np.isclose(m1, 0.0): |m1| <= 1e-8
+      Sp[k] = fabs(m1) <= 1e-8 ? 1e-5 : 3.0 / (m1 * np_[k]);
+    }
+    if (have_old && sqrt(red[2 * KC]) < thr) {
+      conv = true;
+      break;
+    }
+    oldpro = pro;
+    have_old = true;
+  }
+  // change of the outer loop: new vs old pro-atom on the radial grid
+  double y1 = 0.0, y2 = 0.0;
+#pragma unroll
+  for (int k = 0; k < KC; ++k) {
+    const double S2 = old_par[3 * (f0 + k) + 1];
+    y1 += Np[k] * g[k] * pow(Sp[k], e3[k]) * exp(-Sp[k] * rn[k]);
+    y2 += old_par[3 * (f0 + k)] * g[k] * pow(S2, e3[k]) * exp(-S2 * rn[k]);
+  }
+  const double dy = y1 - y2;
+  double msd = wr * (4.0 * HP_PI * (r * r)) * dy * dy;
+  block_allsum<1>(&msd, lds, slot);
+  if (threadIdx.x == 0) {
+    int st = 0;
+    double sumN = 0.0;
+#pragma unroll
+    for (int k = 0; k < KC; ++k) sumN += Np[k];
+    if (conv) {
+      if (!(fabs(pop - sumN) <= 1e-4 + 1e-5 * fabs(sumN))) st |= HP_ST_POP;
+    } else {
+      st |= HP_ST_NOCONV;
+    }
+#pragma unroll
+    for (int k = 0; k < KC; ++k) {
+      par[3 * (f0 + k)] = Np[k];
+      par[3 * (f0 + k) + 1] = Sp[k];
+    }
+    inner_niter[a] = conv ? it + 1 : maxit;
+    status[a] = st;
+    charges[a] = m.pseudo[a] - pop;
+    change_part[a] = msd;
+  }
+}
+
+template <int KC, int KM>
+__device__ __forceinline__ void nlis_dispatch(int K, const DevMol &m, int a, int f0, int s0, int R, double *par,
+                                              const double *old_par, const double *gnorm, const double *rhobar,
+                                              double thr, int maxit, int *inner_niter, int *status, double *charges,
+                                              double *change_part, double *lds) {
+  if constexpr (KC <= KM) {
+    if (K == KC) {
+      nlis_solve<KC>(m, a, f0, s0, R, par, old_par, gnorm, rhobar, thr, maxit, inner_niter, status, charges,
+                     change_part, lds);
+      return;
+    }
+    nlis_dispatch<KC + 1, KM>(K, m, a, f0, s0, R, par, old_par, gnorm, rhobar, thr, maxit, inner_niter, status,
+                              charges, change_part, lds);
+  }
+}
+
+#define HP_NLIS_KM 8
+static __global__ __launch_bounds__(512) void k_nlis_inner(const int *__restrict__ done, DevMol m,
+                                                    const double *__restrict__ rhobar, double *__restrict__ par,
+                                                    const double *__restrict__ old_par, const int *__restrict__ fn_off,
+                                                    const double *__restrict__ gnorm, double thr, int maxit,
+                                                    int *__restrict__ inner_niter, int *__restrict__ status,
+                                                    double *__restrict__ charges, double *__restrict__ change_part) {
+  __shared__ double lds[2 * 16 * (2 * HP_NLIS_KM + 1)];
+  if (done && *done) return;
+  const int a = blockIdx.x;
+  const int s0 = m.atom_shell_off[a], R = m.atom_shell_off[a + 1] - s0;
+  const int f0 = fn_off[a], K = __builtin_amdgcn_readfirstlane(fn_off[a + 1] - f0);
+  nlis_dispatch<1, HP_NLIS_KM>(K, m, a, f0, s0, R, par, old_par, gnorm, rhobar, thr, maxit, inner_niter, status,
+                               charges, change_part, lds);
 }
 
 // ---------------------------------------------------------------------------

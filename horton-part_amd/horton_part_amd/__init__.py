@@ -1,4 +1,4 @@
-"""MI355X-native stockholder partitioning (MBIS / GISA / aLISA).
+"""MI355X-native stockholder partitioning (MBIS / GISA / aLISA / NLIS / GMBIS).
 
 Drop-in for the grid-based stockholder loop of horton-part: the scheme
 classes keep the reference's ``WPart`` API and run the loop in
@@ -8,8 +8,10 @@ classes keep the reference's ``WPart`` API and run the loop in
 from .alisa import LinearISAWPart
 from .basis import ExpBasisFuncHelper, evaluate_function
 from .gisa import GaussianISAWPart
+from .gmbis import GMBISWPart, get_initial_gmbis_propars
 from .grid import AtomGrid, BeckeWeights, ExpRTransform, MolGrid, OneDGrid, UniformInteger
 from .mbis import MBISWPart, get_initial_mbis_propars, get_nshell
+from .nlis import NLISWPart, get_initial_nlis_propars, get_nlis_nshell
 from .serving import PartitionServer
 from .utils import wpart_schemes
 
@@ -17,6 +19,11 @@ __all__ = [
     "MBISWPart",
     "GaussianISAWPart",
     "LinearISAWPart",
+    "NLISWPart",
+    "GMBISWPart",
+    "get_nlis_nshell",
+    "get_initial_nlis_propars",
+    "get_initial_gmbis_propars",
     "PartitionServer",
     "wpart_schemes",
     "get_nshell",

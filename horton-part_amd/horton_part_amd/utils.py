@@ -42,8 +42,8 @@ radius_becke = {k: (None if v is None else v * ANGSTROM) for k, v in enumerate(_
 def wpart_schemes(scheme):
     """Return the partitioning class registered under ``scheme``.
 
-    Only the stockholder schemes of the accelerated path are provided:
-    ``"mbis"``, ``"gisa"`` and ``"lisa"``.
+    The stockholder schemes of the accelerated path are provided:
+    ``"mbis"``, ``"gisa"``, ``"lisa"``, ``"nlis"`` and ``"gmbis"``.
     """
     if scheme == "mbis":
         from .mbis import MBISWPart
@@ -57,6 +57,14 @@ def wpart_schemes(scheme):
         from .alisa import LinearISAWPart
 
         return LinearISAWPart
+    if scheme == "nlis":
+        from .nlis import NLISWPart
+
+        return NLISWPart
+    if scheme == "gmbis":
+        from .gmbis import GMBISWPart
+
+        return GMBISWPart
     raise NotImplementedError(f"scheme {scheme!r} is not part of this package")
 
 

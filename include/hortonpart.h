@@ -36,6 +36,7 @@ extern "C" {
 #define HP_SCHEME_MBIS 0 /* mbis.py:206-362 */
 #define HP_SCHEME_GISA 1 /* gisa.py:109-345 */
 #define HP_SCHEME_LISA 2 /* alisa.py:1155-1335 */
+#define HP_SCHEME_NLIS 3 /* nlis.py:201-367 and gmbis.py:75-156: (N, S, n) per shell */
 
 /* inner solvers */
 #define HP_SOLVER_MBIS 0     /* mbis.py:81-163 opt_mbis_propars */
@@ -43,6 +44,7 @@ extern "C" {
 #define HP_SOLVER_SC 2       /* alisa.py:193-291 solver_sc */
 #define HP_SOLVER_NEWTON 3   /* alisa.py:543-610 solver_newton (exact) */
 #define HP_SOLVER_MNEWTON 4  /* alisa.py:613-680 solver_m_newton (line search) */
+#define HP_SOLVER_NLIS 5     /* nlis.py:99-195 opt_nlis_propars */
 
 /* arithmetic of the grid passes */
 #define HP_PREC_FP64 0  /* everything in FP64 */
@@ -89,7 +91,8 @@ typedef struct {
                     grid only (the loop of 2; moments on the molecular grid) */
   const int32_t *fn_offsets; /* natom+1: functions of atom a are [off[a], off[a+1]) */
   const double *fn_alpha;    /* GISA/LISA: exponent of each function */
-  const double *fn_order;    /* GISA/LISA: power n of r in exp(-alpha r^n) */
+  const double *fn_order;    /* GISA/LISA/NLIS: power n of r in exp(-alpha r^n)
+                                (NLIS: the n of the parameters, fixed) */
   const double *fn_norm;     /* GISA/LISA: n alpha^(3/n) / (4 pi Gamma(3/n)) */
   const double *qp_matrix;   /* GISA: concatenated K_a x K_a blocks of P */
   double threshold;          /* outer threshold (iterstock.py:99) */
@@ -146,7 +149,9 @@ int hp_part_create(int device, const hp_molecule_desc *mol, const hp_scheme_desc
 int hp_part_destroy(hp_part *part);
 
 /* Set the current pro-atom parameters (initial values: mbis.py:291-305,
- * gisa.py:68-88).  Length: 2*sum(K_a) for MBIS, sum(K_a) otherwise. */
+ * gisa.py:68-88, nlis.py:330-346, gmbis.py:138-152).  Length: 2*sum(K_a)
+ * for MBIS, 3*sum(K_a) for NLIS (whose n must equal fn_order), sum(K_a)
+ * otherwise. */
 int hp_part_set_propars(hp_part *part, const double *propars, int64_t n);
 
 /* One update_at_weights pass (core/stockholder.py:352-384) at the current

@@ -739,3 +739,149 @@ class OracleLISA(OracleGISA):
             raise NotImplementedError(self.solver)
         self.inner_stats[-1].append(stats)
         return out
+
+
+# ----------------------------------------------------------------------------
+# NLIS / GMBIS: nlis.py:44-195, 201-367; gmbis.py:36-156
+# ----------------------------------------------------------------------------
+
+
+def nlis_shell_norm(n, S):
+    """n S^(3/n) / (4 pi Gamma(3/n)): the unit-population shell prefactor (nlis.py:157)."""
+    return n * S ** (3 / n) / (4 * np.pi * math.gamma(3.0 / n))
+
+
+def get_nlis_nshell(number, nshell_dict):
+    """nlis.py:44-56."""
+    return nshell_dict.get(number, get_nshell(number))
+
+
+def get_initial_nlis_propars(number, exp_n_dict, nshell_dict):
+    """nlis.py:59-96: (N, S, n) per shell, N = Z/K, S geometric from 2Z to 0.5."""
+    nbs = get_nlis_nshell(number, nshell_dict)
+    propars = np.ones(3 * nbs, float)
+    S0 = 2.0 * number
+    alpha = (0.5 / S0) ** (1.0 / (nbs - 1)) if nbs > 1 else 1.0
+    for k in range(nbs):
+        propars[3 * k] = number / nbs
+        propars[3 * k + 1] = S0 * alpha**k
+        propars[3 * k + 2] = exp_n_dict[(number, k)] if (number, k) in exp_n_dict else 1.0
+    return propars
+
+
+def get_initial_gmbis_propars(number, exp_n_dict):
+    """gmbis.py:36-72: MBIS shell counts and populations, S geometric from 2Z to 2."""
+    nshell = get_nshell(number)
+    propars = np.zeros(3 * nshell, float)
+    S0 = 2.0 * number
+    alpha = (2.0 / S0) ** (1.0 / (nshell - 1)) if nshell > 1 else 1.0
+    nel = np.array([2.0, 8.0, 8.0, 18.0, 18.0, 32.0, 32.0])
+    for k in range(nshell):
+        propars[3 * k] = nel[k]
+        propars[3 * k + 1] = S0 * alpha**k
+        propars[3 * k + 2] = exp_n_dict[(number, k)] if (number, k) in exp_n_dict else 1.0
+    propars[-3] = number - propars[:-3:3].sum()
+    return propars
+
+
+def opt_nlis_propars(rho, propars, weights, r, threshold, density_cutoff=1e-15, stats=None):
+    """nlis.py:99-195.  Unlike MBIS, the exponent update is 3 / (n m1) with m1
+    the weighted moment of the unit-population shell (no factor N)."""
+    nshell = len(propars) // 3
+    terms = np.zeros((nshell, len(r)), float)
+    oldpro = None
+    pop = np.einsum("p,p->", weights, rho)
+    for irep in range(2000):
+        for k in range(nshell):
+            S, n = propars[3 * k + 1], propars[3 * k + 2]
+            terms[k] = n * S ** (3 / n) * np.exp(-S * r**n) / (4 * np.pi * math.gamma(3.0 / n))
+        pro = np.sum(terms * propars[::3, None], axis=0)
+        sick = (rho < density_cutoff) | (pro < density_cutoff)
+        with np.errstate(all="ignore"):
+            ratio = rho / pro
+        ratio[sick] = 0.0
+        terms_ratio = terms * ratio
+        for k in range(nshell):
+            N, n = propars[3 * k], propars[3 * k + 2]
+            m0 = np.einsum("p,p->", weights, terms_ratio[k] * N)
+            m1 = np.einsum("p,p,p->", weights, terms_ratio[k], r**n)
+            propars[3 * k] = m0
+            propars[3 * k + 1] = 1e-5 if np.isclose(m1, 0.0) else 3 / (m1 * n)
+        if oldpro is None:
+            change = 1e100
+        else:
+            error = oldpro - pro
+            change = np.sqrt(np.einsum("p,p,p->", weights, error, error))
+        if change < threshold:
+            popok = bool(np.isclose(pop, np.sum(propars[0::3]), atol=1e-4))
+            if not popok:
+                warnings.warn("The sum of propars are not equal to the atomic pop.", OracleWarning)
+            if stats is not None:
+                stats.update(niter=irep + 1, converged=True, popok=popok)
+            return propars
+        oldpro = pro
+    warnings.warn("NLIS not converged, but still go ahead!", OracleWarning)
+    if stats is not None:
+        stats.update(niter=2000, converged=False, popok=True)
+    return propars
+
+
+class OracleNLIS(OracleWPart):
+    """nlis.py:201-367 (exp_n_dict, nshell_dict as NLISWPart takes them)."""
+
+    def __init__(self, *args, exp_n_dict=None, nshell_dict=None, **kwargs):
+        self.exp_n_dict = exp_n_dict if exp_n_dict is not None else {}
+        self.nshell_dict = nshell_dict or {}
+        super().__init__(*args, **kwargs)
+
+    def _initial(self, z):
+        return get_initial_nlis_propars(z, self.exp_n_dict, self.nshell_dict)
+
+    def init_propars(self):
+        """nlis.py:330-346."""
+        per_atom = [self._initial(int(z)) for z in self.numbers]
+        self.nshells = [len(p) // 3 for p in per_atom]
+        self.ranges = [0] + list(np.cumsum([len(p) for p in per_atom]))
+        return np.concatenate(per_atom)
+
+    def _rho(self, i, propars, r):
+        my = propars[self.ranges[i] : self.ranges[i + 1]]
+        y = np.zeros(len(r))
+        for k in range(self.nshells[i]):
+            N, S, n = my[3 * k : 3 * k + 3]
+            y += N * n * S ** (3 / n) * np.exp(-S * r**n) / (4 * np.pi * math.gamma(3.0 / n))
+        return y
+
+    def eval_proatom(self, i, propars):
+        """nlis.py:302-328."""
+        return self._rho(i, propars, self.radial_distances[i])
+
+    def get_proatom_rho(self, i, propars):
+        """nlis.py:272-300 (density part)."""
+        r = self.radial_distances[i] if self.on_molgrid else self.rgrid(i).points
+        return self._rho(i, propars, r)
+
+    def update_atom(self, i, propars):
+        """mbis.py:167-203 with opt_nlis_propars (nlis.py:348-349)."""
+        rhoa, weights, r = self.atom_problem(i)
+        my = propars[self.ranges[i] : self.ranges[i + 1]]
+        stats = {}
+        my[:] = opt_nlis_propars(rhoa, my.copy(), weights, r, self.inner_threshold, stats=stats)
+        self.inner_stats[-1].append(stats)
+        self.cache["charges"][i] = self.pseudo_numbers[i] - np.einsum("p,p->", weights, rhoa)
+
+    def finalize(self):
+        """nlis.py:351-367."""
+        propars = self.cache["propars"]
+        vc = np.array([-propars[self.ranges[i + 1] - 3] for i in range(self.natom)])
+        vw = np.array([1.0 / propars[self.ranges[i + 1] - 2] for i in range(self.natom)])
+        self.cache["valence_charges"] = vc
+        self.cache["valence_widths"] = vw
+        self.cache["core_charges"] = self.cache["charges"] - vc
+
+
+class OracleGMBIS(OracleNLIS):
+    """gmbis.py:75-156: NLIS with MBIS shell counts and initial values."""
+
+    def _initial(self, z):
+        return get_initial_gmbis_propars(z, self.exp_n_dict)

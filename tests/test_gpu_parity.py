@@ -383,3 +383,33 @@ def test_tail_skip_with_cutoff_radius():
     cache = oracle.do_partitioning()
     _compare(part, cache, oracle)
     full.close()
+
+
+# --- NLIS / GMBIS (nlis.py, gmbis.py) ----------------------------------------------
+
+
+@pytest.mark.parametrize("scheme", ["nlis", "gmbis"])
+def test_nlis_gmbis_chain(scheme):
+    """NLIS (4 shells per element, some with n = 2) and GMBIS (MBIS shells,
+    n = 1) on a chain against the oracle: shells N g(n, S) exp(-S r^n) in the
+    weights pass, the NLIS fixed point in k_nlis_inner."""
+    require_gpu()
+    from horton_part_amd import GMBISWPart, NLISWPart
+    from oracle.horton_oracle import OracleGMBIS, OracleNLIS
+
+    _say(f"case: {scheme} chain 5 atoms seed 17, 50x110")
+    w = workloads.chain(natom=5, seed=17, nrad=50, nang=110)
+    args = (w.coordinates, w.numbers, w.pseudo_numbers, w.grid, w.density)
+    if scheme == "nlis":
+        nd = {z: 3 for z in (1, 6, 7, 8)}
+        exp_n = {(z, 0): 2.0 for z in (6, 7, 8)}
+        part = NLISWPart(*args, maxiter=60, exp_n_dict=exp_n, nshell_dict=nd)
+        oracle = OracleNLIS(*args, maxiter=60, exp_n_dict=exp_n, nshell_dict=nd)
+    else:
+        part = GMBISWPart(*args, maxiter=60, exp_n_dict={})
+        oracle = OracleGMBIS(*args, maxiter=60, exp_n_dict={})
+    part.do_partitioning()
+    cache = oracle.do_partitioning()
+    _compare(part, cache, oracle, qtol=1e-10, ptol=1e-8)
+    np.testing.assert_allclose(part["valence_charges"], cache["valence_charges"], atol=1e-10)
+    np.testing.assert_allclose(part["core_charges"], cache["core_charges"], atol=1e-10)

@@ -9,6 +9,7 @@ force on tiny inputs (QP enumeration, Hessian double loops), closed forms
 """
 
 import itertools
+import math
 import json
 import os
 import warnings
@@ -415,3 +416,82 @@ def test_oracle_entropy_closed_forms():
     keep = ~cut & (rho0 >= 1e-15)
     expect = np.sum(grid.weights[keep] * rho[keep] * np.log(1.9))
     assert o.compute_entropy() == pytest.approx(expect, rel=1e-12)
+
+
+# --- NLIS / GMBIS ----------------------------------------------------------------
+
+
+@pytest.mark.parametrize("nshell", [4, 5, 6])
+def test_nlis_nshell_kat(nshell):
+    """Reference tests/test_nlis.py:37-48."""
+    from horton_part_amd import get_nlis_nshell
+
+    nd = {n: nshell for n in range(1, 119)}
+    for z in (1, 2, 3, 17, 18, 21, 44, 72, 104):
+        assert ho.get_nlis_nshell(z, nd) == nshell
+        assert get_nlis_nshell(z, nd) == nshell
+    assert ho.get_nlis_nshell(8, {}) == 2
+
+
+def test_nlis_initial_propars_kat():
+    """Reference tests/test_nlis.py:51-70 (hydrogen, 4 shells, n = 2)."""
+    from horton_part_amd import get_initial_nlis_propars
+
+    nd = {n: 4 for n in range(1, 119)}
+    exp_n = {(1, k): 2 for k in range(4)}
+    expect = [0.25, 2.0, 2.0, 0.25, 1.25992105, 2.0, 0.25, 0.79370053, 2.0, 0.25, 0.5, 2.0]
+    assert ho.get_initial_nlis_propars(1, exp_n, nd) == pytest.approx(expect)
+    assert get_initial_nlis_propars(1, exp_n, nd) == pytest.approx(expect)
+
+
+def test_gmbis_initial_propars_match_mbis_populations():
+    """gmbis.py:36-72: the MBIS shell populations and first exponent 2Z, the
+    last exponent 2 (MBIS's geometric progression), n = 1 unless given."""
+    from horton_part_amd import get_initial_gmbis_propars
+
+    for z in (1, 6, 8, 16):
+        g = ho.get_initial_gmbis_propars(z, {})
+        m = ho.get_initial_mbis_propars(z)
+        np.testing.assert_allclose(g[0::3], m[0::2])
+        np.testing.assert_allclose(g[1::3], m[1::2])
+        assert (g[2::3] == 1.0).all()
+        np.testing.assert_array_equal(get_initial_gmbis_propars(z, {}), g)
+    assert ho.get_initial_gmbis_propars(8, {(8, 1): 2.0})[5] == 2.0
+
+
+@pytest.mark.parametrize("n", [1.0, 1.5, 2.0])
+def test_nlis_single_shell_fixed_point(n):
+    """A density that is exactly one normalised shell N g(n, S) exp(-S r^n) is
+    a fixed point: the ratio is 1, N is reproduced by the population and
+    S by 3 / (n <r^n>), because <r^n> = 3 / (n S) for that shell."""
+    rg = exp_radial_grid(400)
+    r = rg.points
+    w = 4 * np.pi * r**2 * rg.weights
+    N, S = 1.7, 2.3
+    rho = N * n * S ** (3 / n) / (4 * np.pi * math.gamma(3 / n)) * np.exp(-S * r**n)
+    out = ho.opt_nlis_propars(rho, np.array([1.0, 1.0, n]), w, r, 1e-12)
+    np.testing.assert_allclose(out, [N, S, n], rtol=1e-7)
+
+
+def test_nlis_first_update_by_hand():
+    """One update of two shells written as explicit loops (nlis.py:140-175):
+    N_k <- sum_p w_p t_kp rho_p / pro_p N_k and S_k <- 3 / (n_k sum_p w_p
+    t_kp rho_p / pro_p r_p^n_k), t the unit-population shell.  A threshold
+    above the first iteration's change (1e100) returns after one update."""
+    r = np.array([0.1, 0.4, 0.9, 1.7, 3.0])
+    w = np.array([0.2, 0.5, 0.7, 0.4, 0.1])
+    rho = np.array([3.0, 1.1, 0.35, 0.06, 0.004])
+    pars = np.array([1.2, 3.0, 1.0, 0.8, 0.9, 2.0])
+    t = []
+    for k in range(2):
+        S, n = pars[3 * k + 1], pars[3 * k + 2]
+        t.append([n * S ** (3 / n) * math.exp(-S * x**n) / (4 * math.pi * math.gamma(3 / n)) for x in r])
+    pro = [pars[0] * t[0][p] + pars[3] * t[1][p] for p in range(5)]
+    expect = []
+    for k in range(2):
+        N, n = pars[3 * k], pars[3 * k + 2]
+        m0 = sum(w[p] * t[k][p] * rho[p] / pro[p] * N for p in range(5))
+        m1 = sum(w[p] * t[k][p] * rho[p] / pro[p] * r[p] ** n for p in range(5))
+        expect += [m0, 3 / (m1 * n), n]
+    got = ho.opt_nlis_propars(rho, pars.copy(), w, r, 1e200)
+    np.testing.assert_allclose(got, expect, rtol=1e-13)
