@@ -562,11 +562,11 @@ def main():
     ap.add_argument("--warmup", type=int, default=None,
                     help="serving: requests of another stream served untimed first; single molecules: untimed partitions")
     ap.add_argument("--requests", type=int, default=4096, help="serving: requests of the stream (4096: the whole stream)")
-    ap.add_argument("--builders", type=int, default=4, help="serving: host threads that build the requests")
+    ap.add_argument("--builders", type=int, default=8, help="serving: host threads that build the requests")
     ap.add_argument("--lookahead", type=int, default=24, help="serving: requests built ahead of submission")
     ap.add_argument("--arena-points", type=int, default=96_000_000, help="serving: grid points resident at once")
     ap.add_argument("--max-inflight", type=int, default=256, help="serving: molecules resident at once")
-    ap.add_argument("--fp64-every", type=int, default=32, help="serving: FP64 one-at-a-time sample: every n-th request")
+    ap.add_argument("--fp64-every", type=int, default=64, help="serving: FP64 one-at-a-time sample: every n-th request")
     ap.add_argument("--workload", default="serving",
                     choices=["serving", "water", "chain20", "chain20-gisa", "cluster200", "protein2000"])
     ap.add_argument("--ticks", type=int, default=8, help="serving: outer iterations per host step")

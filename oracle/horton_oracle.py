@@ -885,3 +885,204 @@ class OracleGMBIS(OracleNLIS):
 
     def _initial(self, z):
         return get_initial_gmbis_propars(z, self.exp_n_dict)
+
+
+# ----------------------------------------------------------------------------
+# gLISA: glisa.py:55-1028 (the global problem over every coefficient on the
+# molecular grid; solvers sc, newton, m-newton)
+# ----------------------------------------------------------------------------
+
+
+def fix_propars(exp_array, propars, delta):
+    """utils.py:556-577: from the most diffuse function on, the indices whose
+    coefficient is below 1e-4 in magnitude and whose step is negative."""
+    fixed = []
+    for index in np.argsort(exp_array):
+        if abs(propars[index]) < 1e-4 and delta[index] < 0.0:
+            fixed.append(index)
+        else:
+            break
+    return fixed
+
+
+class OracleGLISA(OracleWPart):
+    """glisa.py:55-1028.  ``solver`` is "sc", "newton" or "m-newton";
+    ``solver_options`` as the reference's solver methods take them."""
+
+    def __init__(self, *args, basis=None, solver="m-newton", solver_options=None, **kwargs):
+        self.basis = basis
+        self.solver = solver
+        self.solver_options = dict(solver_options or {})
+        super().__init__(*args, **kwargs)
+
+    def init_propars(self):
+        """gisa.py:68-88 and the basis on the molecular grid (glisa.py:250-258)."""
+        self.ranges = [0]
+        for z in self.numbers:
+            self.ranges.append(self.ranges[-1] + self.basis.get_nshell(int(z)))
+        propars = np.ones(self.ranges[-1])
+        for i, z in enumerate(self.numbers):
+            inits = self.basis.initials[int(z)].copy()
+            inits[inits < 1e-4] = 1e-4
+            propars[self.ranges[i] : self.ranges[i + 1]] = inits / np.sum(inits) * self.pseudo_numbers[i]
+        propars[:] = propars / np.sum(propars) * self.nelec
+        rows = []
+        for i, z in enumerate(self.numbers):
+            r = self.radial_distances[i]
+            rows += [self.basis.shell(int(z), k, 1.0, r) for k in range(self.basis.get_nshell(int(z)))]
+        self.pro_shells = np.array(rows)
+        return propars
+
+    def calc_promol_dens(self, propars):
+        """glisa.py:346-348 (no floors)."""
+        return np.einsum("np,n->p", self.pro_shells, propars)
+
+    def working_matrix(self, rho, rho0, nderiv=0):
+        """glisa.py:411-479: objective, gradient and Hessian."""
+        sick = (rho < self.density_cutoff) | (rho0 < self.density_cutoff)
+        with np.errstate(all="ignore"):
+            ratio = np.divide(rho, rho0, out=np.zeros_like(rho), where=~sick)
+            ln_ratio = np.log(ratio, out=np.zeros_like(ratio), where=~sick)
+        f = self.grid.integrate(rho * ln_ratio)
+        if nderiv == 0:
+            return f
+        M = len(self.pro_shells)
+        grad = np.zeros(M)
+        hess = np.zeros((M, M))
+        w = self.grid.weights
+        for i in range(M):
+            with np.errstate(all="ignore"):
+                dfi = rho * self.pro_shells[i] / rho0
+            dfi[sick] = 0.0
+            grad[i] = -np.sum(w * dfi)
+            if nderiv > 1:
+                with np.errstate(all="ignore"):
+                    hi = dfi[None, :] * self.pro_shells[i:] / rho0[None, :]
+                hi[:, sick] = 0.0
+                hess[i, i:] = hi @ w
+                hess[i:, i] = hess[i, i:]
+        return (f, grad) if nderiv == 1 else (f, grad, hess)
+
+    def get_proatom_rho(self, i, propars):
+        """gisa.py:42-65 (glisa.py:149-163)."""
+        pops = propars[self.ranges[i] : self.ranges[i + 1]]
+        r = self.radial_distances[i] if self.on_molgrid else self.rgrid(i).points
+        return self.basis.proatom(int(self.numbers[i]), pops, r, 0)
+
+    def is_promol_valid(self, propars, check_mono=False):
+        """glisa.py:303-330."""
+        for i in range(self.natom):
+            rho0 = self.get_proatom_rho(i, propars)
+            if (rho0 < self.negative_cutoff).any():
+                return False
+            if check_mono and not self.on_molgrid and (rho0[:-1] - rho0[1:] < self.negative_cutoff).any():
+                return False
+        return True
+
+    def function_g(self, x):
+        """glisa.py:850-879."""
+        rho = self.moldens
+        old_rho0 = self.calc_promol_dens(x)
+        sick = (rho < self.density_cutoff) | (old_rho0 < self.density_cutoff)
+        new_x = np.zeros_like(x)
+        for n in range(len(x)):
+            with np.errstate(all="ignore"):
+                integrand = rho * self.pro_shells[n] * x[n] / old_rho0
+            integrand[sick] = 0.0
+            new_x[n] = self.grid.integrate(integrand)
+        return new_x
+
+    def solve_sc(self, propars):
+        """glisa.py:805-848."""
+        counter = 0
+        while True:
+            old = propars.copy()
+            propars[:] = self.function_g(propars)
+            change = self.compute_change(propars, old)
+            rho0 = self.calc_promol_dens(old)
+            self.history_entropies.append(self.working_matrix(self.moldens, rho0))
+            self.history_changes.append(change)
+            self.history_propars.append(propars.copy())
+            counter += 1
+            if change < self.threshold:
+                return propars, counter
+
+    def solve_newton(self, propars, mode, maxiter=100, linspace_size=40, tau=1.0, check_mono=False):
+        """glisa.py:617-803 with linesearch_mode "valid-promol"."""
+        rho = self.moldens
+        pop = self.grid.integrate(rho)
+        for irep in range(maxiter):
+            old = propars.copy()
+            pro = self.calc_promol_dens(propars)
+            f, df, hess = self.working_matrix(rho, pro, 2)
+            try:
+                delta = scipy_solve(hess, -1 - df, assume_a="sym")
+            except np.linalg.LinAlgError as e:
+                raise RuntimeError(e)
+            if mode == "exact":
+                propars[:] = propars + delta
+            else:
+                fixed = np.ones_like(delta)
+                for i, z in enumerate(self.numbers):
+                    sl = slice(self.ranges[i], self.ranges[i + 1])
+                    for j in fix_propars(self.basis.exponents[int(z)], propars[sl], delta[sl]):
+                        fixed[self.ranges[i] + j] = 0
+                for x in np.linspace(tau, 0, linspace_size, endpoint=False):
+                    new = propars + fixed * (x * delta)
+                    if self.is_promol_valid(new, check_mono=check_mono):
+                        break
+                else:
+                    raise RuntimeError("Line search failed!")
+                propars[:] = new
+            entropy = f  # _compute_entropy(rho, pro) is the objective at pro
+            change = self.compute_change(propars, old)
+            self.history_entropies.append(entropy)
+            self.history_propars.append(propars.copy())
+            self.history_changes.append(change)
+            if change < self.threshold:
+                if (pro < self.negative_cutoff).any():
+                    raise RuntimeError("Negative pro-atom density found!")
+                check_population(propars, pop, self.population_cutoff)
+                return propars, irep + 1
+        raise RuntimeError("Not converged!")
+
+    def do_partitioning(self):
+        """glisa.py:248-281."""
+        propars = self.init_propars()
+        if self.solver == "sc":
+            propars, niter = self.solve_sc(propars)
+        elif self.solver in ("newton", "m-newton"):
+            opts = dict(self.solver_options)
+            opts.setdefault("maxiter", 100)
+            propars, niter = self.solve_newton(propars, "exact" if self.solver == "newton" else "modified", **opts)
+        else:
+            raise NotImplementedError(self.solver)
+        self.cache["propars"] = propars
+        self.cache["niter"] = niter
+        self.on_molgrid_weights = True
+        self._weights_on_molgrid(propars)
+        charges = np.array([self.pseudo_numbers[i] - self.grid.integrate(self.cache[f"at_weights_{i}"] * self.moldens)
+                            for i in range(self.natom)])
+        self.cache["charges"] = charges
+        self.cache["populations"] = self.numbers - charges
+        self.cache["pseudo_populations"] = self.pseudo_numbers - charges
+        self.cache["change"] = self.history_changes[-1]
+        for k in ("propars", "entropies", "changes"):
+            self.cache[f"history_{k}"] = np.array(getattr(self, f"history_{k}"))
+        return self.cache
+
+    def _weights_on_molgrid(self, propars):
+        """update_at_weights(force_on_molgrid=True) (stockholder.py:352-384)."""
+        promol = np.zeros(self.grid.size)
+        proat = []
+        for i in range(self.natom):
+            work = self.basis.proatom(int(self.numbers[i]), propars[self.ranges[i] : self.ranges[i + 1]],
+                                      self.radial_distances[i], 0)
+            promol += work
+            promol += 1e-100
+            proat.append(work)
+        for i in range(self.natom):
+            w = proat[i] / promol
+            np.clip(w, 0, 1, out=w)
+            self.cache[f"at_weights_{i}"] = w
+        self.cache["promoldens"] = promol
