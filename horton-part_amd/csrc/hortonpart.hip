@@ -478,6 +478,45 @@ int gt2_iteration(hp_part *p, int precision, double thr, int min_stop, int max_i
   return 0;
 }
 
+// Partial sums of the moments: one row of HP_NMOM per radial shell (atom
+// grids) or per (atom, point chunk) (grid_type 3, the molecular grid).
+size_t moments_partials(const hp_part *p) {
+  if (!p->only_molgrid) return (size_t)p->nshell;
+  return (size_t)p->natom * (size_t)((p->npoint + HP_MOM_CHUNK - 1) / HP_MOM_CHUNK);
+}
+
+// Launch the moments passes: partials into part_buf (moments_partials rows),
+// per-atom sums into out_buf (natom rows).  grid_type 3: every atom's AIM
+// density on the whole molecular grid with the molecular weights
+// (core/base.py:352-402 with local = False); otherwise on the atom's own grid
+// with the atom-grid weights.
+int launch_moments(hp_part *p, double *part_buf, double *out_buf) {
+  DevFn f = p->fn();
+  if (p->only_molgrid) {
+    const int nchunk = (int)((p->npoint + HP_MOM_CHUNK - 1) / HP_MOM_CHUNK);
+    const dim3 g((unsigned)nchunk, (unsigned)p->natom);
+    if (p->fnkind == HP_FN_SLATER)
+      hipLaunchKernelGGL((k_moments_mol<HP_FN_SLATER>), g, dim3(256), 0, p->stream, p->mol(), f, p->promol, part_buf);
+    else if (p->fnkind == HP_FN_GAUSS)
+      hipLaunchKernelGGL((k_moments_mol<HP_FN_GAUSS>), g, dim3(256), 0, p->stream, p->mol(), f, p->promol, part_buf);
+    else
+      hipLaunchKernelGGL((k_moments_mol<HP_FN_GENERAL>), g, dim3(256), 0, p->stream, p->mol(), f, p->promol, part_buf);
+    HP_LAUNCHED();
+    hipLaunchKernelGGL(k_moments_reduce_mol, dim3(p->natom), dim3(64), 0, p->stream, nchunk, part_buf, out_buf);
+  } else {
+    if (p->fnkind == HP_FN_SLATER)
+      hipLaunchKernelGGL((k_moments<HP_FN_SLATER>), dim3(p->nshell), dim3(256), 0, p->stream, p->mol(), f, part_buf);
+    else if (p->fnkind == HP_FN_GAUSS)
+      hipLaunchKernelGGL((k_moments<HP_FN_GAUSS>), dim3(p->nshell), dim3(256), 0, p->stream, p->mol(), f, part_buf);
+    else
+      hipLaunchKernelGGL((k_moments<HP_FN_GENERAL>), dim3(p->nshell), dim3(256), 0, p->stream, p->mol(), f, part_buf);
+    HP_LAUNCHED();
+    hipLaunchKernelGGL(k_moments_reduce, dim3(p->natom), dim3(64), 0, p->stream, p->mol(), part_buf, out_buf);
+  }
+  HP_LAUNCHED();
+  return 0;
+}
+
 int run_stage_gt2(hp_part *p, int precision, double thr, int min_stop, int max_iter) {
   HP_HIP(hipMemsetAsync(p->state, 0, sizeof(int), p->stream));
   while (true) {
@@ -988,37 +1027,11 @@ int hp_part_moments(hp_part *p, int lmax, double *cart, double *pure, double *ra
   if (p->niter == 0 && !p->weights_ready) return fail(HP_E_ARG, "run the partitioning first");
   HP_HIP(hipSetDevice(p->device));
   int rc = 0;
-  // grid_type 3: every atom's AIM density on the whole molecular grid with the
-  // molecular weights (core/base.py:352-402 with local = False); otherwise on
-  // the atom's own grid with the atom-grid weights
-  const int nchunk = p->only_molgrid ? (int)((p->npoint + HP_MOM_CHUNK - 1) / HP_MOM_CHUNK) : 0;
-  const size_t npart = p->only_molgrid ? (size_t)p->natom * nchunk : (size_t)p->nshell;
+  const size_t npart = moments_partials(p);
   if (!p->mom_part && ((rc = dalloc(&p->mom_part, npart * HP_NMOM)) ||
                        (rc = dalloc(&p->mom_out, (size_t)p->natom * HP_NMOM))))
     return rc;
-  DevFn f = p->fn();
-  if (p->only_molgrid) {
-    const dim3 g((unsigned)nchunk, (unsigned)p->natom);
-    if (p->fnkind == HP_FN_SLATER)
-      hipLaunchKernelGGL((k_moments_mol<HP_FN_SLATER>), g, dim3(256), 0, p->stream, p->mol(), f, p->promol, p->mom_part);
-    else if (p->fnkind == HP_FN_GAUSS)
-      hipLaunchKernelGGL((k_moments_mol<HP_FN_GAUSS>), g, dim3(256), 0, p->stream, p->mol(), f, p->promol, p->mom_part);
-    else
-      hipLaunchKernelGGL((k_moments_mol<HP_FN_GENERAL>), g, dim3(256), 0, p->stream, p->mol(), f, p->promol, p->mom_part);
-    HP_LAUNCHED();
-    hipLaunchKernelGGL(k_moments_reduce_mol, dim3(p->natom), dim3(64), 0, p->stream, nchunk, p->mom_part, p->mom_out);
-  } else {
-    if (p->fnkind == HP_FN_SLATER)
-      hipLaunchKernelGGL((k_moments<HP_FN_SLATER>), dim3(p->nshell), dim3(256), 0, p->stream, p->mol(), f, p->mom_part);
-    else if (p->fnkind == HP_FN_GAUSS)
-      hipLaunchKernelGGL((k_moments<HP_FN_GAUSS>), dim3(p->nshell), dim3(256), 0, p->stream, p->mol(), f, p->mom_part);
-    else
-      hipLaunchKernelGGL((k_moments<HP_FN_GENERAL>), dim3(p->nshell), dim3(256), 0, p->stream, p->mol(), f,
-                         p->mom_part);
-    HP_LAUNCHED();
-    hipLaunchKernelGGL(k_moments_reduce, dim3(p->natom), dim3(64), 0, p->stream, p->mol(), p->mom_part, p->mom_out);
-  }
-  HP_LAUNCHED();
+  if ((rc = launch_moments(p, p->mom_part, p->mom_out))) return rc;
   std::vector<double> h((size_t)p->natom * HP_NMOM);
   std::vector<double> z(p->natom);
   HP_HIP(hipMemcpyAsync(h.data(), p->mom_out, sizeof(double) * h.size(), hipMemcpyDeviceToHost, p->stream));
