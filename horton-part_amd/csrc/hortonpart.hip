@@ -16,6 +16,7 @@
 
 #include "hortonpart.h"
 #include "hp_device.h"
+#include "hp_glisa.h"
 #include "hp_gt2.h"
 #include "hp_host.h"
 
@@ -89,6 +90,11 @@ struct hp_part {
   int tail_cull = 1;          // weights pass may skip atoms bounded below one rounding
   int params_nonneg = 1;      // the parameters last set are all >= 0 (tail skipping needs it)
   unsigned long long *evals_done = nullptr;  // [FP32, FP64] evaluations of the last run
+  // gLISA global terms (allocated at the first call)
+  int gl_M = 0, gl_Mp = 0, gl_npair = 0, gl_nchunk = 0, gl_nblock = 0;
+  double *gl_G = nullptr, *gl_D = nullptr, *gl_E = nullptr, *gl_obj = nullptr, *gl_grad = nullptr,
+         *gl_hess = nullptr, *gl_out = nullptr, *gl_coef = nullptr, *gl_pro = nullptr;
+  int2 *gl_pairs = nullptr;
   double *corr = nullptr;     // per radial shell: FP64 - FP32 spherical average
   // host state
   int niter = 0;
@@ -157,7 +163,9 @@ void free_part(hp_part *p) {
                   p->fn_order, p->coef,       p->alpha,   p->qpP,       p->qp_off,      p->par,      p->old_par,
                   p->rhobar,  p->ent_part,    p->promol,  p->charges,   p->change_part, p->out,      p->mom_part,
                   p->mom_out, p->scratch_pts, p->inner_niter, p->status, p->status_all, p->hpar,    p->hch,
-                  p->hent,    p->hchg,    p->corr,      p->evals_done, p->grp_first};
+                  p->hent,    p->hchg,    p->corr,      p->evals_done, p->grp_first, p->gl_G, p->gl_D,
+                  p->gl_E,    p->gl_obj,  p->gl_grad,   p->gl_hess,  p->gl_out,    p->gl_coef, p->gl_pro,
+                  p->gl_pairs};
   for (void *b : bufs)
     if (b) (void)hipFree(b);
   if (p->h_out) (void)hipHostFree(p->h_out);
@@ -480,8 +488,8 @@ int gt2_iteration(hp_part *p, int precision, double thr, int min_stop, int max_i
 
 // Partial sums of the moments: one row of HP_NMOM per radial shell (atom
 // grids) or per (atom, point chunk) (grid_type 3, the molecular grid).
-size_t moments_partials(const hp_part *p) {
-  if (!p->only_molgrid) return (size_t)p->nshell;
+size_t moments_partials(const hp_part *p, bool molgrid) {
+  if (!molgrid) return (size_t)p->nshell;
   return (size_t)p->natom * (size_t)((p->npoint + HP_MOM_CHUNK - 1) / HP_MOM_CHUNK);
 }
 
@@ -490,9 +498,9 @@ size_t moments_partials(const hp_part *p) {
 // density on the whole molecular grid with the molecular weights
 // (core/base.py:352-402 with local = False); otherwise on the atom's own grid
 // with the atom-grid weights.
-int launch_moments(hp_part *p, double *part_buf, double *out_buf) {
+int launch_moments(hp_part *p, double *part_buf, double *out_buf, bool molgrid) {
   DevFn f = p->fn();
-  if (p->only_molgrid) {
+  if (molgrid) {
     const int nchunk = (int)((p->npoint + HP_MOM_CHUNK - 1) / HP_MOM_CHUNK);
     const dim3 g((unsigned)nchunk, (unsigned)p->natom);
     if (p->fnkind == HP_FN_SLATER)
@@ -1027,11 +1035,11 @@ int hp_part_moments(hp_part *p, int lmax, double *cart, double *pure, double *ra
   if (p->niter == 0 && !p->weights_ready) return fail(HP_E_ARG, "run the partitioning first");
   HP_HIP(hipSetDevice(p->device));
   int rc = 0;
-  const size_t npart = moments_partials(p);
+  const size_t npart = moments_partials(p, p->only_molgrid);
   if (!p->mom_part && ((rc = dalloc(&p->mom_part, npart * HP_NMOM)) ||
                        (rc = dalloc(&p->mom_out, (size_t)p->natom * HP_NMOM))))
     return rc;
-  if ((rc = launch_moments(p, p->mom_part, p->mom_out))) return rc;
+  if ((rc = launch_moments(p, p->mom_part, p->mom_out, p->only_molgrid))) return rc;
   std::vector<double> h((size_t)p->natom * HP_NMOM);
   std::vector<double> z(p->natom);
   HP_HIP(hipMemcpyAsync(h.data(), p->mom_out, sizeof(double) * h.size(), hipMemcpyDeviceToHost, p->stream));
@@ -1050,6 +1058,142 @@ int hp_part_moments(hp_part *p, int lmax, double *cart, double *pure, double *ra
     if (radial)
       for (int i = 0; i < 4; ++i) radial[a * 4 + i] = v[36 + i];
   }
+  return 0;
+}
+
+static int glisa_setup(hp_part *p) {
+  if (p->gl_G) return 0;
+  const int M = p->nfn, Mp = ((M + 15) / 16) * 16, T = Mp / 16;
+  p->gl_M = M;
+  p->gl_Mp = Mp;
+  p->gl_npair = T * (T + 1) / 2;
+  p->gl_nchunk = (int)((p->npoint + HP_GLISA_CHUNK - 1) / HP_GLISA_CHUNK);
+  p->gl_nblock = (int)((p->npoint + 255) / 256);
+  std::vector<int2> pairs;
+  for (int I = 0; I < T; ++I)
+    for (int J = I; J < T; ++J) pairs.push_back(make_int2(I, J));
+  int rc = 0;
+  if ((rc = dalloc(&p->gl_G, (size_t)p->npoint * Mp)) || (rc = dalloc(&p->gl_D, (size_t)p->npoint)) ||
+      (rc = dalloc(&p->gl_E, (size_t)p->npoint)) || (rc = dalloc(&p->gl_obj, 2 * (size_t)p->gl_nblock)) ||
+      (rc = dalloc(&p->gl_grad, (size_t)p->gl_nchunk * M)) ||
+      (rc = dalloc(&p->gl_hess, (size_t)p->gl_npair * p->gl_nchunk * 256)) ||
+      (rc = dalloc(&p->gl_out, 2 + (size_t)M + (size_t)M * M)) || (rc = dalloc(&p->gl_coef, 2 * (size_t)M)) ||
+      (rc = dalloc(&p->gl_pro, 2 * (size_t)p->natom * (p->gl_nchunk + 1))) ||
+      (rc = dupload(&p->gl_pairs, pairs.data(), pairs.size(), p->stream)))
+    return rc;
+  return 0;
+}
+
+int hp_part_global_terms(hp_part *p, const double *coefs, int nderiv, double *objective, double *gradient,
+                         double *hessian, double *min_promol) {
+  if (!p || !coefs) return fail(HP_E_ARG, "NULL argument");
+  if (p->scheme != HP_SCHEME_LISA) return fail(HP_E_UNSUPP, "the global terms need a linear-coefficient basis (LISA)");
+  if (nderiv < 0 || nderiv > 2) return fail(HP_E_ARG, "nderiv must be 0, 1 or 2");
+  HP_HIP(hipSetDevice(p->device));
+  if (int rc = glisa_setup(p)) return rc;
+  hipStream_t s = p->stream;
+  const int M = p->gl_M, Mp = p->gl_Mp;
+  HP_HIP(hipMemcpyAsync(p->gl_coef, coefs, sizeof(double) * M, hipMemcpyHostToDevice, s));
+  DevFn f = p->fn();
+  f.alpha = p->fn_alpha;
+  const dim3 g1((unsigned)p->gl_nblock), b1(256);
+#define HP_GL_POINTS(FK)                                                                                          \
+  hipLaunchKernelGGL((k_glisa_points<FK>), g1, b1, 0, s, p->mol(), f, p->fn_norm, p->gl_coef, M, Mp, p->dc, p->gl_G, \
+                     p->gl_D, p->gl_E, p->gl_obj)
+  if (p->fnkind == HP_FN_SLATER)
+    HP_GL_POINTS(HP_FN_SLATER);
+  else if (p->fnkind == HP_FN_GAUSS)
+    HP_GL_POINTS(HP_FN_GAUSS);
+  else
+    HP_GL_POINTS(HP_FN_GENERAL);
+#undef HP_GL_POINTS
+  HP_LAUNCHED();
+  if (nderiv >= 1) {
+    hipLaunchKernelGGL(k_glisa_grad, dim3((unsigned)p->gl_nchunk), dim3(256), 0, s, p->npoint, M, Mp, p->gl_G, p->gl_E,
+                       p->gl_grad);
+    HP_LAUNCHED();
+  }
+  if (nderiv == 2) {
+    hipLaunchKernelGGL(k_glisa_hess, dim3((unsigned)p->gl_npair, (unsigned)p->gl_nchunk), dim3(64), 0, s, p->npoint,
+                       Mp, p->gl_pairs, p->gl_G, p->gl_D, p->gl_hess);
+    HP_LAUNCHED();
+  }
+  const int nt = std::max({1, nderiv >= 1 ? M : 0, nderiv == 2 ? p->gl_npair * 256 : 0});
+  hipLaunchKernelGGL(k_glisa_reduce, dim3((unsigned)((nt + 255) / 256)), dim3(256), 0, s, p->gl_nblock, p->gl_obj,
+                     p->gl_nchunk, M, nderiv >= 1 ? 1 : 0, p->gl_grad, nderiv == 2 ? p->gl_npair : 0, p->gl_pairs,
+                     nderiv == 2 ? p->gl_hess : nullptr, p->gl_out);
+  HP_LAUNCHED();
+  if (objective) HP_HIP(hipMemcpyAsync(objective, p->gl_out, sizeof(double), hipMemcpyDeviceToHost, s));
+  if (min_promol)
+    HP_HIP(hipMemcpyAsync(min_promol, p->gl_out + 1 + M + (size_t)M * M, sizeof(double), hipMemcpyDeviceToHost, s));
+  if (gradient && nderiv >= 1)
+    HP_HIP(hipMemcpyAsync(gradient, p->gl_out + 1, sizeof(double) * M, hipMemcpyDeviceToHost, s));
+  if (hessian && nderiv == 2)
+    HP_HIP(hipMemcpyAsync(hessian, p->gl_out + 1 + M, sizeof(double) * M * M, hipMemcpyDeviceToHost, s));
+  HP_HIP(hipStreamSynchronize(s));
+  return 0;
+}
+
+int hp_part_global_proatoms(hp_part *p, const double *coefs_new, const double *coefs_old, double *min_proatom,
+                            double *msd) {
+  if (!p || !coefs_new || !coefs_old) return fail(HP_E_ARG, "NULL argument");
+  if (p->scheme != HP_SCHEME_LISA) return fail(HP_E_UNSUPP, "the global terms need a linear-coefficient basis (LISA)");
+  HP_HIP(hipSetDevice(p->device));
+  if (int rc = glisa_setup(p)) return rc;
+  hipStream_t s = p->stream;
+  const int M = p->gl_M;
+  HP_HIP(hipMemcpyAsync(p->gl_coef, coefs_new, sizeof(double) * M, hipMemcpyHostToDevice, s));
+  HP_HIP(hipMemcpyAsync(p->gl_coef + M, coefs_old, sizeof(double) * M, hipMemcpyHostToDevice, s));
+  DevFn f = p->fn();
+  f.alpha = p->fn_alpha;
+  const dim3 g((unsigned)p->gl_nchunk, (unsigned)p->natom), b(256);
+  double *part = p->gl_pro, *out = p->gl_pro + 2 * (size_t)p->natom * p->gl_nchunk;
+#define HP_GL_PRO(FK)                                                                                             \
+  hipLaunchKernelGGL((k_glisa_proatoms<FK>), g, b, 0, s, p->mol(), f, p->fn_norm, p->gl_coef, p->gl_coef + M, part)
+  if (p->fnkind == HP_FN_SLATER)
+    HP_GL_PRO(HP_FN_SLATER);
+  else if (p->fnkind == HP_FN_GAUSS)
+    HP_GL_PRO(HP_FN_GAUSS);
+  else
+    HP_GL_PRO(HP_FN_GENERAL);
+#undef HP_GL_PRO
+  HP_LAUNCHED();
+  hipLaunchKernelGGL(k_glisa_proatoms_reduce, dim3((p->natom + 63) / 64), dim3(64), 0, s, p->natom, p->gl_nchunk, part,
+                     out);
+  HP_LAUNCHED();
+  std::vector<double> h(2 * (size_t)p->natom);
+  HP_HIP(hipMemcpyAsync(h.data(), out, sizeof(double) * h.size(), hipMemcpyDeviceToHost, s));
+  HP_HIP(hipStreamSynchronize(s));
+  for (int a = 0; a < p->natom; ++a) {
+    if (min_proatom) min_proatom[a] = h[2 * a];
+    if (msd) msd[a] = h[2 * a + 1];
+  }
+  return 0;
+}
+
+int hp_part_populations(hp_part *p, double *pop) {
+  if (!p || !pop) return fail(HP_E_ARG, "NULL argument");
+  if (p->grid_type != 2) return fail(HP_E_UNSUPP, "molecular-grid populations need grid_type 2 or 3");
+  if (p->niter == 0 && !p->weights_ready) return fail(HP_E_ARG, "run a weights pass first");
+  HP_HIP(hipSetDevice(p->device));
+  double *part = nullptr, *out = nullptr;
+  const size_t npart = moments_partials(p, true);
+  int rc = 0;
+  if ((rc = dalloc(&part, npart * HP_NMOM)) || (rc = dalloc(&out, (size_t)p->natom * HP_NMOM))) {
+    if (part) (void)hipFree(part);
+    return rc;
+  }
+  rc = launch_moments(p, part, out, true);
+  std::vector<double> h((size_t)p->natom * HP_NMOM);
+  if (!rc) {
+    hipError_t e = hipMemcpyAsync(h.data(), out, sizeof(double) * h.size(), hipMemcpyDeviceToHost, p->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(p->stream);
+    if (e != hipSuccess) rc = fail(HP_E_HIP, std::string("populations: ") + hipGetErrorString(e));
+  }
+  (void)hipFree(part);
+  (void)hipFree(out);
+  if (rc) return rc;
+  for (int a = 0; a < p->natom; ++a) pop[a] = h[(size_t)a * HP_NMOM];
   return 0;
 }
 

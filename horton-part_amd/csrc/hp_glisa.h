@@ -1,0 +1,211 @@
+// gLISA: the global problem over every pro-atom coefficient on the molecular
+// grid (glisa.py:248-281, 346-348, 411-479, 617-879).
+//
+// With M = sum_a K_a basis functions g_i (unit coefficient) and coefficients
+// c, the promolecule is rho0 = sum_i c_i g_i (no floors) and, over the points
+// where neither rho nor rho0 is below the density cut-off,
+//   f      = sum_p w rho ln(rho / rho0)
+//   grad_i = -sum_p w rho g_i / rho0
+//   H_ij   = sum_p w rho g_i g_j / rho0^2
+// k_glisa_points evaluates the basis once per point into a point-major table
+// G (row p holds g_0(p) .. g_{M-1}(p), padded to a multiple of 16 with
+// zeros), the weights D = w rho / rho0^2 and E = w rho / rho0, and the
+// objective partials; k_glisa_grad reduces E g_i per point chunk;
+// k_glisa_hess accumulates 16 x 16 tiles of G^T diag(D) G on the matrix cores
+// (mfma_f64_16x16x4f64, one wavefront per tile pair and point chunk); fixed-
+// order reductions over the chunks follow, so results do not depend on
+// scheduling.
+#pragma once
+#include "hp_device.h"
+
+typedef double hp_gd4 __attribute__((ext_vector_type(4)));
+
+#define HP_GLISA_CHUNK 4096  // points per workgroup of the reductions
+
+// Basis value of function f at distance r (and r^2) from its atom.
+template <int FK>
+__device__ __forceinline__ double glisa_basis(const DevFn &fn, const double *norm, int f, double r, double r2) {
+  double arg;
+  if (FK == HP_FN_SLATER) {
+    arg = r;
+  } else if (FK == HP_FN_GAUSS) {
+    arg = r2;
+  } else {
+    const double n = fn.order[f];
+    arg = (n == 1.0) ? r : ((n == 2.0) ? r2 : pow(r, n));
+  }
+  return norm[f] * exp(-fn.alpha[f] * arg);
+}
+
+// One thread per point: G row, D, E and the objective's block partial.
+template <int FK>
+__global__ __launch_bounds__(256) void k_glisa_points(DevMol m, DevFn fn, const double *__restrict__ norm,
+                                                      const double *__restrict__ coef, int M, int Mp, double dc,
+                                                      double *__restrict__ G, double *__restrict__ D,
+                                                      double *__restrict__ E, double *__restrict__ obj_part) {
+  __shared__ double lds[32];
+  __shared__ double s_min[4];
+  const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  double fo = 0.0, dummy = 0.0, mn = 1.0e308;
+  if (p < m.npoint) {
+    const double x = m.px[p], y = m.py[p], z = m.pz[p];
+    double rho0 = 0.0;
+    double *row = G + p * Mp;
+    for (int a = 0; a < m.natom; ++a) {
+      const double dx = x - m.cx[a], dy = y - m.cy[a], dz = z - m.cz[a];
+      const double r2 = dx * dx + dy * dy + dz * dz;
+      const double r = sqrt(r2);
+      for (int f = fn.fn_off[a]; f < fn.fn_off[a + 1]; ++f) {
+        const double g = glisa_basis<FK>(fn, norm, f, r, r2);
+        row[f] = g;
+        rho0 += coef[f] * g;
+      }
+    }
+    for (int f = M; f < Mp; ++f) row[f] = 0.0;
+    const double rho = m.rho[p], w = m.wmol[p];
+    const bool sick = (rho < dc) || (rho0 < dc);
+    D[p] = sick ? 0.0 : w * rho / (rho0 * rho0);
+    E[p] = sick ? 0.0 : w * rho / rho0;
+    fo = sick ? 0.0 : w * rho * log(rho / rho0);
+    mn = rho0;
+  }
+  const double wmin = wave_min(mn);
+  if ((threadIdx.x & 63) == 0) s_min[threadIdx.x >> 6] = wmin;
+  block_sum2(fo, dummy, lds);
+  if (threadIdx.x == 0) {
+    obj_part[2 * blockIdx.x] = fo;
+    obj_part[2 * blockIdx.x + 1] = fmin(fmin(s_min[0], s_min[1]), fmin(s_min[2], s_min[3]));
+  }
+}
+
+// Gradient partials: workgroup c sums E g_i over its chunk for every i.
+__global__ __launch_bounds__(256) void k_glisa_grad(int64_t npoint, int M, int Mp, const double *__restrict__ G,
+                                                    const double *__restrict__ E, double *__restrict__ grad_part) {
+  __shared__ double lds[32];
+  const int64_t p0 = (int64_t)blockIdx.x * HP_GLISA_CHUNK;
+  const int64_t p1 = p0 + HP_GLISA_CHUNK < npoint ? p0 + HP_GLISA_CHUNK : npoint;
+  for (int i = 0; i < M; ++i) {
+    double s = 0.0, dummy = 0.0;
+    for (int64_t p = p0 + threadIdx.x; p < p1; p += blockDim.x) s += E[p] * G[p * Mp + i];
+    block_sum2(s, dummy, lds);
+    if (threadIdx.x == 0) grad_part[(int64_t)blockIdx.x * M + i] = s;
+    __syncthreads();
+  }
+}
+
+// Hessian tile partials: one wavefront per (tile pair I <= J, point chunk).
+// Each step is a rank-4 update with 4 points: lane l supplies
+// A[i][k] = D(p) G(p, 16 I + i) and B[k][j] = G(p, 16 J + j) with i = j = l % 16
+// and p = p0 + k, k = l / 16.  Output layout (gfx950, FP64 16x16x4): lane l,
+// register v holds C[4 v + l / 16][l % 16].
+__global__ __launch_bounds__(64) void k_glisa_hess(int64_t npoint, int Mp, const int2 *__restrict__ pairs,
+                                                   const double *__restrict__ G, const double *__restrict__ D,
+                                                   double *__restrict__ hess_part) {
+  const int2 pr = pairs[blockIdx.x];
+  const int lane = threadIdx.x;
+  const int i = lane & 15, k = lane >> 4;
+  const int64_t p0 = (int64_t)blockIdx.y * HP_GLISA_CHUNK;
+  const int64_t p1 = p0 + HP_GLISA_CHUNK < npoint ? p0 + HP_GLISA_CHUNK : npoint;
+  hp_gd4 acc = {0.0, 0.0, 0.0, 0.0};
+  for (int64_t q = p0; q < p1; q += 4) {
+    const int64_t p = q + k;
+    double av = 0.0, bv = 0.0;
+    if (p < p1) {
+      av = D[p] * G[p * Mp + 16 * pr.x + i];
+      bv = G[p * Mp + 16 * pr.y + i];
+    }
+    acc = __builtin_amdgcn_mfma_f64_16x16x4f64(av, bv, acc, 0, 0, 0);
+  }
+  double *out = hess_part + ((int64_t)blockIdx.x * gridDim.y + blockIdx.y) * 256;
+#pragma unroll
+  for (int v = 0; v < 4; ++v) out[(4 * v + k) * 16 + i] = acc[v];
+}
+
+// Fixed-order sums over the chunks: objective, gradient (negated) and the
+// symmetric Hessian.
+__global__ __launch_bounds__(256) void k_glisa_reduce(int nblock_obj, const double *__restrict__ obj_part, int nchunk,
+                                                      int M, int do_grad, const double *__restrict__ grad_part, int npair,
+                                                      const int2 *__restrict__ pairs,
+                                                      const double *__restrict__ hess_part, double *__restrict__ out) {
+  // out: [0] objective, [1 .. 1+M) gradient, [1+M .. 1+M+M^2) Hessian, then
+  // the smallest promolecule value
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t == 0) {
+    double s = 0.0, mn = 1.0e308;
+    for (int b = 0; b < nblock_obj; ++b) {
+      s += obj_part[2 * b];
+      mn = fmin(mn, obj_part[2 * b + 1]);
+    }
+    out[0] = s;
+    out[1 + M + (int64_t)M * M] = mn;
+  }
+  if (do_grad && t < M) {
+    double s = 0.0;
+    for (int c = 0; c < nchunk; ++c) s += grad_part[(int64_t)c * M + t];
+    out[1 + t] = -s;
+  }
+  if (hess_part && t < npair * 256) {
+    const int pi = t / 256, e = t % 256;
+    const int2 pr = pairs[pi];
+    const int ii = 16 * pr.x + e / 16, jj = 16 * pr.y + e % 16;
+    if (ii < M && jj < M && ii <= jj) {
+      double s = 0.0;
+      for (int c = 0; c < nchunk; ++c) s += hess_part[((int64_t)pi * nchunk + c) * 256 + e];
+      out[1 + M + (int64_t)ii * M + jj] = s;
+      out[1 + M + (int64_t)jj * M + ii] = s;
+    }
+  }
+}
+
+// Per (point chunk, atom): the smallest value of the atom's pro-atom at cnew
+// and sum_p w (rho0_a(cnew) - rho0_a(cold))^2 over the molecular grid.
+template <int FK>
+__global__ __launch_bounds__(256) void k_glisa_proatoms(DevMol m, DevFn fn, const double *__restrict__ norm,
+                                                        const double *__restrict__ cnew,
+                                                        const double *__restrict__ cold,
+                                                        double *__restrict__ part) {
+  __shared__ double lds[32];
+  const int a = blockIdx.y;
+  const int64_t p0 = (int64_t)blockIdx.x * HP_GLISA_CHUNK;
+  const int64_t p1 = p0 + HP_GLISA_CHUNK < m.npoint ? p0 + HP_GLISA_CHUNK : m.npoint;
+  double mn = 1.0e308, msd = 0.0;
+  for (int64_t p = p0 + threadIdx.x; p < p1; p += blockDim.x) {
+    const double dx = m.px[p] - m.cx[a], dy = m.py[p] - m.cy[a], dz = m.pz[p] - m.cz[a];
+    const double r2 = dx * dx + dy * dy + dz * dz;
+    const double r = sqrt(r2);
+    double y1 = 0.0, y2 = 0.0;
+    for (int f = fn.fn_off[a]; f < fn.fn_off[a + 1]; ++f) {
+      const double g = glisa_basis<FK>(fn, norm, f, r, r2);
+      y1 += cnew[f] * g;
+      y2 += cold[f] * g;
+    }
+    mn = fmin(mn, y1);
+    const double d = y1 - y2;
+    msd += m.wmol[p] * d * d;
+  }
+  const double lane_min = wave_min(mn);
+  double dummy = 0.0;
+  block_sum2(msd, dummy, lds);
+  __shared__ double s_min[8];
+  if ((threadIdx.x & 63) == 0) s_min[threadIdx.x >> 6] = lane_min;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double v = s_min[0];
+    for (int w = 1; w < (int)(blockDim.x >> 6); ++w) v = fmin(v, s_min[w]);
+    part[((int64_t)a * gridDim.x + blockIdx.x) * 2] = v;
+    part[((int64_t)a * gridDim.x + blockIdx.x) * 2 + 1] = msd;
+  }
+}
+
+__global__ void k_glisa_proatoms_reduce(int natom, int nchunk, const double *__restrict__ part,
+                                        double *__restrict__ out) {
+  const int a = blockIdx.x * blockDim.x + threadIdx.x;
+  if (a >= natom) return;
+  double mn = 1.0e308, msd = 0.0;
+  for (int c = 0; c < nchunk; ++c) {
+    mn = fmin(mn, part[((int64_t)a * nchunk + c) * 2]);
+    msd += part[((int64_t)a * nchunk + c) * 2 + 1];
+  }
+  out[2 * a] = mn;
+  out[2 * a + 1] = msd;
+}

@@ -1,4 +1,4 @@
-"""MI355X-native stockholder partitioning (MBIS / GISA / aLISA / NLIS / GMBIS).
+"""MI355X-native stockholder partitioning (MBIS / GISA / aLISA / gLISA / NLIS / GMBIS).
 
 Drop-in for the grid-based stockholder loop of horton-part: the scheme
 classes keep the reference's ``WPart`` API and run the loop in
@@ -8,6 +8,7 @@ classes keep the reference's ``WPart`` API and run the loop in
 from .alisa import LinearISAWPart
 from .basis import ExpBasisFuncHelper, evaluate_function
 from .gisa import GaussianISAWPart
+from .glisa import GlobalLinearISAWPart
 from .gmbis import GMBISWPart, get_initial_gmbis_propars
 from .grid import AtomGrid, BeckeWeights, ExpRTransform, MolGrid, OneDGrid, UniformInteger
 from .mbis import MBISWPart, get_initial_mbis_propars, get_nshell
@@ -19,6 +20,7 @@ __all__ = [
     "MBISWPart",
     "GaussianISAWPart",
     "LinearISAWPart",
+    "GlobalLinearISAWPart",
     "NLISWPart",
     "GMBISWPart",
     "get_nlis_nshell",

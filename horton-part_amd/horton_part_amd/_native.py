@@ -53,6 +53,9 @@ EXPORTED_SYMBOLS = [
     "hp_part_at_weights",
     "hp_part_promoldens",
     "hp_part_moments",
+    "hp_part_global_terms",
+    "hp_part_global_proatoms",
+    "hp_part_populations",
     "hp_part_kernel_time",
     "hp_part_get_work",
     "hp_batch_create",
@@ -237,6 +240,9 @@ def load():
     lib.hp_part_promoldens.argtypes = [ctypes.c_void_p, _pd, ctypes.c_int64]
     lib.hp_part_moments.argtypes = [ctypes.c_void_p, ctypes.c_int, _pd, _pd, _pd]
     lib.hp_part_kernel_time.argtypes = [ctypes.c_void_p, ctypes.c_int, _pd, _pi64]
+    lib.hp_part_global_terms.argtypes = [ctypes.c_void_p, _pd, ctypes.c_int, _pd, _pd, _pd, _pd]
+    lib.hp_part_global_proatoms.argtypes = [ctypes.c_void_p, _pd, _pd, _pd, _pd]
+    lib.hp_part_populations.argtypes = [ctypes.c_void_p, _pd]
     lib.hp_part_get_work.argtypes = [ctypes.c_void_p, _P(PartWork)]
     lib.hp_batch_create.argtypes = [ctypes.c_int, _P(BatchConfig), _P(ctypes.c_void_p)]
     lib.hp_batch_destroy.argtypes = [ctypes.c_void_p]
@@ -411,6 +417,35 @@ class DevicePartition:
         rad = np.empty((self.natom, 4))
         check(self.lib.hp_part_moments(self.handle, int(lmax), ptr(cart), ptr(pure), ptr(rad)))
         return cart, pure, rad
+
+    def global_terms(self, coefs, nderiv=2):
+        """gLISA's objective, gradient, Hessian and promolecule minimum at
+        ``coefs`` (``hp_part_global_terms``)."""
+        coefs = np.ascontiguousarray(coefs, dtype=float)
+        m = coefs.size
+        f = ctypes.c_double()
+        mn = ctypes.c_double()
+        grad = np.empty(m) if nderiv >= 1 else None
+        hess = np.empty((m, m)) if nderiv == 2 else None
+        check(self.lib.hp_part_global_terms(self.handle, ptr(coefs), int(nderiv), ctypes.byref(f), ptr(grad), ptr(hess),
+                                            ctypes.byref(mn)))
+        return f.value, grad, hess, mn.value
+
+    def global_proatoms(self, coefs_new, coefs_old):
+        """Per atom: minimum of its pro-atom at ``coefs_new`` and the squared
+        change against ``coefs_old`` on the molecular grid."""
+        cn = np.ascontiguousarray(coefs_new, dtype=float)
+        co = np.ascontiguousarray(coefs_old, dtype=float)
+        mn = np.empty(self.natom)
+        msd = np.empty(self.natom)
+        check(self.lib.hp_part_global_proatoms(self.handle, ptr(cn), ptr(co), ptr(mn), ptr(msd)))
+        return mn, msd
+
+    def populations(self):
+        """AIM populations on the molecular grid from the last weights pass."""
+        out = np.empty(self.natom)
+        check(self.lib.hp_part_populations(self.handle, ptr(out)))
+        return out
 
     def work(self):
         """Algorithmic work counts (``hp_part_get_work``) as a dict."""

@@ -43,7 +43,7 @@ def wpart_schemes(scheme):
     """Return the partitioning class registered under ``scheme``.
 
     The stockholder schemes of the accelerated path are provided:
-    ``"mbis"``, ``"gisa"``, ``"lisa"``, ``"nlis"`` and ``"gmbis"``.
+    ``"mbis"``, ``"gisa"``, ``"lisa"``, ``"glisa"``, ``"nlis"`` and ``"gmbis"``.
     """
     if scheme == "mbis":
         from .mbis import MBISWPart
@@ -57,6 +57,10 @@ def wpart_schemes(scheme):
         from .alisa import LinearISAWPart
 
         return LinearISAWPart
+    if scheme == "glisa":
+        from .glisa import GlobalLinearISAWPart
+
+        return GlobalLinearISAWPart
     if scheme == "nlis":
         from .nlis import NLISWPart
 

@@ -214,6 +214,36 @@ int hp_part_promoldens(hp_part *part, double *out, int64_t n);
  * molecular weights (local = False, core/base.py:455-459). */
 int hp_part_moments(hp_part *part, int lmax, double *cartesian, double *pure, double *radial);
 
+/* ---- gLISA: the global problem (glisa.py:55-1028) ----
+ * A partitioning created with HP_SCHEME_LISA and grid_type 2 or 3 holds the
+ * basis functions g_i (fn_alpha, fn_order, fn_norm) at unit coefficient.
+ * These entry points evaluate, on the molecular grid with the molecular
+ * weights, what gLISA's solvers need per step; the solvers' own logic (the
+ * sc fixed point, the Newton step and its line search) runs in the caller. */
+
+/* At the coefficients c (sum K_a values): the promolecule rho0 = sum_i c_i g_i
+ * (calc_promol_dens, glisa.py:346-348, no floors), the objective
+ * f = sum_p w rho ln(rho/rho0) (also the entropy of stockholder.py:145-151 at
+ * rho0), for nderiv >= 1 the gradient -sum_p w rho g_i / rho0 and for
+ * nderiv = 2 the Hessian sum_p w rho g_i g_j / rho0^2 (M x M, row-major, on
+ * the matrix cores) of _working_matrix (glisa.py:411-479), points where rho or
+ * rho0 is below density_cutoff left out; min_promol receives min_p rho0.
+ * Any output may be NULL. */
+int hp_part_global_terms(hp_part *part, const double *coefs, int nderiv, double *objective, double *gradient,
+                         double *hessian, double *min_promol);
+
+/* Per atom over the molecular grid: the smallest value of its pro-atom at
+ * coefs_new (is_proatom_valid, glisa.py:315-330) and
+ * sum_p w (rho0_a(new) - rho0_a(old))^2 (compute_change on the molecular
+ * grid, iterstock.py:32-45).  Either output may be NULL. */
+int hp_part_global_proatoms(hp_part *part, const double *coefs_new, const double *coefs_old, double *min_proatom,
+                            double *msd);
+
+/* AIM populations sum_p w rho clip(rho0_a / promol) on the molecular grid,
+ * from the last weights pass (hp_part_weights or hp_part_run): the charges
+ * step of glisa.py:270-278.  grid_type 2 or 3. */
+int hp_part_populations(hp_part *part, double *populations);
+
 /* Timing of the pro-atom/weights kernel over the last hp_part_run, measured
  * with HIP events on the library's stream: total milliseconds and launches. */
 int hp_part_kernel_time(hp_part *part, int kernel, double *total_ms, int64_t *launches);

@@ -52,7 +52,7 @@ def _check_inputs(case, fx):
     np.testing.assert_allclose(case.input_digest(), fx["digest"], rtol=1e-12, atol=0)
 
 
-def _compare(name, propars, charges, niter, fx, inner=None, qtol=1e-10, ptol=1e-9, hist=None):
+def _compare(name, propars, charges, niter, fx, inner=None, qtol=1e-10, ptol=1e-9, hist=None, erel=1e-9):
     dq = float(np.max(np.abs(charges - fx["charges"])))
     rel = float(np.max(np.abs(propars - fx["propars"]) / np.maximum(np.abs(fx["propars"]), 1e-300)))
     _say(f"  {name}: niter gpu={niter} oracle={int(fx['niter'])} max|dq|={dq:.2e} rel dpar={rel:.2e}")
@@ -64,7 +64,17 @@ def _compare(name, propars, charges, niter, fx, inner=None, qtol=1e-10, ptol=1e-
     if hist is not None:
         changes, entropies = hist
         np.testing.assert_allclose(changes, fx["history_changes"], rtol=1e-6, atol=1e-14)
-        np.testing.assert_allclose(entropies, fx["history_entropies"], rtol=1e-9, atol=1e-12)
+        np.testing.assert_allclose(entropies, fx["history_entropies"], rtol=erel, atol=1e-12)
+
+
+# Cases whose problem does not fix its solution to 1e-10: the 24-atom aLISA
+# cluster's m-newton steps solve Hessians of condition numbers up to ~1e12,
+# and the oracle itself, given the density perturbed by 1e-15 relative
+# (three draws), moves its charges by up to 1.4e-8 e and its parameters by
+# up to 1.3e-3 relative with the same iteration counts.  Its charges are held
+# to 5e-8 e (about 3.5x that spread), its entropies to 1e-8 relative (the same
+# order as the charges' spread) and its parameters are not compared.
+TOLERANCE = {"c2_cluster24_lisa": dict(qtol=5e-8, ptol=np.inf, erel=1e-8)}
 
 
 @pytest.mark.parametrize("name", SINGLE)
@@ -80,7 +90,7 @@ def test_config_fp64(name):
     _check_inputs(case, fx)
     part = case.part()
     part.do_partitioning()
-    np.testing.assert_allclose(part["populations"], case.workload.numbers - fx["charges"], rtol=0, atol=1e-10)
+    tol = TOLERANCE.get(name, {})
     _compare(
         name,
         part["propars"],
@@ -89,7 +99,9 @@ def test_config_fp64(name):
         fx,
         inner=part.inner_niter,
         hist=(part["history_changes"], part["history_entropies"]),
+        **tol,
     )
+    np.testing.assert_allclose(part["populations"], case.workload.numbers - fx["charges"], rtol=0, atol=tol.get("qtol", 1e-10))
     part.close()
 
 

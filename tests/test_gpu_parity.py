@@ -413,3 +413,87 @@ def test_nlis_gmbis_chain(scheme):
     _compare(part, cache, oracle, qtol=1e-10, ptol=1e-8)
     np.testing.assert_allclose(part["valence_charges"], cache["valence_charges"], atol=1e-10)
     np.testing.assert_allclose(part["core_charges"], cache["core_charges"], atol=1e-10)
+
+
+# --- gLISA (glisa.py): the global problem ---------------------------------------------
+
+
+def _glisa_pair(w, basis_name, solver, **kw):
+    from horton_part_amd import GlobalLinearISAWPart
+    from oracle.horton_oracle import OracleGLISA
+
+    helper = _slater6(w.numbers) if basis_name == "slater6" else ExpBasisFuncHelper.from_function_type("gauss")
+    part = GlobalLinearISAWPart(w.coordinates, w.numbers, w.pseudo_numbers, w.grid, w.density, solver=solver,
+                                basis_func=helper, **kw)
+    okw = {k: v for k, v in kw.items() if k in ("threshold", "maxiter", "grid_type")}
+    oracle = OracleGLISA(w.coordinates, w.numbers, w.pseudo_numbers, w.grid, w.density,
+                         basis=OracleBasis(helper.orders, helper.exponents, helper.initials), solver=solver, **okw)
+    return part, oracle
+
+
+def test_glisa_global_terms_against_oracle():
+    """The device objective, gradient and MFMA Hessian of the global problem
+    (glisa.py:411-479) against the oracle's at the initial coefficients and at
+    perturbed ones."""
+    require_gpu()
+    _say("case: gLISA global terms, chain 5 atoms seed 11, 40x50 (gauss basis, M = 22)")
+    w = workloads.chain(natom=5, seed=11, nrad=40, nang=50)
+    part, oracle = _glisa_pair(w, "gauss", "sc")
+    c = part._init_propars()
+    oc = oracle.init_propars()
+    np.testing.assert_allclose(c, oc, rtol=1e-14)
+    dev = part._ensure_device()
+    rng = np.random.default_rng(5)
+    for trial in range(2):
+        x = c if trial == 0 else c * (1 + 0.3 * rng.uniform(-1, 1, size=c.size))
+        f, g, H, mn = dev.global_terms(x, nderiv=2)
+        pro = oracle.calc_promol_dens(x)
+        fo, go, Ho = oracle.working_matrix(oracle.moldens, pro, 2)
+        _say(f"  trial {trial}: |df|={abs(f - fo):.2e} max rel dgrad={np.max(np.abs(g - go) / np.abs(go)):.2e} "
+             f"max rel dH={np.max(np.abs(H - Ho) / np.maximum(np.abs(Ho), 1e-300)):.2e}")
+        assert f == pytest.approx(fo, rel=1e-12)
+        np.testing.assert_allclose(g, go, rtol=1e-12)
+        np.testing.assert_allclose(H, Ho, rtol=1e-11, atol=1e-13 * np.abs(Ho).max())
+        assert mn == pytest.approx(pro.min(), rel=1e-13)
+
+
+@pytest.mark.parametrize("grid_type", [1, 2, 3])
+def test_glisa_sc(grid_type):
+    """gLISA with the self-consistent solver (the reference's tested one,
+    tests/scripts/test_partition_density.py:156-199) against the oracle."""
+    require_gpu()
+    _say(f"case: gLISA sc grid_type {grid_type}, chain 3 atoms seed 5, 30x26")
+    w = workloads.chain(natom=3, seed=5, nrad=30, nang=26)
+    part, oracle = _glisa_pair(w, "slater6", "sc", grid_type=grid_type)
+    part.do_partitioning()
+    cache = oracle.do_partitioning()
+    dq = np.max(np.abs(part["charges"] - cache["charges"]))
+    rel = np.max(np.abs(part["propars"] - cache["propars"]) / np.abs(cache["propars"]))
+    _say(f"  niter gpu={part['niter']} oracle={cache['niter']} max|dq|={dq:.2e} rel dpar={rel:.2e}")
+    assert part["niter"] == cache["niter"]
+    assert dq <= 1e-10
+    assert rel <= 1e-8
+    np.testing.assert_allclose(part["history_changes"], cache["history_changes"], rtol=1e-6, atol=1e-14)
+    np.testing.assert_allclose(part["history_entropies"], cache["history_entropies"], rtol=1e-10)
+    for a in range(3):
+        np.testing.assert_allclose(part[f"at_weights_{a}"], oracle.cache[f"at_weights_{a}"], rtol=1e-10, atol=1e-14)
+
+
+def test_glisa_newton_failures_like_reference():
+    """m-newton's line search finds no valid step on these inputs, and the
+    device path raises the reference's RuntimeError where the oracle does."""
+    require_gpu()
+    _say("case: gLISA m-newton on the reference water fixture (expects RuntimeError: Line search failed!)")
+    coords, nums, pseudo, grid, dens = load_water_fixture()
+    from horton_part_amd import GlobalLinearISAWPart
+    from oracle.horton_oracle import OracleGLISA
+
+    h = ExpBasisFuncHelper.from_function_type("gauss")
+    oracle = OracleGLISA(coords, nums, pseudo, grid, dens, basis=OracleBasis(h.orders, h.exponents, h.initials),
+                         solver="m-newton")
+    with pytest.raises(RuntimeError, match="Line search failed"):
+        oracle.do_partitioning()
+    part = GlobalLinearISAWPart(coords, nums, pseudo, grid, dens, solver="m-newton", basis_func=h)
+    with pytest.raises(RuntimeError, match="Line search failed"):
+        part.do_partitioning()
+    assert len(part.history_changes) == len(oracle.history_changes)

@@ -495,3 +495,82 @@ def test_nlis_first_update_by_hand():
         expect += [m0, 3 / (m1 * n), n]
     got = ho.opt_nlis_propars(rho, pars.copy(), w, r, 1e200)
     np.testing.assert_allclose(got, expect, rtol=1e-13)
+
+
+# --- gLISA -----------------------------------------------------------------------
+
+
+def _glisa_small(solver="sc"):
+    w = workloads.chain(natom=2, seed=3, nrad=20, nang=26)
+    h = ExpBasisFuncHelper.from_function_type("gauss")
+    o = ho.OracleGLISA(w.coordinates, w.numbers, w.pseudo_numbers, w.grid, w.density,
+                       basis=ho.OracleBasis(h.orders, h.exponents, h.initials), solver=solver)
+    return o, o.init_propars()
+
+
+def test_glisa_working_matrix_finite_differences():
+    """glisa.py:411-479: the gradient and Hessian are the first and second
+    derivatives of f(c) = sum_p w rho ln(rho / rho0(c)) (central differences)."""
+    o, c = _glisa_small()
+    rho = o.moldens
+    f0, g, H = o.working_matrix(rho, o.calc_promol_dens(c), 2)
+    h = 1e-6
+    for i in range(len(c)):
+        e = np.zeros_like(c)
+        e[i] = h * max(abs(c[i]), 1e-3)
+        fp = o.working_matrix(rho, o.calc_promol_dens(c + e))
+        fm = o.working_matrix(rho, o.calc_promol_dens(c - e))
+        assert (fp - fm) / (2 * e[i]) == pytest.approx(g[i], rel=1e-6, abs=1e-8)
+        _, gp = o.working_matrix(rho, o.calc_promol_dens(c + e), 1)
+        _, gm = o.working_matrix(rho, o.calc_promol_dens(c - e), 1)
+        np.testing.assert_allclose((gp - gm) / (2 * e[i]), H[i], rtol=1e-5, atol=1e-7)
+
+
+def test_glisa_working_matrix_bruteforce():
+    """The Hessian sum written as explicit loops over points on a tiny grid."""
+    o, c = _glisa_small()
+    rho = o.moldens
+    pro = o.calc_promol_dens(c)
+    _, g, H = o.working_matrix(rho, pro, 2)
+    w = o.grid.weights
+    G = o.pro_shells
+    M, N = G.shape
+    for i in range(M):
+        gi = 0.0
+        for p in range(N):
+            if rho[p] >= 1e-15 and pro[p] >= 1e-15:
+                gi -= w[p] * rho[p] * G[i, p] / pro[p]
+        assert gi == pytest.approx(g[i], rel=1e-12)
+    for i, j in ((0, 0), (0, M - 1), (M // 2, M - 2)):
+        hij = sum(w[p] * rho[p] * G[i, p] * G[j, p] / pro[p] ** 2 for p in range(N) if rho[p] >= 1e-15 and pro[p] >= 1e-15)
+        assert hij == pytest.approx(H[i, j], rel=1e-12)
+
+
+def test_glisa_sc_fixed_point_conditions():
+    """At the converged self-consistent solution (glisa.py:805-879) every
+    coefficient that is not zero satisfies sum_p w rho g_i / rho0 = 1, i.e.
+    the gradient is -1 there, and the map returns its input."""
+    o, c = _glisa_small()
+    cache = o.do_partitioning()
+    x = cache["propars"]
+    _, g = o.working_matrix(o.moldens, o.calc_promol_dens(x), 1)
+    big = np.abs(x) > 1e-3
+    np.testing.assert_allclose(g[big], -1.0, atol=1e-4)
+    np.testing.assert_allclose(o.function_g(x), x, rtol=1e-4, atol=1e-6)
+    assert abs(cache["charges"].sum() - (o.numbers.sum() - o.grid.integrate(o.moldens))) < 1e-8
+
+
+def test_fix_propars_rule():
+    """utils.py:556-577: from the most diffuse function on, indices with
+    |c| < 1e-4 and a negative step, stopping at the first that is not."""
+    from horton_part_amd.glisa import fix_propars
+
+    exps = np.array([5.0, 0.5, 2.0, 0.1])
+    pars = np.array([1.0, 5e-5, 0.3, 0.0])
+    delta = np.array([-1.0, -0.2, -0.1, -0.5])
+    assert list(ho.fix_propars(exps, pars, delta)) == [3, 1]
+    assert list(fix_propars(exps, pars, delta)) == [3, 1]
+    delta[1] = 0.2
+    assert list(ho.fix_propars(exps, pars, delta)) == [3]
+    pars[3] = 1.0
+    assert list(ho.fix_propars(exps, pars, delta)) == []
