@@ -11,6 +11,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -543,6 +544,38 @@ int run_stage_gt2(hp_part *p, int precision, double thr, int min_stop, int max_i
 
 }  // namespace
 
+// ---- static workspace of hp_part_moments_static ------------------------------
+// Callers that cannot allocate device memory (code under stream capture,
+// serving loops) get the moments' partial sums in this file-scope array
+// instead of a buffer the library allocates.  Every GPU holds its own copy
+// until the process ends; host code reaches it only through the address that
+// hipGetSymbolAddress returns for the current device.
+#define HP_STATIC_WORKSPACE_DOUBLES (1 << 20)  // 8 MiB per GPU
+static __device__ __align__(256) double scratch[HP_STATIC_WORKSPACE_DOUBLES];
+
+namespace {
+
+constexpr int kMaxDevices = 64;
+std::mutex g_workspace_mu;
+double *g_workspace[kMaxDevices] = {};
+
+// The device address of `scratch` on `device` (which the caller made current),
+// looked up once per device.
+int static_workspace(int device, double **out) {
+  if (device < 0 || device >= kMaxDevices) return fail(HP_E_ARG, "device ordinal out of range");
+  std::lock_guard<std::mutex> lock(g_workspace_mu);
+  if (!g_workspace[device]) {
+    void *ptr = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+    if (e != hipSuccess) return fail(HP_E_HIP, std::string("hipGetSymbolAddress: ") + hipGetErrorString(e));
+    g_workspace[device] = static_cast<double *>(ptr);
+  }
+  *out = g_workspace[device];
+  return 0;
+}
+
+}  // namespace
+
 extern "C" {
 
 int hp_abi_version(void) { return HP_ABI_VERSION; }
@@ -1058,6 +1091,33 @@ int hp_part_moments(hp_part *p, int lmax, double *cart, double *pure, double *ra
     if (radial)
       for (int i = 0; i < 4; ++i) radial[a * 4 + i] = v[36 + i];
   }
+  return 0;
+}
+
+int hp_part_moments_static(hp_part *p, int lmax, double *cart, double *pure, double *radial) {
+  if (!p) return fail(HP_E_ARG, "NULL part");
+  if (lmax != 3) return fail(HP_E_UNSUPP, "the device moments are implemented for lmax = 3");
+  if (p->niter == 0 && !p->weights_ready) return fail(HP_E_ARG, "run the partitioning first");
+  // partials, per-atom sums, then cart | pure | radial of every atom
+  const size_t npart = moments_partials(p, p->only_molgrid);
+  const size_t need = (npart + 2 * (size_t)p->natom) * HP_NMOM;
+  if (need > (size_t)HP_STATIC_WORKSPACE_DOUBLES)
+    return fail(HP_E_ARG, "moments of this molecule need " + std::to_string(need) + " doubles of workspace, beyond the " +
+                              std::to_string(HP_STATIC_WORKSPACE_DOUBLES) +
+                              " of the static workspace ((partial rows + 2 natom) x 40 doubles); use hp_part_moments");
+  HP_HIP(hipSetDevice(p->device));
+  double *ws = nullptr;
+  if (int rc = static_workspace(p->device, &ws)) return rc;
+  double *part_buf = ws, *out_buf = ws + npart * HP_NMOM, *lay = out_buf + (size_t)p->natom * HP_NMOM;
+  if (int rc = launch_moments(p, part_buf, out_buf, p->only_molgrid)) return rc;
+  hipLaunchKernelGGL(k_moments_layout, dim3((p->natom + 63) / 64), dim3(64), 0, p->stream, p->natom, out_buf,
+                     p->pseudo, lay);
+  HP_LAUNCHED();
+  const size_t A = (size_t)p->natom;
+  if (cart) HP_HIP(hipMemcpyAsync(cart, lay, sizeof(double) * A * 20, hipMemcpyDeviceToHost, p->stream));
+  if (pure) HP_HIP(hipMemcpyAsync(pure, lay + A * 20, sizeof(double) * A * 16, hipMemcpyDeviceToHost, p->stream));
+  if (radial) HP_HIP(hipMemcpyAsync(radial, lay + A * 36, sizeof(double) * A * 4, hipMemcpyDeviceToHost, p->stream));
+  HP_HIP(hipStreamSynchronize(p->stream));
   return 0;
 }
 

@@ -2004,6 +2004,23 @@ static __global__ void k_moments_reduce_mol(int nchunk, const double *__restrict
   out[(int64_t)a * HP_NMOM + i] = v;
 }
 
+// Final layout of the moments from the per-atom sums: cart (natom x 20),
+// pure (natom x 16), radial (natom x 4) one after the other, the electron
+// sign applied and pseudo_numbers added to the monopoles (core/base.py:374-402).
+static __global__ void k_moments_layout(int natom, const double *__restrict__ mom, const double *__restrict__ pseudo,
+                                        double *__restrict__ out) {
+  const int a = blockIdx.x * blockDim.x + threadIdx.x;
+  if (a >= natom) return;
+  const double *v = mom + (int64_t)a * HP_NMOM;
+  double *cart = out + (int64_t)a * 20, *pure = out + (int64_t)natom * 20 + (int64_t)a * 16,
+         *rad = out + (int64_t)natom * 36 + (int64_t)a * 4;
+  for (int i = 0; i < 20; ++i) cart[i] = -v[i];
+  cart[0] += pseudo[a];
+  for (int i = 0; i < 16; ++i) pure[i] = -v[20 + i];
+  pure[0] += pseudo[a];
+  for (int i = 0; i < 4; ++i) rad[i] = v[36 + i];
+}
+
 // Sum of the per-shell partials (one workgroup of 256 threads): the entropy
 // of a weights pass, in the order k_finalize sums it.
 static __global__ __launch_bounds__(256) void k_sum_shells(int nshell, const double *__restrict__ part,

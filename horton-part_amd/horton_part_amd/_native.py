@@ -53,6 +53,7 @@ EXPORTED_SYMBOLS = [
     "hp_part_at_weights",
     "hp_part_promoldens",
     "hp_part_moments",
+    "hp_part_moments_static",
     "hp_part_global_terms",
     "hp_part_global_proatoms",
     "hp_part_populations",
@@ -239,6 +240,7 @@ def load():
     lib.hp_part_at_weights.argtypes = [ctypes.c_void_p, ctypes.c_int, _pd, ctypes.c_int64]
     lib.hp_part_promoldens.argtypes = [ctypes.c_void_p, _pd, ctypes.c_int64]
     lib.hp_part_moments.argtypes = [ctypes.c_void_p, ctypes.c_int, _pd, _pd, _pd]
+    lib.hp_part_moments_static.argtypes = [ctypes.c_void_p, ctypes.c_int, _pd, _pd, _pd]
     lib.hp_part_kernel_time.argtypes = [ctypes.c_void_p, ctypes.c_int, _pd, _pi64]
     lib.hp_part_global_terms.argtypes = [ctypes.c_void_p, _pd, ctypes.c_int, _pd, _pd, _pd, _pd]
     lib.hp_part_global_proatoms.argtypes = [ctypes.c_void_p, _pd, _pd, _pd, _pd]

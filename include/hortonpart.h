@@ -214,6 +214,17 @@ int hp_part_promoldens(hp_part *part, double *out, int64_t n);
  * molecular weights (local = False, core/base.py:455-459). */
 int hp_part_moments(hp_part *part, int lmax, double *cartesian, double *pure, double *radial);
 
+/* hp_part_moments without allocating: the same kernels, with their partial
+ * sums in a static device workspace of 2^20 doubles (8 MiB) that each GPU
+ * holds until the process ends.  No counterpart in the reference: it is for
+ * callers that cannot allocate device memory (code under stream capture,
+ * serving loops).  Molecules whose moments need more than the workspace,
+ * (partial rows + 2 natom) x 40 doubles with one partial row per radial
+ * shell (grid_type 3: per atom and 4096-point chunk), get HP_E_ARG.  Every
+ * call on one GPU uses that GPU's workspace: calls that overlap on a GPU
+ * (several host threads) race, so serialise them. */
+int hp_part_moments_static(hp_part *part, int lmax, double *cartesian, double *pure, double *radial);
+
 /* ---- gLISA: the global problem (glisa.py:55-1028) ----
  * A partitioning created with HP_SCHEME_LISA and grid_type 2 or 3 holds the
  * basis functions g_i (fn_alpha, fn_order, fn_norm) at unit coefficient.
