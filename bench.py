@@ -255,8 +255,10 @@ def serve_stream(server, reqs, builder, policy, realtime=True, queue_max=6, prog
     builder.fill(0)
     t0 = time.perf_counter()
     next_report = 0.1
+    host = {"step_s": 0.0, "poll_s": 0.0, "submit_s": 0.0, "idle_s": 0.0, "steps": 0}
     while len(done) < n:
         now = time.perf_counter() - t0
+        ts = time.perf_counter()
         while (i < n and (not realtime or arrival[i] <= now) and builder.ready(i)
                and server.n_queued < queue_max):
             it = builder.take(i)
@@ -264,16 +266,24 @@ def serve_stream(server, reqs, builder, policy, realtime=True, queue_max=6, prog
             live[i] = it
             i += 1
             builder.fill(i)
+        host["submit_s"] += time.perf_counter() - ts
         if server.pending == 0:
             time.sleep(0.0005)
+            host["idle_s"] += time.perf_counter() - ts
             continue
+        ts = time.perf_counter()
         server.step()
+        tp = time.perf_counter()
+        host["step_s"] += tp - ts
+        host["steps"] += 1
         for d in server.poll():
             done[d.tag] = (arrival[d.tag], time.perf_counter() - t0, d)
             live.pop(d.tag, None)
+        host["poll_s"] += time.perf_counter() - tp
         if progress and len(done) >= next_report * n:
             progress(len(done), n, time.perf_counter() - t0, i)
             next_report += 0.1
+    serve_stream.host = host
     return [done[k] for k in range(n)]
 
 
@@ -508,6 +518,7 @@ def run_serving(args):
         "max_dq_vs_fp64": dq,
         "max_dq_scope": f"the {len(fp64)} sampled requests: served result vs the FP64 path one at a time",
         "dq_over_1e-6": over,
+        "host_loop": serve_stream.host,
         "batch": {
             "ticks": d["ticks"],
             "molecule_iterations": d["iterations"],
@@ -567,6 +578,8 @@ def main():
     ap.add_argument("--arena-points", type=int, default=96_000_000, help="serving: grid points resident at once")
     ap.add_argument("--max-inflight", type=int, default=256, help="serving: molecules resident at once")
     ap.add_argument("--fp64-every", type=int, default=64, help="serving: FP64 one-at-a-time sample: every n-th request")
+    ap.add_argument("--switch-interval", type=float, default=0.0,
+                    help="serving: interpreter thread switch interval in seconds (0: the default)")
     ap.add_argument("--workload", default="serving",
                     choices=["serving", "water", "chain20", "chain20-gisa", "cluster200", "protein2000"])
     ap.add_argument("--ticks", type=int, default=8, help="serving: outer iterations per host step")
@@ -587,6 +600,10 @@ def main():
     if _native.device_count() < 1:
         raise SystemExit("no GPU visible")
     if args.workload == "serving":
+        if args.switch_interval > 0:
+            # the loop that steps the batch gets the interpreter back sooner
+            # from the threads that build requests
+            sys.setswitchinterval(args.switch_interval)
         args.steps = 20 if args.steps is None else args.steps
         args.warmup = 4 if args.warmup is None else args.warmup
         return run_serving(args)
