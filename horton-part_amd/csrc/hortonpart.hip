@@ -20,6 +20,7 @@
 #include "hp_glisa.h"
 #include "hp_gt2.h"
 #include "hp_host.h"
+#include "hp_table.h"
 
 namespace {
 
@@ -97,6 +98,14 @@ struct hp_part {
          *gl_hess = nullptr, *gl_out = nullptr, *gl_coef = nullptr, *gl_pro = nullptr;
   int2 *gl_pairs = nullptr;
   double *corr = nullptr;     // per radial shell: FP64 - FP32 spherical average
+  // tabulated pro-atoms (ISA, HI, Hirshfeld): knots, their radial weights,
+  // the current cubic pieces, HI's current knot densities and records, and
+  // the per-shell atom-grid populations of the last weights pass
+  int ntab = 0;
+  int *tab_off = nullptr, *rec_off = nullptr, *rec_charge = nullptr;
+  int64_t *rec_val = nullptr;
+  double *tab_x = nullptr, *tab_w = nullptr, *tab_c = nullptr, *cur_rho = nullptr, *rec_rho = nullptr,
+         *rec_c = nullptr, *pop_part = nullptr;
   // host state
   int niter = 0;
   double last_change = 0.0, last_entropy = 0.0;
@@ -143,7 +152,22 @@ struct hp_part {
     // MBIS and GISA updates keep non-negative coefficients non-negative
     f.tail_cull = scheme != HP_SCHEME_LISA && fnkind != HP_FN_GENERAL && tail_cull && params_nonneg;
     f.evals_done = evals_done;
+    f.tab_off = tab_off;
+    f.tab_x = tab_x;
+    f.tab_c = tab_c;
     return f;
+  }
+  DevRec rec() const {
+    DevRec r;
+    r.rec_off = rec_off;
+    r.rec_charge = rec_charge;
+    r.rec_val = rec_val;
+    r.rec_rho = rec_rho;
+    r.rec_c = rec_c;
+    return r;
+  }
+  bool tabulated() const {
+    return scheme == HP_SCHEME_ISA || scheme == HP_SCHEME_HI || scheme == HP_SCHEME_HIRSHFELD;
   }
 };
 
@@ -166,7 +190,8 @@ void free_part(hp_part *p) {
                   p->mom_out, p->scratch_pts, p->inner_niter, p->status, p->status_all, p->hpar,    p->hch,
                   p->hent,    p->hchg,    p->corr,      p->evals_done, p->grp_first, p->gl_G, p->gl_D,
                   p->gl_E,    p->gl_obj,  p->gl_grad,   p->gl_hess,  p->gl_out,    p->gl_coef, p->gl_pro,
-                  p->gl_pairs};
+                  p->gl_pairs, p->tab_off, p->rec_off, p->rec_charge, p->rec_val, p->tab_x, p->tab_w,
+                  p->tab_c,   p->cur_rho, p->rec_rho,   p->rec_c,    p->pop_part};
   for (void *b : bufs)
     if (b) (void)hipFree(b);
   if (p->h_out) (void)hipHostFree(p->h_out);
@@ -342,6 +367,48 @@ int enqueue_iteration(hp_part *p, int precision, double thr, int min_stop, int m
   return 0;
 }
 
+// Build the tabulated pro-atoms from the parameters (ISA, HI; Hirshfeld's
+// tables are fixed at creation).
+void launch_table_begin(hp_part *p) {
+  hipStream_t s = p->stream;
+  if (p->scheme == HP_SCHEME_ISA)
+    hipLaunchKernelGGL(k_isa_begin, dim3(p->natom), dim3(64), 0, s, p->state, p->tab_off, p->tab_x, p->par, p->old_par,
+                       p->tab_c);
+  else if (p->scheme == HP_SCHEME_HI)
+    hipLaunchKernelGGL(k_hi_begin, dim3(p->natom), dim3(64), 0, s, p->state, p->natom, p->tab_off, p->rec(), p->pseudo,
+                       p->par, p->old_par, p->tab_c, p->cur_rho, p->status);
+}
+
+void launch_table_weights(hp_part *p) {
+  hipLaunchKernelGGL(k_table_weights, dim3(p->nshell), dim3(256), 0, p->stream, p->state, p->mol(), p->fn(), p->dc,
+                     p->rhobar, p->pop_part, p->ent_part, p->promol);
+}
+
+// One outer iteration of ISA or HI: tables, weights pass, per-atom update,
+// stop test (the kernels skip their work once the device flag is set).
+int enqueue_iteration_table(hp_part *p, double thr, int min_stop, int max_iter, int slot) {
+  hipStream_t s = p->stream;
+  launch_table_begin(p);
+  HP_LAUNCHED();
+  HP_HIP(hipEventRecord(p->slot_ev[slot][0], s));
+  launch_table_weights(p);
+  HP_LAUNCHED();
+  HP_HIP(hipEventRecord(p->slot_ev[slot][1], s));
+  if (p->scheme == HP_SCHEME_ISA)
+    hipLaunchKernelGGL(k_isa_update, dim3(p->natom), dim3(64), 0, s, p->state, p->mol(), p->tab_off, p->tab_x, p->tab_w,
+                       p->rhobar, p->par, p->old_par, p->charges, p->change_part, p->status);
+  else
+    hipLaunchKernelGGL(k_hi_update, dim3(p->natom), dim3(64), 0, s, p->state, p->mol(), p->tab_off, p->tab_x, p->tab_w,
+                       p->rec(), p->pop_part, p->cur_rho, p->par, p->charges, p->change_part, p->status);
+  HP_LAUNCHED();
+  HP_HIP(hipEventRecord(p->slot_ev[slot][2], s));
+  hipLaunchKernelGGL(k_finalize, dim3(1), dim3(256), 0, s, p->state, thr, min_stop, max_iter, p->nshell, p->ent_part,
+                     p->natom, p->change_part, p->status, p->status_all, p->par, p->npar, p->charges, p->hpar, p->hch,
+                     p->hent, p->hchg, p->out, p->old_par, 0);
+  HP_LAUNCHED();
+  return 0;
+}
+
 // Run iterations until the device stop test fires: batches of speculative
 // iterations, one host synchronisation per batch.
 int run_stage(hp_part *p, int precision, double thr, int min_stop, int max_iter, int rollback = 0) {
@@ -355,7 +422,8 @@ int run_stage(hp_part *p, int precision, double thr, int min_stop, int max_iter,
     for (int c = 0; c < nb; ++c) {
       // iteration before + c, if the loop has not stopped by then
       const bool refresh = p->mixed_refresh > 0 && (before + c) % p->mixed_refresh == 0;
-      int rc = enqueue_iteration(p, precision, thr, min_stop, max_iter, c, refresh, rollback);
+      int rc = p->tabulated() ? enqueue_iteration_table(p, thr, min_stop, max_iter, c)
+                              : enqueue_iteration(p, precision, thr, min_stop, max_iter, c, refresh, rollback);
       if (rc) return rc;
     }
     HP_HIP(hipMemcpyAsync(p->h_out, p->out, 4 * sizeof(double), hipMemcpyDeviceToHost, s));
@@ -504,7 +572,9 @@ int launch_moments(hp_part *p, double *part_buf, double *out_buf, bool molgrid) 
   if (molgrid) {
     const int nchunk = (int)((p->npoint + HP_MOM_CHUNK - 1) / HP_MOM_CHUNK);
     const dim3 g((unsigned)nchunk, (unsigned)p->natom);
-    if (p->fnkind == HP_FN_SLATER)
+    if (p->fnkind == HP_FN_TABLE)
+      hipLaunchKernelGGL((k_moments_mol<HP_FN_TABLE>), g, dim3(256), 0, p->stream, p->mol(), f, p->promol, part_buf);
+    else if (p->fnkind == HP_FN_SLATER)
       hipLaunchKernelGGL((k_moments_mol<HP_FN_SLATER>), g, dim3(256), 0, p->stream, p->mol(), f, p->promol, part_buf);
     else if (p->fnkind == HP_FN_GAUSS)
       hipLaunchKernelGGL((k_moments_mol<HP_FN_GAUSS>), g, dim3(256), 0, p->stream, p->mol(), f, p->promol, part_buf);
@@ -513,7 +583,9 @@ int launch_moments(hp_part *p, double *part_buf, double *out_buf, bool molgrid) 
     HP_LAUNCHED();
     hipLaunchKernelGGL(k_moments_reduce_mol, dim3(p->natom), dim3(64), 0, p->stream, nchunk, part_buf, out_buf);
   } else {
-    if (p->fnkind == HP_FN_SLATER)
+    if (p->fnkind == HP_FN_TABLE)
+      hipLaunchKernelGGL((k_moments<HP_FN_TABLE>), dim3(p->nshell), dim3(256), 0, p->stream, p->mol(), f, part_buf);
+    else if (p->fnkind == HP_FN_SLATER)
       hipLaunchKernelGGL((k_moments<HP_FN_SLATER>), dim3(p->nshell), dim3(256), 0, p->stream, p->mol(), f, part_buf);
     else if (p->fnkind == HP_FN_GAUSS)
       hipLaunchKernelGGL((k_moments<HP_FN_GAUSS>), dim3(p->nshell), dim3(256), 0, p->stream, p->mol(), f, part_buf);
@@ -605,10 +677,13 @@ int hp_part_create(int device, const hp_molecule_desc *md, const hp_scheme_desc 
       !md->atom_weights || !md->sph_weights || !md->shell_atom || !md->shell_offsets || !md->shell_radius ||
       !md->shell_rweight)
     return fail(HP_E_ARG, "molecule description has a NULL array");
-  if (sd->scheme < 0 || sd->scheme > 3) return fail(HP_E_ARG, "unknown scheme");
+  if (sd->scheme < 0 || sd->scheme > HP_SCHEME_HIRSHFELD) return fail(HP_E_ARG, "unknown scheme");
   if (sd->grid_type < 1 || sd->grid_type > 3) return fail(HP_E_ARG, "grid_type must be 1, 2 or 3");
   if (sd->grid_type >= 2 && sd->check_mono) return fail(HP_E_UNSUPP, "check_mono is defined on radial grids (grid_type 1)");
-  if (!sd->fn_offsets) return fail(HP_E_ARG, "fn_offsets is NULL");
+  const bool table = sd->scheme >= HP_SCHEME_ISA;
+  if (table && sd->grid_type != 1 && sd->scheme != HP_SCHEME_HIRSHFELD)
+    return fail(HP_E_UNSUPP, "ISA and Hirshfeld-I run on atom grids (grid_type 1)");
+  if (!table && !sd->fn_offsets) return fail(HP_E_ARG, "fn_offsets is NULL");
   const bool mbis = sd->scheme == HP_SCHEME_MBIS;
   if (mbis && sd->solver != HP_SOLVER_MBIS) return fail(HP_E_ARG, "MBIS uses HP_SOLVER_MBIS");
   if (sd->scheme == HP_SCHEME_GISA && sd->solver != HP_SOLVER_QP) return fail(HP_E_ARG, "GISA uses HP_SOLVER_QP");
@@ -619,8 +694,9 @@ int hp_part_create(int device, const hp_molecule_desc *md, const hp_scheme_desc 
   if (sd->scheme == HP_SCHEME_LISA &&
       !(sd->solver == HP_SOLVER_SC || sd->solver == HP_SOLVER_NEWTON || sd->solver == HP_SOLVER_MNEWTON))
     return fail(HP_E_UNSUPP, "LISA solver not available on the device");
-  if (sd->solver == HP_SOLVER_QP && !sd->qp_matrix) return fail(HP_E_ARG, "qp_matrix is NULL");
-  if (!mbis && !nlis && (!sd->fn_alpha || !sd->fn_order || !sd->fn_norm)) return fail(HP_E_ARG, "basis arrays are NULL");
+  if (!table && sd->solver == HP_SOLVER_QP && !sd->qp_matrix) return fail(HP_E_ARG, "qp_matrix is NULL");
+  if (!table && !mbis && !nlis && (!sd->fn_alpha || !sd->fn_order || !sd->fn_norm))
+    return fail(HP_E_ARG, "basis arrays are NULL");
   if (sd->maxiter < 1 || sd->inner_maxiter < 1) return fail(HP_E_ARG, "iteration caps must be positive");
   if (sd->solver == HP_SOLVER_MNEWTON && sd->newton_linspace < 2) return fail(HP_E_ARG, "linspace size < 2");
 
@@ -649,12 +725,52 @@ int hp_part_create(int device, const hp_molecule_desc *md, const hp_scheme_desc 
   if (maxL > 1024) return fail(HP_E_UNSUPP, "more than 1024 points in one radial shell");
   for (int a = 0; a < A; ++a) maxR = std::max(maxR, atom_shell_off[a + 1] - atom_shell_off[a]);
   if (pick_ppl(maxR) == 0) return fail(HP_E_UNSUPP, "more than 512 radial points per atom");
-  const int nfn = sd->fn_offsets[A];
-  if (sd->fn_offsets[0] != 0) return fail(HP_E_ARG, "fn_offsets[0] must be 0");
-  for (int a = 0; a < A; ++a) {
+  const int nfn = table ? 0 : sd->fn_offsets[A];
+  if (!table && sd->fn_offsets[0] != 0) return fail(HP_E_ARG, "fn_offsets[0] must be 0");
+  for (int a = 0; a < A && !table; ++a) {
     const int K = sd->fn_offsets[a + 1] - sd->fn_offsets[a];
     if (K <= 0) return fail(HP_E_ARG, "every atom needs at least one function");
     maxK = std::max(maxK, K);
+  }
+  // tabulated pro-atoms: knots per atom, records per atom and their values
+  int ntab = 0;
+  std::vector<int64_t> rec_val;
+  if (table) {
+    if (!sd->tab_offsets || !sd->tab_knots || !sd->tab_rweights) return fail(HP_E_ARG, "table arrays are NULL");
+    if (sd->tab_offsets[0] != 0) return fail(HP_E_ARG, "tab_offsets[0] must be 0");
+    for (int a = 0; a < A; ++a) {
+      const int k0 = sd->tab_offsets[a], n = sd->tab_offsets[a + 1] - k0;
+      const int nmin = sd->scheme == HP_SCHEME_ISA ? 4 : 2;
+      if (n < nmin || n > HP_TAB_MAXN)
+        return fail(n < nmin ? HP_E_ARG : HP_E_UNSUPP, "knots per atom out of range (ISA 4, otherwise 2, to 512)");
+      for (int i = 1; i < n; ++i)
+        if (!(sd->tab_knots[k0 + i] > sd->tab_knots[k0 + i - 1])) return fail(HP_E_ARG, "knots must ascend");
+      if (sd->scheme == HP_SCHEME_ISA) {
+        // the spherical averages of the atom's shells are the values at its knots
+        const int s0 = atom_shell_off[a];
+        if (n != atom_shell_off[a + 1] - s0) return fail(HP_E_ARG, "ISA knots must be the atom's radial shells");
+        for (int i = 0; i < n; ++i)
+          if (sd->tab_knots[k0 + i] != md->shell_radius[s0 + i])
+            return fail(HP_E_ARG, "ISA knots must be the atom's radial shells");
+      }
+    }
+    ntab = sd->tab_offsets[A];
+    if (sd->scheme != HP_SCHEME_ISA) {
+      if (!sd->rec_offsets || !sd->rec_charges || !sd->rec_rho || !sd->rec_coefs)
+        return fail(HP_E_ARG, "record arrays are NULL");
+      if (sd->rec_offsets[0] != 0) return fail(HP_E_ARG, "rec_offsets[0] must be 0");
+      int64_t v = 0;
+      for (int a = 0; a < A; ++a) {
+        const int nr = sd->rec_offsets[a + 1] - sd->rec_offsets[a];
+        if (nr < 1) return fail(HP_E_ARG, "every atom needs at least one record");
+        if (sd->scheme == HP_SCHEME_HIRSHFELD && nr != 1) return fail(HP_E_ARG, "Hirshfeld takes one record per atom");
+        const int n = sd->tab_offsets[a + 1] - sd->tab_offsets[a];
+        for (int j = 0; j < nr; ++j) {
+          rec_val.push_back(v);
+          v += n;
+        }
+      }
+    }
   }
   if (mbis && maxK > 7) return fail(HP_E_UNSUPP, "MBIS supports at most 7 shells per atom");
   if (nlis && maxK > HP_NLIS_KM) return fail(HP_E_UNSUPP, "NLIS supports at most 8 shells per atom");
@@ -662,8 +778,8 @@ int hp_part_create(int device, const hp_molecule_desc *md, const hp_scheme_desc 
   if (sd->grid_type >= 2 && sd->solver == HP_SOLVER_MNEWTON && sd->newton_linspace > HP_GT2_NLIN)
     return fail(HP_E_UNSUPP, "grid_type 2 line search supports at most 32 candidates");
   if (maxK > HP_KMAX) return fail(HP_E_UNSUPP, "too many functions per atom");
-  int fnkind = HP_FN_SLATER;
-  if (!mbis) {
+  int fnkind = table ? HP_FN_TABLE : HP_FN_SLATER;
+  if (!mbis && !table) {
     bool all1 = true, all2 = true;
     for (int f = 0; f < nfn; ++f) {
       if (!(sd->fn_order[f] > 0.0) || !(nlis || sd->fn_alpha[f] >= 0.0))
@@ -680,7 +796,8 @@ int hp_part_create(int device, const hp_molecule_desc *md, const hp_scheme_desc 
   p->nshell = S;
   p->npoint = N;
   p->nfn = nfn;
-  p->npar = mbis ? 2 * nfn : (nlis ? 3 * nfn : nfn);
+  p->npar = mbis ? 2 * nfn : (nlis ? 3 * nfn : (table ? (sd->scheme == HP_SCHEME_ISA ? ntab : A) : nfn));
+  p->ntab = ntab;
   p->scheme = sd->scheme;
   p->solver = sd->solver;
   // grid_type 3 runs the molecular-grid loop of grid_type 2 (core/base.py:490-499:
@@ -755,8 +872,31 @@ int hp_part_create(int device, const hp_molecule_desc *md, const hp_scheme_desc 
       (rc = dupload(&p->shell_off, md->shell_offsets, S + 1, s)) ||
       (rc = dupload(&p->atom_shell_off, atom_shell_off.data(), A + 1, s)))
     return bail(rc);
-  std::vector<int> fn_off(sd->fn_offsets, sd->fn_offsets + A + 1);
+  std::vector<int> fn_off(A + 1, 0);
+  if (!table) fn_off.assign(sd->fn_offsets, sd->fn_offsets + A + 1);
   if ((rc = dupload(&p->fn_off, fn_off.data(), A + 1, s))) return bail(rc);
+  if (table) {
+    std::vector<int> toff(sd->tab_offsets, sd->tab_offsets + A + 1);
+    if ((rc = dupload(&p->tab_off, toff.data(), A + 1, s)) || (rc = dupload(&p->tab_x, sd->tab_knots, ntab, s)) ||
+        (rc = dupload(&p->tab_w, sd->tab_rweights, ntab, s)) || (rc = dalloc(&p->tab_c, (size_t)4 * ntab)) ||
+        (rc = dalloc(&p->cur_rho, ntab)) || (rc = dalloc(&p->pop_part, S)))
+      return bail(rc);
+    if (sd->scheme != HP_SCHEME_ISA) {
+      const int nrec = sd->rec_offsets[A];
+      const int64_t nval = rec_val.empty() ? 0 : rec_val.back() + (sd->tab_offsets[A] - sd->tab_offsets[A - 1]);
+      std::vector<int> roff(sd->rec_offsets, sd->rec_offsets + A + 1), rch(sd->rec_charges, sd->rec_charges + nrec);
+      if ((rc = dupload(&p->rec_off, roff.data(), A + 1, s)) || (rc = dupload(&p->rec_charge, rch.data(), nrec, s)) ||
+          (rc = dupload(&p->rec_val, rec_val.data(), rec_val.size(), s)) ||
+          (rc = dupload(&p->rec_rho, sd->rec_rho, nval, s)) || (rc = dupload(&p->rec_c, sd->rec_coefs, 4 * nval, s)))
+        return bail(rc);
+      if (sd->scheme == HP_SCHEME_HIRSHFELD) {
+        // the fixed pro-atoms: record a is atom a's table
+        const hipError_t ce =
+            hipMemcpyAsync(p->tab_c, p->rec_c, sizeof(double) * 4 * (size_t)ntab, hipMemcpyDeviceToDevice, s);
+        if (ce != hipSuccess) return bail(fail(HP_E_HIP, std::string("table copy: ") + hipGetErrorString(ce)));
+      }
+    }
+  }
   const std::vector<int> grp_first = hp::weights_groups(atom_shell_off, p->wshape.g);
   p->ngroup = (int)grp_first.size();
   if ((rc = dupload(&p->grp_first, grp_first.data(), grp_first.size(), s))) return bail(rc);
@@ -835,7 +975,8 @@ int hp_part_create(int device, const hp_molecule_desc *md, const hp_scheme_desc 
     }
     p->evals_per_pass = ev;
   } else {
-    p->evals_per_pass = (double)N * (double)nfn;
+    // tabulated pro-atoms: one spline evaluation per point and atom
+    p->evals_per_pass = (double)N * (double)(table ? A : nfn);
   }
   // state
   p->hist_cap = 2 * p->maxiter + 64;
@@ -848,7 +989,10 @@ int hp_part_create(int device, const hp_molecule_desc *md, const hp_scheme_desc 
       (rc = dalloc(&p->hent, p->hist_cap)) || (rc = dalloc(&p->hchg, p->hist_cap)) || (rc = dalloc(&p->corr, S)) ||
       (rc = dalloc(&p->evals_done, HP_EVAL_SLOTS)))
     return bail(rc);
-  if (p->grid_type == 2) {
+  if (p->grid_type == 2 && table) {
+    // Hirshfeld on the molecular grid: the promolecule for the populations
+    if ((rc = dalloc(&p->promol, N))) return bail(rc);
+  } else if (p->grid_type == 2) {
     p->gt2_nchunk = (int)((N + HP_GT2_CHUNK - 1) / HP_GT2_CHUNK);
     if ((rc = dalloc(&p->promol, N)) || (rc = dalloc(&p->gt2, A)) ||
         (rc = dalloc(&p->gt2_partial, (size_t)A * p->gt2_nchunk * HP_GT2_NV)) || (rc = dalloc(&p->gt2_active, 1)) ||
@@ -899,19 +1043,31 @@ int hp_part_weights(hp_part *p, double *rhobar, double *entropy) {
   hipStream_t s = p->stream;
   // the kernels skip their work while the device stop flag is set
   HP_HIP(hipMemsetAsync(p->state, 0, sizeof(int), s));
-  const int n = std::max(p->npar, p->nfn);
-  hipLaunchKernelGGL(k_iter_begin, dim3((n + 255) / 256), dim3(256), 0, s, p->state, table_kind(p),
-                     p->npar, p->nfn, p->par, p->old_par, p->fn_norm, p->fn_alpha, p->coef, p->alpha);
-  HP_LAUNCHED();
-  HP_HIP(hipEventRecord(p->ev[0], s));
-  launch_weights<double>(p, p->grid_type == 2);
+  if (p->tabulated()) {
+    HP_HIP(hipMemsetAsync(p->status, 0, sizeof(int) * p->natom, s));
+    launch_table_begin(p);
+    HP_LAUNCHED();
+    HP_HIP(hipEventRecord(p->ev[0], s));
+    launch_table_weights(p);
+  } else {
+    const int n = std::max(p->npar, p->nfn);
+    hipLaunchKernelGGL(k_iter_begin, dim3((n + 255) / 256), dim3(256), 0, s, p->state, table_kind(p),
+                       p->npar, p->nfn, p->par, p->old_par, p->fn_norm, p->fn_alpha, p->coef, p->alpha);
+    HP_LAUNCHED();
+    HP_HIP(hipEventRecord(p->ev[0], s));
+    launch_weights<double>(p, p->grid_type == 2);
+  }
   HP_LAUNCHED();
   HP_HIP(hipEventRecord(p->ev[1], s));
   hipLaunchKernelGGL(k_sum_shells, dim3(1), dim3(256), 0, s, p->nshell, p->ent_part, p->out + 1);
   HP_LAUNCHED();
   if (rhobar) HP_HIP(hipMemcpyAsync(rhobar, p->rhobar, sizeof(double) * p->nshell, hipMemcpyDeviceToHost, s));
   HP_HIP(hipMemcpyAsync(p->h_out + 1, p->out + 1, sizeof(double), hipMemcpyDeviceToHost, s));
+  std::vector<int> st(p->tabulated() ? p->natom : 0);
+  if (!st.empty()) HP_HIP(hipMemcpyAsync(st.data(), p->status, sizeof(int) * st.size(), hipMemcpyDeviceToHost, s));
   HP_HIP(hipStreamSynchronize(s));
+  for (int v : st)
+    if (v & HP_ST_NORECORD) return fail(HP_E_SOLVER, "no pro-atom for an atom's charge; see hp_part_status");
   float ms = 0.f;
   HP_HIP(hipEventElapsedTime(&ms, p->ev[0], p->ev[1]));
   p->k_ms[HP_KERNEL_WEIGHTS] += ms;
@@ -928,6 +1084,8 @@ int hp_part_run(hp_part *p, int precision, int n_polish, int *niter) {
   // the histories hold 2*maxiter + 64 iterations
   if (precision == HP_PREC_MIXED && (n_polish < 0 || n_polish > p->maxiter + 64))
     return fail(HP_E_ARG, "n_polish must be in [0, maxiter + 64]");
+  if (p->scheme == HP_SCHEME_HIRSHFELD) return fail(HP_E_UNSUPP, "Hirshfeld does not iterate: use hp_part_weights");
+  if (p->tabulated() && precision != HP_PREC_FP64) return fail(HP_E_UNSUPP, "ISA and Hirshfeld-I run in FP64");
   HP_HIP(hipSetDevice(p->device));
   for (int k = 0; k < 3; ++k) {
     p->k_ms[k] = 0.0;
@@ -1026,7 +1184,10 @@ static int point_weights(hp_part *p, int atom, int64_t p0, int64_t n, double *ou
   }
   const dim3 grid((unsigned)((n + 255) / 256)), block(256);
   DevFn f = p->fn();
-  if (p->fnkind == HP_FN_SLATER)
+  if (p->fnkind == HP_FN_TABLE)
+    hipLaunchKernelGGL((k_point_weights<HP_FN_TABLE>), grid, block, 0, p->stream, p->mol(), f, atom, p0, n,
+                       p->scratch_pts);
+  else if (p->fnkind == HP_FN_SLATER)
     hipLaunchKernelGGL((k_point_weights<HP_FN_SLATER>), grid, block, 0, p->stream, p->mol(), f, atom, p0, n,
                        p->scratch_pts);
   else if (p->fnkind == HP_FN_GAUSS)
@@ -1233,9 +1394,23 @@ int hp_part_global_proatoms(hp_part *p, const double *coefs_new, const double *c
 
 int hp_part_populations(hp_part *p, double *pop) {
   if (!p || !pop) return fail(HP_E_ARG, "NULL argument");
-  if (p->grid_type != 2) return fail(HP_E_UNSUPP, "molecular-grid populations need grid_type 2 or 3");
+  if (p->grid_type != 2 && !p->tabulated()) return fail(HP_E_UNSUPP, "molecular-grid populations need grid_type 2 or 3");
   if (p->niter == 0 && !p->weights_ready) return fail(HP_E_ARG, "run a weights pass first");
   HP_HIP(hipSetDevice(p->device));
+  if (p->tabulated() && !p->only_molgrid) {
+    // atom-grid populations of the last tabulated weights pass, summed per
+    // atom over its shells in order (grid_type 2 keeps the weights on the
+    // molecular grid but integrates on the atom grids, core/base.py:293-299)
+    std::vector<double> part(p->nshell);
+    HP_HIP(hipMemcpyAsync(part.data(), p->pop_part, sizeof(double) * part.size(), hipMemcpyDeviceToHost, p->stream));
+    HP_HIP(hipStreamSynchronize(p->stream));
+    for (int a = 0; a < p->natom; ++a) {
+      double v = 0.0;
+      for (int si = p->h_atom_shell_off[a]; si < p->h_atom_shell_off[a + 1]; ++si) v += part[si];
+      pop[a] = v;
+    }
+    return 0;
+  }
   double *part = nullptr, *out = nullptr;
   const size_t npart = moments_partials(p, true);
   int rc = 0;

@@ -23,6 +23,7 @@
 
 #define HP_HD __host__ __device__
 #include "hp_exp_tab.h"
+#include "hp_spline.h"
 
 #define HP_KMAX 16      // functions per atom handled by the per-atom kernels
 
@@ -37,6 +38,7 @@
 #define HP_ST_NOTMONO 128
 #define HP_ST_SINGULAR 256
 #define HP_ST_NONFINITE 512
+#define HP_ST_NORECORD 1024
 #define HP_PI 3.14159265358979323846
 // The FP32 stage of the mixed path ends when the change is below
 // threshold * HP_MIXED_MARGIN: just above the final test, so that the FP64
@@ -46,7 +48,7 @@
 // status bits that end the outer loop (the reference raises on them)
 #define HP_ST_FATAL                                                                                         \
   (HP_ST_NEGDENS | HP_ST_LINESEARCH | HP_ST_NEWTONMAX | HP_ST_QPFAIL | HP_ST_NOTMONO | HP_ST_SINGULAR | \
-   HP_ST_NONFINITE)
+   HP_ST_NONFINITE | HP_ST_NORECORD)
 
 // Stage of a molecule in flight (HpSlot::run)
 #define HP_RUN_IDLE 0  // finished, failed, or waiting for the host
@@ -120,6 +122,11 @@ struct DevFn {
   int tail_cull;         // weights pass: skip atoms whose bounded contribution is
                          // negligible (schemes with non-negative coefficients)
   unsigned long long *evals_done;  // HP_EVAL_SLOTS counters per precision (may be null)
+  // tabulated pro-atoms (HP_FN_TABLE): atom b's knots tab_x[tab_off[b] ..
+  // tab_off[b+1]) and 4 cubic coefficients per knot in tab_c
+  const int *tab_off = nullptr;
+  const double *tab_x = nullptr;
+  const double *tab_c = nullptr;
 };
 
 // Counters of performed function evaluations: 64 per precision, one per
@@ -133,6 +140,7 @@ struct DevFn {
 #define HP_FN_SLATER 0   // all functions exp(-a r)
 #define HP_FN_GAUSS 1    // all functions exp(-a r^2)
 #define HP_FN_GENERAL 2  // exp(-a r^n), n per function
+#define HP_FN_TABLE 3    // piecewise cubic radial table per atom (ISA, Hirshfeld, Hirshfeld-I)
 
 // ---------------------------------------------------------------------------
 // reductions (deterministic: fixed tree, result broadcast from lane 0)
@@ -331,9 +339,21 @@ static __global__ void k_iter_begin(const int *__restrict__ done, int scheme, in
   }
 }
 
+// (dx^2 + dy^2) + dz^2 with every operation rounded on its own.
+__device__ __forceinline__ double dist2_rounded(double dx, double dy, double dz) {
+#pragma clang fp contract(off)
+  return (dx * dx + dy * dy) + dz * dz;
+}
+
 // Pro-atom density of atom b at squared distance r2 (and distance r).
 template <typename R, int FK>
 __device__ __forceinline__ R proatom_value(const DevFn &fn, int b, R r, R r2) {
+  if (FK == HP_FN_TABLE) {
+    // the spline plus the 1e-100 of eval_proatom (core/stockholder.py:347-349,
+    // hirshfeld_i.py:158)
+    const int k0 = fn.tab_off[b];
+    return (R)(table_eval(fn.tab_x + k0, fn.tab_c + 4 * (int64_t)k0, fn.tab_off[b + 1] - k0, (double)r) + 1e-100);
+  }
   R pb = (R)0;
   const int f1 = fn.fn_off[b + 1];
   for (int f = fn.fn_off[b]; f < f1; ++f) {
@@ -365,7 +385,8 @@ __device__ __forceinline__ void promol_at(const DevMol &m, const DevFn &fn, int 
   for (int t = 0; t < nb_n; ++t) {
     const int b = nb ? nb[t] : t;
     const double dx = x - m.cx[b], dy = y - m.cy[b], dz = z - m.cz[b];
-    const R r2 = (R)(dx * dx + dy * dy + dz * dz);
+    // tabulated pro-atoms: the distance as numpy's norm forms it
+    const R r2 = (R)(FK == HP_FN_TABLE ? dist2_rounded(dx, dy, dz) : dx * dx + dy * dy + dz * dz);
     const R r = (FK == HP_FN_GAUSS && !cull) ? (R)0 : hp_sqrt<R>(r2);
     R pb = (R)0;
     if (!cull || r <= rc) pb = proatom_value<R, FK>(fn, b, r, r2);

@@ -1,4 +1,4 @@
-"""MI355X-native stockholder partitioning (MBIS / GISA / aLISA / gLISA / NLIS / GMBIS).
+"""MI355X-native stockholder partitioning (Hirshfeld / Hirshfeld-I / ISA / MBIS / GISA / aLISA / gLISA / NLIS / GMBIS).
 
 Drop-in for the grid-based stockholder loop of horton-part: the scheme
 classes keep the reference's ``WPart`` API and run the loop in
@@ -10,13 +10,22 @@ from .basis import ExpBasisFuncHelper, evaluate_function
 from .gisa import GaussianISAWPart
 from .glisa import GlobalLinearISAWPart
 from .gmbis import GMBISWPart, get_initial_gmbis_propars
-from .grid import AtomGrid, BeckeWeights, ExpRTransform, MolGrid, OneDGrid, UniformInteger
+from .grid import AtomGrid, BeckeWeights, ExpRTransform, MolGrid, OneDGrid, PowerRTransform, UniformInteger
+from .hirshfeld import HirshfeldWPart
+from .hirshfeld_i import HirshfeldIWPart
+from .isa import ISAWPart
 from .mbis import MBISWPart, get_initial_mbis_propars, get_nshell
 from .nlis import NLISWPart, get_initial_nlis_propars, get_nlis_nshell
+from .proatomdb import ProAtomDB, ProAtomRecord
 from .serving import PartitionServer
 from .utils import wpart_schemes
 
 __all__ = [
+    "HirshfeldWPart",
+    "HirshfeldIWPart",
+    "ISAWPart",
+    "ProAtomDB",
+    "ProAtomRecord",
     "MBISWPart",
     "GaussianISAWPart",
     "LinearISAWPart",
@@ -36,6 +45,7 @@ __all__ = [
     "AtomGrid",
     "BeckeWeights",
     "ExpRTransform",
+    "PowerRTransform",
     "OneDGrid",
     "UniformInteger",
 ]

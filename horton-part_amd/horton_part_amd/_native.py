@@ -7,6 +7,7 @@ missing or fails to load, and the scheme classes call it before any work.
 
 import ctypes
 import os
+import threading
 from pathlib import Path
 
 import numpy as np
@@ -18,7 +19,7 @@ HP_E_HIP = -2
 HP_E_SOLVER = -3
 HP_E_UNSUPP = -4
 
-SCHEME = {"mbis": 0, "gisa": 1, "lisa": 2}
+SCHEME = {"mbis": 0, "gisa": 1, "lisa": 2, "nlis": 3, "isa": 4, "hi": 5, "hirshfeld": 6}
 SOLVER = {"mbis": 0, "qp": 1, "sc": 2, "newton": 3, "m-newton": 4}
 PREC = {"fp64": 0, "mixed": 1}
 
@@ -32,6 +33,7 @@ ST_NEG_PARS = 64
 ST_NOT_MONOTONIC = 128
 ST_SINGULAR = 256
 ST_NONFINITE = 512
+ST_NO_RECORD = 1024
 
 KERNEL_WEIGHTS = 0
 KERNEL_INNER = 1
@@ -128,6 +130,13 @@ class SchemeDesc(ctypes.Structure):
         ("mixed_switch", _d),
         ("mixed_refresh", ctypes.c_int),
         ("full_sum", ctypes.c_int),
+        ("tab_offsets", _pi32),
+        ("tab_knots", _pd),
+        ("tab_rweights", _pd),
+        ("rec_offsets", _pi32),
+        ("rec_charges", _pi32),
+        ("rec_rho", _pd),
+        ("rec_coefs", _pd),
     ]
 
 
@@ -261,7 +270,7 @@ def load():
     for name in EXPORTED_SYMBOLS:
         if name not in ("hp_abi_version", "hp_last_error"):
             getattr(lib, name).restype = ctypes.c_int
-    if lib.hp_abi_version() != 2:
+    if lib.hp_abi_version() != 3:
         raise RuntimeError("libhortonpart ABI version mismatch")
     _LIB = lib
     return lib
@@ -444,7 +453,8 @@ class DevicePartition:
         return mn, msd
 
     def populations(self):
-        """AIM populations on the molecular grid from the last weights pass."""
+        """AIM populations from the last weights pass: on the molecular grid
+        (grid_type 2, 3), or on the atom grids for the tabulated schemes."""
         out = np.empty(self.natom)
         check(self.lib.hp_part_populations(self.handle, ptr(out)))
         return out
@@ -470,16 +480,55 @@ class PinnedArray(np.ndarray):
             self._owner = getattr(obj, "_owner", None)
 
 
+class _PinnedPool:
+    """Page-locked host blocks kept for reuse.  Freeing page-locked memory
+    waits for the device to go idle, which would drain the work queued ahead
+    of a serving loop whenever a finished request drops its inputs; blocks
+    therefore return here (size classes of an eighth of an octave, at least
+    4 KiB) and are freed only while more than ``keep_bytes`` sit unused."""
+
+    def __init__(self, keep_bytes=64 << 30):
+        self._lock = threading.Lock()
+        self._free = {}
+        self._idle = 0
+        self.keep_bytes = keep_bytes
+
+    @staticmethod
+    def size_class(nbytes):
+        n = max(int(nbytes), 4096)
+        step = 1 << max((n - 1).bit_length() - 3, 12)
+        return -(-n // step) * step
+
+    def get(self, nbytes):
+        size = self.size_class(nbytes)
+        with self._lock:
+            blocks = self._free.get(size)
+            if blocks:
+                self._idle -= size
+                return blocks.pop(), size
+        p = ctypes.c_void_p()
+        check(load().hp_host_alloc(size, ctypes.byref(p)))
+        return p, size
+
+    def put(self, ptr, size):
+        with self._lock:
+            if self._idle + size <= self.keep_bytes:
+                self._free.setdefault(size, []).append(ptr)
+                self._idle += size
+                return
+        load().hp_host_free(ptr)
+
+
+_PINNED_POOL = _PinnedPool()
+
+
 class _PinnedOwner:
     def __init__(self, nbytes):
-        self.lib = load()
-        p = ctypes.c_void_p()
-        check(self.lib.hp_host_alloc(int(nbytes), ctypes.byref(p)))
-        self.ptr = p
+        self.ptr, self.size = _PINNED_POOL.get(nbytes)
 
     def __del__(self):
         try:
-            self.lib.hp_host_free(self.ptr)
+            _PINNED_POOL.put(self.ptr, self.size)
         except Exception:
             pass
 

@@ -6,9 +6,12 @@ only a small surface of it (call sites ``core/base.py:26,264,298,374-402``,
 ``mbis.py:176-184``, ``gisa.py:283-293``; the test set-up is
 ``tests/test_wpart.py:47-51``).  This module restates that surface:
 
-* ``UniformInteger`` / ``ExpRTransform`` / ``OneDGrid``: radial rules.
-  ``ExpRTransform(rmin, rmax, b)`` maps ``x`` to ``rmin*exp(x*ln(rmax/rmin)/b)``;
-  the radii of the reference's water fixture agree with it to 2e-15.
+* ``UniformInteger`` / ``ExpRTransform`` / ``PowerRTransform`` / ``OneDGrid``:
+  radial rules.  ``ExpRTransform(rmin, rmax, b)`` maps ``x`` to
+  ``rmin*exp(x*ln(rmax/rmin)/b)``; the radii of the reference's water fixture
+  agree with it to 2e-15.  ``PowerRTransform(rmin, rmax, b)`` maps ``x`` to
+  ``rmin*(x+1)**p`` with ``p = ln(rmax/rmin)/ln(b+1)`` (the radial grids of
+  the pro-atom database files, ``tests/common.py:78-110``).
 * ``AtomGrid``: an atom-centred product grid with points ``c + r_i u_j`` in
   shell-major order and weights ``w_i * r_i**2 * 4*pi*wl_j`` (``sum wl = 1``).
 * ``MolGrid``: the concatenation of atom grids (atom-major) with molecular
@@ -33,6 +36,7 @@ __all__ = [
     "OneDGrid",
     "UniformInteger",
     "ExpRTransform",
+    "PowerRTransform",
     "AtomGrid",
     "MolGrid",
     "BeckeWeights",
@@ -96,6 +100,24 @@ class ExpRTransform:
         alpha = np.log(self.rmax / self.rmin) / b
         r = self.rmin * np.exp(oned.points * alpha)
         return OneDGrid(r, oned.weights * r * alpha)
+
+
+class PowerRTransform:
+    """Power radial map ``r(x) = rmin * (x + 1)**p``, ``p = ln(rmax/rmin) / ln(b + 1)``:
+    on ``UniformInteger(b + 1)`` the radii run from ``rmin`` to ``rmax``."""
+
+    def __init__(self, rmin, rmax, b=None):
+        if rmin <= 0 or rmax <= rmin:
+            raise ValueError("PowerRTransform needs 0 < rmin < rmax")
+        self.rmin = float(rmin)
+        self.rmax = float(rmax)
+        self.b = b
+
+    def transform_1d_grid(self, oned):
+        b = self.b if self.b is not None else oned.points[-1]
+        power = np.log(self.rmax / self.rmin) / np.log(b + 1.0)
+        r = self.rmin * np.power(oned.points + 1.0, power)
+        return OneDGrid(r, oned.weights * power * self.rmin * np.power(oned.points + 1.0, power - 1.0))
 
 
 def exp_radial_grid(nrad, rmin=5e-4, rmax=2e1):

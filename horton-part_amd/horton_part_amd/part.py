@@ -13,12 +13,13 @@ import logging
 import warnings
 
 import numpy as np
+from scipy.interpolate import CubicHermiteSpline, CubicSpline
 
 from . import _native
 from .cache import Cache, JustOnceClass, just_once
 from .utils import DENSITY_CUTOFF, NEGATIVE_CUTOFF, POPULATION_CUTOFF, typecheck_geo
 
-__all__ = ["Part", "WPart", "AbstractISAWPart", "get_ncart_cumul", "get_npure_cumul"]
+__all__ = ["Part", "WPart", "ProatomSplines", "AbstractISAWPart", "get_ncart_cumul", "get_npure_cumul"]
 
 
 def get_ncart_cumul(lmax):
@@ -254,69 +255,6 @@ class WPart(Part):
             self._radial_distances_host = [np.linalg.norm(self.grid.points - c, axis=1) for c in self.coordinates]
         return self._radial_distances_host
 
-
-class AbstractISAWPart(WPart):
-    """Iterative stockholder outer loop on the GPU (core/iterstock.py:61-193).
-
-    Extra keyword arguments (defaults reproduce the reference):
-      precision: "fp64" (default) or "mixed" (FP32 grid arithmetic, then FP64 polish)
-      n_polish: minimum FP64 iterations after the mixed stage (default 1)
-      mixed_switch: the FP32 stage ends when the change is below
-        max(1.01 threshold, mixed_switch) (default 1e-5); its last iteration
-        is redone in FP64 and FP64 iterations run until the change is below
-        threshold (grid_type 2: max(threshold, mixed_switch))
-      mixed_refresh: every this many FP32-stage iterations one runs in FP64
-        and refreshes the per-shell correction FP64 - FP32 of the spherical
-        averages (default 0: no correction)
-      cutoff_radius: pro-atom tails beyond this distance are zero (default None)
-      full_sum: evaluate every pro-atom on every point (default False: MBIS
-        and GISA skip, per radial shell, atoms whose pro-atom is bounded there
-        below 2^-64 of the promolecule, which changes no result beyond
-        rounding; LISA always sums every atom)
-      device: GPU ordinal (default 0)
-    """
-
-    _scheme_name = None
-    _solver_name = None
-
-    def __init__(
-        self,
-        coordinates,
-        numbers,
-        pseudo_numbers,
-        grid,
-        moldens,
-        spindens=None,
-        lmax=3,
-        logger=None,
-        threshold=1e-6,
-        maxiter=500,
-        inner_threshold=1e-8,
-        grid_type=1,
-        precision="fp64",
-        n_polish=1,
-        mixed_switch=1e-5,
-        mixed_refresh=0,
-        cutoff_radius=None,
-        full_sum=False,
-        device=0,
-        **kwargs,
-    ):
-        self._threshold = threshold
-        self._inner_threshold = inner_threshold if inner_threshold < threshold else threshold
-        self._maxiter = maxiter
-        if precision not in ("fp64", "mixed"):
-            raise ValueError("precision must be 'fp64' or 'mixed'")
-        self._precision = precision
-        self._n_polish = n_polish
-        self._mixed_switch = mixed_switch
-        self._mixed_refresh = mixed_refresh
-        self._cutoff_radius = cutoff_radius
-        self._full_sum = bool(full_sum)
-        self._device = device
-        self._dev = None
-        super().__init__(coordinates, numbers, pseudo_numbers, grid, moldens, spindens, lmax, logger, grid_type=grid_type, **kwargs)
-
     # -- device description ---------------------------------------------------
     def _molecule_arrays(self):
         grid = self.grid
@@ -397,6 +335,122 @@ class AbstractISAWPart(WPart):
             "shell_radius": np.zeros(nblock),
             "shell_rweight": np.zeros(nblock),
         }
+
+
+class ProatomSplines:
+    """Pro-atom splines on the host (core/stockholder.py:177-349, 386-398):
+    the reference's API to a pro-atom as a radial spline, for schemes whose
+    ``get_proatom_rho`` returns values on ``get_rgrid(index)``.  The
+    partitioning itself evaluates pro-atoms on the device."""
+
+    def fix_proatom_rho(self, index, rho, deriv):
+        """Negative densities become zero and the derivative is dropped
+        (core/stockholder.py:201-225)."""
+        if rho.min() < 0:
+            rgrid = self.get_rgrid(index)
+            original = rgrid.integrate(rho)
+            rho[rho < 0] = 0.0
+            deriv = None
+            self.logger.info("                Pro-atom not positive everywhere. Lost %.1e electrons" % (rgrid.integrate(rho) - original))
+        return rho, deriv
+
+    def get_proatom_spline(self, index, *args, **kwargs):
+        """CubicHermiteSpline through (rho, deriv), or CubicSpline through rho
+        when there is no derivative, on the pro-atom's radial grid
+        (core/stockholder.py:227-264)."""
+        rho, deriv = self.get_proatom_rho(index, *args, **kwargs)
+        rho, deriv = self.fix_proatom_rho(index, rho, deriv)
+        points = self.get_rgrid(index).points
+        if deriv is None:
+            return CubicSpline(points, rho, True)
+        return CubicHermiteSpline(points, rho, deriv, True)
+
+    def eval_spline(self, index, spline, output, grid, label="noname"):
+        """``spline`` at the distances of ``grid.points`` from atom ``index``
+        (core/stockholder.py:266-300)."""
+        r = np.linalg.norm(grid.points - self.coordinates[index], axis=1)
+        output[:] = spline(r)
+
+    def eval_proatom(self, index, output, grid):
+        """The pro-atom spline at ``grid``'s points plus 1e-100
+        (core/stockholder.py:302-349)."""
+        spline = self.get_proatom_spline(index)
+        output[:] = 0.0
+        self.eval_spline(index, spline, output, grid, label="proatom")
+        output += 1e-100
+        assert np.isfinite(output).all()
+
+    @just_once
+    def do_prosplines(self):
+        """Cache the pro-atom density splines (core/stockholder.py:386-398)."""
+        self.do_partitioning()
+        for index in range(self.natom):
+            key = ("spline_prodensity", index)
+            if key not in self.cache:
+                self.cache.dump(key, self.get_proatom_spline(index), tags="o")
+
+
+class AbstractISAWPart(WPart):
+    """Iterative stockholder outer loop on the GPU (core/iterstock.py:61-193).
+
+    Extra keyword arguments (defaults reproduce the reference):
+      precision: "fp64" (default) or "mixed" (FP32 grid arithmetic, then FP64 polish)
+      n_polish: minimum FP64 iterations after the mixed stage (default 1)
+      mixed_switch: the FP32 stage ends when the change is below
+        max(1.01 threshold, mixed_switch) (default 1e-5); its last iteration
+        is redone in FP64 and FP64 iterations run until the change is below
+        threshold (grid_type 2: max(threshold, mixed_switch))
+      mixed_refresh: every this many FP32-stage iterations one runs in FP64
+        and refreshes the per-shell correction FP64 - FP32 of the spherical
+        averages (default 0: no correction)
+      cutoff_radius: pro-atom tails beyond this distance are zero (default None)
+      full_sum: evaluate every pro-atom on every point (default False: MBIS
+        and GISA skip, per radial shell, atoms whose pro-atom is bounded there
+        below 2^-64 of the promolecule, which changes no result beyond
+        rounding; LISA always sums every atom)
+      device: GPU ordinal (default 0)
+    """
+
+    _scheme_name = None
+    _solver_name = None
+
+    def __init__(
+        self,
+        coordinates,
+        numbers,
+        pseudo_numbers,
+        grid,
+        moldens,
+        spindens=None,
+        lmax=3,
+        logger=None,
+        threshold=1e-6,
+        maxiter=500,
+        inner_threshold=1e-8,
+        grid_type=1,
+        precision="fp64",
+        n_polish=1,
+        mixed_switch=1e-5,
+        mixed_refresh=0,
+        cutoff_radius=None,
+        full_sum=False,
+        device=0,
+        **kwargs,
+    ):
+        self._threshold = threshold
+        self._inner_threshold = inner_threshold if inner_threshold < threshold else threshold
+        self._maxiter = maxiter
+        if precision not in ("fp64", "mixed"):
+            raise ValueError("precision must be 'fp64' or 'mixed'")
+        self._precision = precision
+        self._n_polish = n_polish
+        self._mixed_switch = mixed_switch
+        self._mixed_refresh = mixed_refresh
+        self._cutoff_radius = cutoff_radius
+        self._full_sum = bool(full_sum)
+        self._device = device
+        self._dev = None
+        super().__init__(coordinates, numbers, pseudo_numbers, grid, moldens, spindens, lmax, logger, grid_type=grid_type, **kwargs)
 
     def _scheme_arrays(self):
         """Scheme-specific entries of ``hp_scheme_desc`` (subclasses)."""

@@ -43,8 +43,21 @@ def wpart_schemes(scheme):
     """Return the partitioning class registered under ``scheme``.
 
     The stockholder schemes of the accelerated path are provided:
-    ``"mbis"``, ``"gisa"``, ``"lisa"``, ``"glisa"``, ``"nlis"`` and ``"gmbis"``.
+    ``"h"``, ``"hi"``, ``"is"``, ``"mbis"``, ``"gisa"``, ``"lisa"``,
+    ``"glisa"``, ``"nlis"`` and ``"gmbis"``.
     """
+    if scheme == "h":
+        from .hirshfeld import HirshfeldWPart
+
+        return HirshfeldWPart
+    if scheme == "hi":
+        from .hirshfeld_i import HirshfeldIWPart
+
+        return HirshfeldIWPart
+    if scheme == "is":
+        from .isa import ISAWPart
+
+        return ISAWPart
     if scheme == "mbis":
         from .mbis import MBISWPart
 

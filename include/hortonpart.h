@@ -24,7 +24,7 @@
 extern "C" {
 #endif
 
-#define HP_ABI_VERSION 2
+#define HP_ABI_VERSION 3
 
 /* error codes (negative returns) */
 #define HP_E_ARG -1      /* invalid argument or shape */
@@ -37,6 +37,9 @@ extern "C" {
 #define HP_SCHEME_GISA 1 /* gisa.py:109-345 */
 #define HP_SCHEME_LISA 2 /* alisa.py:1155-1335 */
 #define HP_SCHEME_NLIS 3 /* nlis.py:201-367 and gmbis.py:75-156: (N, S, n) per shell */
+#define HP_SCHEME_ISA 4  /* isa.py:37-122: spline pro-atoms through the spherical averages */
+#define HP_SCHEME_HI 5   /* hirshfeld_i.py:36-176: interpolated pro-atom database records */
+#define HP_SCHEME_HIRSHFELD 6 /* hirshfeld.py:127-193: fixed neutral records, one weights pass */
 
 /* inner solvers */
 #define HP_SOLVER_MBIS 0     /* mbis.py:81-163 opt_mbis_propars */
@@ -61,6 +64,8 @@ extern "C" {
 #define HP_ST_NOT_MONOTONIC 128     /* utils.py:546-553 RuntimeError */
 #define HP_ST_SINGULAR 256          /* alisa.py:924 scipy LinAlgError (singular Hessian) */
 #define HP_ST_NONFINITE 512         /* alisa.py:924 scipy ValueError (inf/NaN Hessian) */
+#define HP_ST_NORECORD 1024         /* hirshfeld_i.py:124-157: no database record for the charge
+                                       (KeyError) or a pseudo population <= 0 (ValueError) */
 
 /* A molecule on a grid of atom-centred product grids.  The molecular grid is
  * the concatenation of the atom grids (atom-major, radial-shell-major), as
@@ -125,6 +130,30 @@ typedef struct {
                                 rounding.  1: every atom on every point, as the
                                 reference evaluates them.  LISA always sums
                                 every atom (its coefficients may be negative). */
+  /* Tabulated pro-atoms (HP_SCHEME_ISA, HP_SCHEME_HI, HP_SCHEME_HIRSHFELD;
+   * solver and the basis arrays above are ignored, fn_offsets may be NULL).
+   * Atom a's pro-atom is a piecewise cubic over the ascending radial knots
+   * tab_knots[tab_offsets[a] .. tab_offsets[a+1]) (at least 2; at most 512),
+   * whose radial quadrature weights tab_rweights enter the change
+   * (iterstock.py:32-45).  ISA: the knots are the atom's radial shells in
+   * order (at least 4) and the tables are built on the device from the
+   * parameters; HI, HIRSHFELD: each atom's element grid from the database. */
+  const int32_t *tab_offsets; /* natom+1 */
+  const double *tab_knots;
+  const double *tab_rweights;
+  /* HP_SCHEME_HI: the database records of each atom's element,
+   * [rec_offsets[a], rec_offsets[a+1]) for atom a, record j of integer charge
+   * rec_charges[j] (ProAtomRecord, core/proatomdb.py:34-210).  Record values
+   * are stored in record order, each on its atom's knots: densities rec_rho
+   * (n_a per record) and the pieces of its spline rec_coefs (4 n_a per
+   * record: c0 d^3 + c1 d^2 + c2 d + c3 at d = r - knot_i for i < n_a - 1, the
+   * last row unused; ProAtomDB.get_spline, proatomdb.py:378-388).
+   * HP_SCHEME_HIRSHFELD: exactly one record per atom, its fixed pro-atom
+   * (get_proatom_spline of the neutral record, core/stockholder.py:227-264). */
+  const int32_t *rec_offsets; /* natom+1 */
+  const int32_t *rec_charges;
+  const double *rec_rho;
+  const double *rec_coefs;
 } hp_scheme_desc;
 
 typedef struct hp_part hp_part;
@@ -151,7 +180,9 @@ int hp_part_destroy(hp_part *part);
 /* Set the current pro-atom parameters (initial values: mbis.py:291-305,
  * gisa.py:68-88, nlis.py:330-346, gmbis.py:138-152).  Length: 2*sum(K_a)
  * for MBIS, 3*sum(K_a) for NLIS (whose n must equal fn_order), sum(K_a)
- * otherwise. */
+ * for GISA/LISA; ISA: the pro-atom values at the knots (sum n_a, initially
+ * zero, isa.py:93-100); HI: the charges (natom, initially zero,
+ * hirshfeld_i.py:160-164); HIRSHFELD: natom values that are not used. */
 int hp_part_set_propars(hp_part *part, const double *propars, int64_t n);
 
 /* One update_at_weights pass (core/stockholder.py:352-384) at the current
@@ -178,7 +209,9 @@ int hp_part_weights(hp_part *part, double *rhobar, double *entropy);
  * below threshold; maxiter caps the total number of iterations unless the
  * polish needs more.  Returns the number of
  * outer iterations through *niter.  HP_E_SOLVER if a per-atom solver raised;
- * the loop stops at that iteration. */
+ * the loop stops at that iteration.  ISA and HI iterate in FP64 (their
+ * per-atom updates are closed forms: isa.py:102-122, hirshfeld_i.py:166-172);
+ * HIRSHFELD does not iterate (HP_E_UNSUPP: use hp_part_weights). */
 int hp_part_run(hp_part *part, int precision, int n_polish, int *niter);
 
 /* Copy results to host buffers (any pointer may be NULL):
@@ -252,7 +285,10 @@ int hp_part_global_proatoms(hp_part *part, const double *coefs_new, const double
 
 /* AIM populations sum_p w rho clip(rho0_a / promol) on the molecular grid,
  * from the last weights pass (hp_part_weights or hp_part_run): the charges
- * step of glisa.py:270-278.  grid_type 2 or 3. */
+ * step of glisa.py:270-278.  grid_type 2 or 3.  The tabulated schemes give,
+ * on grid types 1 and 2, the atom-grid populations of do_populations
+ * (core/base.py:277-300): sum over atom a's own grid of the atom-grid weight
+ * times w_a rho. */
 int hp_part_populations(hp_part *part, double *populations);
 
 /* Timing of the pro-atom/weights kernel over the last hp_part_run, measured

@@ -27,6 +27,7 @@ import math
 import warnings
 
 import numpy as np
+from scipy.interpolate import CubicHermiteSpline, CubicSpline
 from scipy.linalg import solve as scipy_solve
 
 DENSITY_CUTOFF = 1e-15  # data/constants.yaml:27
@@ -1086,3 +1087,223 @@ class OracleGLISA(OracleWPart):
             np.clip(w, 0, 1, out=w)
             self.cache[f"at_weights_{i}"] = w
         self.cache["promoldens"] = promol
+
+
+# ----------------------------------------------------------------------------
+# tabulated pro-atoms: core/proatomdb.py, hirshfeld.py, hirshfeld_i.py, isa.py
+# ----------------------------------------------------------------------------
+
+
+class OracleRGrid:
+    """A radial quadrature with qc-grid's OneDGrid.integrate (sum w prod f)."""
+
+    def __init__(self, points, weights):
+        self.points = np.asarray(points, float)
+        self.weights = np.asarray(weights, float)
+        self.size = self.points.size
+
+    def integrate(self, *f):
+        prod = self.weights
+        for v in f:
+            prod = prod * v
+        return float(np.sum(prod))
+
+
+def power_rgrid(rmin, rmax, npoint):
+    """qc-grid PowerRTransform(rmin, rmax, npoint - 1) on UniformInteger(npoint)
+    (the record loader, tests/common.py:78-110): r_i = rmin (i + 1)^p with
+    p = ln(rmax / rmin) / ln(npoint), weights dr/di."""
+    i = np.arange(npoint, dtype=float)
+    p = np.log(rmax / rmin) / np.log(npoint)
+    return OracleRGrid(rmin * (i + 1.0) ** p, p * rmin * (i + 1.0) ** (p - 1.0))
+
+
+class OracleRecord:
+    """ProAtomRecord's data (core/proatomdb.py:34-83)."""
+
+    def __init__(self, number, charge, energy, rgrid, rho, deriv=None, pseudo_number=None):
+        self.number = int(number)
+        self.charge = int(charge)
+        self.energy = float(energy)
+        self.rgrid = rgrid
+        self.rho = np.asarray(rho, float)
+        self.deriv = None if deriv is None else np.asarray(deriv, float)
+        self.pseudo_number = self.number if pseudo_number is None else pseudo_number
+
+
+class OracleProAtomDB:
+    """ProAtomDB.get_record/get_rgrid/get_rho/get_spline
+    (core/proatomdb.py:213-388; linear combinations only)."""
+
+    def __init__(self, records):
+        best = {}
+        for r in records:  # lowest energy per (number, charge), proatomdb.py:228-234
+            key = (r.number, r.charge)
+            if key not in best or r.energy < best[key].energy:
+                best[key] = r
+        self.map = best
+        self.rgrids = {}
+        for r in best.values():
+            self.rgrids.setdefault(r.number, r.rgrid)
+
+    def get_record(self, number, charge):
+        return self.map[(int(number), int(charge))]
+
+    def get_rgrid(self, number):
+        return self.rgrids[int(number)]
+
+    def get_charges(self, number):
+        return sorted((c for (n, c) in self.map if n == int(number)), reverse=True)
+
+    def get_rho(self, number, parameters=0, do_deriv=False):
+        if isinstance(parameters, int):
+            r = self.get_record(number, parameters)
+            return (r.rho, r.deriv) if do_deriv else r.rho
+        rho, deriv = 0.0, (0.0 if do_deriv else None)
+        for charge, coeff in parameters.items():
+            if coeff != 0.0:
+                r = self.get_record(number, charge)
+                rho = rho + coeff * r.rho
+                deriv = deriv + coeff * r.deriv if (do_deriv and r.deriv is not None and deriv is not None) else None
+        return (rho, deriv) if do_deriv else rho
+
+    def get_spline(self, number, charge):
+        rho, deriv = self.get_rho(number, charge, do_deriv=True)
+        x = self.get_rgrid(number).points
+        return CubicSpline(x, rho, True) if deriv is None else CubicHermiteSpline(x, rho, deriv, True)
+
+
+def load_record_npz(path):
+    """A database record from the reference's atom npz files (tests/common.py:78-110)."""
+    with np.load(path) as d:
+        rmin, rmax, n = d["rgrid"]
+        pseudo = float(d["pseudo_number"]) if "pseudo_number" in d.files else None
+        return OracleRecord(int(d["number"]), int(d["charge"]), float(d["energy"]), power_rgrid(rmin, rmax, int(n)),
+                            d["dens"], d["deriv"], pseudo_number=pseudo)
+
+
+class _OracleTabulated(OracleWPart):
+    """Shared by the tabulated schemes: pro-atoms are splines of the radial
+    distance, evaluated with scipy as the reference does."""
+
+    def spline_proatom(self, i, spline):
+        """eval_spline + 1e-100 (core/stockholder.py:266-349)."""
+        return spline(self.radial_distances[i]) + 1e-100
+
+
+class OracleHirshfeld(_OracleTabulated):
+    """HirshfeldWPart (hirshfeld.py:127-193): neutral records, one
+    update_at_weights, atom-grid populations (core/base.py:277-311)."""
+
+    def __init__(self, *args, proatomdb=None, **kwargs):
+        super().__init__(*args, **kwargs)
+        self.db = proatomdb
+        for i in range(self.natom):  # check_proatomdb, hirshfeld.py:34-45
+            if self.db.get_record(self.numbers[i], 0).pseudo_number != self.pseudo_numbers[i]:
+                raise ValueError("pseudo number mismatch")
+
+    def rgrid(self, i):
+        return self.db.get_rgrid(self.numbers[i])
+
+    def eval_proatom(self, i, propars):
+        rho, deriv = self.db.get_rho(self.numbers[i], do_deriv=True)
+        rho = np.array(rho, float)
+        if rho.min() < 0:  # fix_proatom_rho, core/stockholder.py:201-225
+            rho[rho < 0] = 0.0
+            deriv = None
+        x = self.rgrid(i).points
+        spline = CubicSpline(x, rho, True) if deriv is None else CubicHermiteSpline(x, rho, deriv, True)
+        return self.spline_proatom(i, spline)
+
+    def do_partitioning(self):
+        self.update_at_weights(None)
+        pseudo = np.zeros(self.natom)
+        for i in range(self.natom):
+            w = self.cache[f"at_weights_{i}"]
+            if self.only_molgrid:
+                pseudo[i] = self.grid.integrate(w, self.moldens)
+            else:
+                if self.on_molgrid:
+                    w = self.to_atomic_grid(i, w)
+                at = self.grid.atgrids[i]
+                pseudo[i] = at.integrate(w, self.to_atomic_grid(i, self.moldens))
+        self.cache["pseudo_populations"] = pseudo
+        self.cache["populations"] = pseudo + self.numbers - self.pseudo_numbers
+        self.cache["charges"] = self.numbers - self.cache["populations"]
+        return self.cache
+
+
+class OracleHI(_OracleTabulated):
+    """HirshfeldIWPart (hirshfeld_i.py:36-176): the parameters are the charges."""
+
+    def __init__(self, *args, proatomdb=None, **kwargs):
+        super().__init__(*args, **kwargs)
+        self.db = proatomdb
+
+    def rgrid(self, i):
+        return self.db.get_rgrid(self.numbers[i])
+
+    def interpolation(self, i, charges):
+        icharge = int(np.floor(charges[i]))  # hirshfeld_i.py:116-122
+        return icharge, charges[i] - icharge
+
+    def get_proatom_rho(self, i, charges):
+        """hirshfeld_i.py:124-134 (density only)."""
+        icharge, x = self.interpolation(i, charges)
+        pp = self.pseudo_numbers[i] - icharge
+        if pp == 1 or x == 0.0:
+            return self.db.get_rho(self.numbers[i], {icharge: 1 - x})
+        if pp > 1:
+            return self.db.get_rho(self.numbers[i], {icharge: 1 - x, icharge + 1: x})
+        raise ValueError("Requesting a pro-atom with a negative (pseudo) population")
+
+    def eval_proatom(self, i, charges):
+        """The greedy eval_proatom: the records' splines combined after
+        evaluation (hirshfeld_i.py:148-158)."""
+        icharge, x = self.interpolation(i, charges)
+        out = self.db.get_spline(self.numbers[i], icharge)(self.radial_distances[i]) * (1 - x)
+        pp = self.pseudo_numbers[i] - icharge
+        if pp > 1 and x != 0.0:
+            out = out + self.db.get_spline(self.numbers[i], icharge + 1)(self.radial_distances[i]) * x
+        elif pp <= 0:
+            raise ValueError("Requesting a pro-atom with a negative (pseudo) population")
+        return out + 1e-100
+
+    def init_propars(self):
+        return np.zeros(self.natom)
+
+    def update_atom(self, i, propars):
+        """hirshfeld_i.py:166-172 with compute_pseudo_population (core/base.py:259-264)."""
+        at = self.grid.atgrids[i]
+        pop = at.integrate(self.cache[f"at_weights_{i}"], self.to_atomic_grid(i, self.moldens))
+        propars[i] = self.pseudo_numbers[i] - pop
+        self.cache["charges"][i] = propars[i]
+
+
+class OracleISA(_OracleTabulated):
+    """ISAWPart (isa.py:37-122): spline pro-atoms through the clipped
+    spherical averages on each atom's radial grid."""
+
+    def __init__(self, *args, **kwargs):
+        super().__init__(*args, **kwargs)
+        self.ranges = np.concatenate([[0], np.cumsum([self.rgrid(i).size for i in range(self.natom)])])
+
+    def init_propars(self):
+        return np.zeros(self.ranges[-1])  # isa.py:93-100
+
+    def get_proatom_rho(self, i, propars):
+        return propars[self.ranges[i] : self.ranges[i + 1]]
+
+    def eval_proatom(self, i, propars):
+        rho = np.array(self.get_proatom_rho(i, propars), float)
+        rho[rho < 0] = 0.0  # fix_proatom_rho
+        return self.spline_proatom(i, CubicSpline(self.rgrid(i).points, rho, True))
+
+    def update_atom(self, i, propars):
+        """isa.py:102-122."""
+        rhoa, _, _ = self.atom_problem(i)
+        r = np.clip(self.rgrid(i).points, 1e-100, 1e10)
+        sph = np.clip(rhoa, 1e-100, np.inf)
+        self.cache[f"spherical_average_{i}"] = sph
+        propars[self.ranges[i] : self.ranges[i + 1]] = sph
+        self.cache["charges"][i] = self.pseudo_numbers[i] - self.rgrid(i).integrate(4 * np.pi * r**2 * sph)

@@ -82,8 +82,13 @@ def test_registry_and_init_params():
     assert wpart_schemes("mbis") is MBISWPart
     assert wpart_schemes("gisa") is GaussianISAWPart
     assert wpart_schemes("lisa") is LinearISAWPart
+    from horton_part_amd import HirshfeldIWPart, HirshfeldWPart, ISAWPart
+
+    assert wpart_schemes("h") is HirshfeldWPart
+    assert wpart_schemes("hi") is HirshfeldIWPart
+    assert wpart_schemes("is") is ISAWPart
     with pytest.raises(NotImplementedError):
-        wpart_schemes("h")
+        wpart_schemes("mulliken")
     for z in (1, 2, 3, 8, 17, 21, 44, 104):
         assert get_nshell(z) == ho.get_nshell(z)
         np.testing.assert_array_equal(get_initial_mbis_propars(z), ho.get_initial_mbis_propars(z))

@@ -29,7 +29,7 @@ def test_header_symbols_exported():
 
 def test_abi_version_and_errors():
     lib = _native.load()
-    assert lib.hp_abi_version() == 2
+    assert lib.hp_abi_version() == 3
     n = ctypes.c_int32(-1)
     lib.hp_device_count(ctypes.byref(n))
     assert n.value >= 0
@@ -94,6 +94,35 @@ def _desc(**over):
 def test_create_rejects_bad_descriptions(over, code):
     """Validation happens before any device call, so it runs on the CPU."""
     arrays, scheme = _desc(**over)
+    with pytest.raises(_native.HPError) as exc:
+        _native.DevicePartition(arrays, scheme)
+    assert exc.value.code == code
+
+
+_TAB = dict(tab_offsets=np.array([0, 4, 8], np.int32), tab_knots=np.tile([0.1, 0.5, 1.0, 2.0], 2),
+            tab_rweights=np.ones(8))
+_REC = dict(rec_offsets=np.array([0, 1, 2], np.int32), rec_charges=np.zeros(2, np.int32), rec_rho=np.ones(8),
+            rec_coefs=np.zeros(32))
+
+
+@pytest.mark.parametrize(
+    "over,code",
+    [
+        (dict(scheme=4), _native.HP_E_ARG),  # no tables
+        (dict(scheme=4, grid_type=2, **_TAB), _native.HP_E_UNSUPP),  # ISA lives on atom grids
+        (dict(scheme=4, **_TAB), _native.HP_E_ARG),  # ISA knots must be the atoms' shells
+        (dict(scheme=5, **_TAB), _native.HP_E_ARG),  # Hirshfeld-I without records
+        (dict(scheme=5, grid_type=3, **_TAB, **_REC), _native.HP_E_UNSUPP),
+        (dict(scheme=5, **dict(_TAB, tab_knots=np.tile([0.1, 0.5, 0.4, 2.0], 2)), **_REC), _native.HP_E_ARG),
+        (dict(scheme=6, **_TAB, **dict(_REC, rec_offsets=np.array([0, 2, 2], np.int32))), _native.HP_E_ARG),
+        (dict(scheme=7), _native.HP_E_ARG),
+    ],
+)
+def test_create_rejects_bad_tables(over, code):
+    """Tabulated pro-atoms (ISA, Hirshfeld-I, Hirshfeld) are validated on the CPU."""
+    arrays, scheme = _desc()
+    scheme.pop("fn_offsets")
+    scheme.update(over)
     with pytest.raises(_native.HPError) as exc:
         _native.DevicePartition(arrays, scheme)
     assert exc.value.code == code

@@ -574,3 +574,124 @@ def test_fix_propars_rule():
     assert list(ho.fix_propars(exps, pars, delta)) == [3]
     pars[3] = 1.0
     assert list(ho.fix_propars(exps, pars, delta)) == []
+
+
+# --- tabulated pro-atoms: Hirshfeld, Hirshfeld-I, ISA --------------------------
+
+
+def _oracle_db(numbers, max_cation, max_anion, level):
+    from common import proatom_files
+
+    return ho.OracleProAtomDB([ho.load_record_npz(f) for f in proatom_files(numbers, max_cation, max_anion, level)])
+
+
+def test_power_rgrid_pinned_by_record_populations():
+    """The records' radial grid r_i = rmin (i+1)^p, p = ln(rmax/rmin)/ln(n):
+    every record integrates to its (pseudo) population and the spline slope
+    through its densities matches its stored derivatives; the other reading
+    of the transform (r_i = rmin i^p', p' = ln(rmax/rmin)/ln(n-1)) fails both."""
+    from common import proatom_files
+    from scipy.interpolate import CubicSpline
+
+    files = proatom_files([1, 6, 8, 14], 4, -2, "hf_sto3g") + proatom_files([1, 8, 14], 4, -2, "hf_lan")
+    assert len(files) == 18
+    for path in files:
+        rec = ho.load_record_npz(path)
+        g = rec.rgrid
+        assert abs(g.integrate(4 * np.pi * g.points**2 * rec.rho) - (rec.pseudo_number - rec.charge)) < 1e-7
+        sel = slice(2, g.size - 3)
+        slope = CubicSpline(g.points, rec.rho)(g.points, 1)[sel]
+        assert np.median(np.abs(slope - rec.deriv[sel]) / np.abs(rec.deriv[sel])) < 3e-3
+        n = g.size
+        p2 = np.log(g.points[-1] / g.points[0]) / np.log(n - 1)
+        r2 = g.points[0] * np.arange(n) ** p2
+        w2 = np.gradient(r2)
+        slope2 = CubicSpline(r2, rec.rho)(r2, 1)[sel]
+        assert np.median(np.abs(slope2 - rec.deriv[sel]) / np.abs(rec.deriv[sel])) > 0.05
+        assert abs(np.sum(w2 * 4 * np.pi * r2**2 * rec.rho) - (rec.pseudo_number - rec.charge)) > 1e-3
+
+
+def test_hi_interpolated_proatom_population_closed_form():
+    """The Hirshfeld-I pro-atom at charge q holds Z_pseudo - q electrons
+    ((1 - x)(Z - i) + x (Z - i - 1) with i = floor q), to the records'
+    normalisation."""
+    db = _oracle_db([1, 8], 1, -1, "hf_sto3g")
+    o = ho.OracleHI(np.zeros((2, 3)), np.array([8, 1]), np.array([8.0, 1.0]), _two_atom_grid(), np.ones(2 * 6 * 6),
+                    proatomdb=db)
+    for q in (-0.73, -0.2, 0.0, 0.35, 0.999):
+        for i in (0, 1):
+            g = o.rgrid(i)
+            rho = o.get_proatom_rho(i, np.array([q, q]))
+            assert abs(g.integrate(4 * np.pi * g.points**2, rho) - (o.pseudo_numbers[i] - q)) < 1e-7
+    # the greedy evaluation equals the spline of the combined record (linear in the records)
+    charges = np.array([-0.4, 0.3])
+    for i in (0, 1):
+        icharge = int(np.floor(charges[i]))
+        x = charges[i] - icharge
+        combined = db.get_spline(o.numbers[i], icharge)(o.radial_distances[i]) * (1 - x)
+        if o.pseudo_numbers[i] - icharge > 1:
+            combined = combined + db.get_spline(o.numbers[i], icharge + 1)(o.radial_distances[i]) * x
+        assert np.allclose(o.eval_proatom(i, charges), combined + 1e-100, rtol=1e-14, atol=0)
+
+
+def _two_atom_grid():
+    from horton_part_amd.grid import BeckeWeights, MolGrid
+
+    return MolGrid.from_size(np.array([8, 1]), np.array([[0, 0, 0], [0, 0, 1.8]]), 6, exp_radial_grid(6), BeckeWeights())
+
+
+def test_hirshfeld_water_kat():
+    """tests/test_wpart.py:70-77: Hirshfeld charges of HF/STO-3G water within 2e-3."""
+    from common import WATER_H_CHARGES
+
+    coords, nums, pseudo, grid, dens = load_water_fixture()
+    o = ho.OracleHirshfeld(coords, nums, pseudo, grid, dens, proatomdb=_oracle_db([8, 6, 1], 1, -1, "hf_sto3g"))
+    c = o.do_partitioning()
+    assert np.abs(c["charges"] - WATER_H_CHARGES).max() < 2e-3
+    # the AIM weights of a point sum to one wherever the density matters
+    for i in range(3):
+        assert np.all(c[f"at_weights_{i}"] <= 1.0) and np.all(c[f"at_weights_{i}"] >= 0.0)
+
+
+def test_hirshfeld_i_water_kat():
+    """tests/test_wpart.py:80-87: Hirshfeld-I charges within 2e-3."""
+    from common import WATER_HI_CHARGES
+
+    coords, nums, pseudo, grid, dens = load_water_fixture()
+    o = ho.OracleHI(coords, nums, pseudo, grid, dens, proatomdb=_oracle_db([8, 6, 1], 1, -1, "hf_sto3g"))
+    c = o.do_partitioning()
+    assert np.abs(c["charges"] - WATER_HI_CHARGES).max() < 2e-3
+    assert c["change"] < 1e-6 and np.array_equal(c["propars"], c["charges"])
+
+
+def test_isa_water_kat():
+    """tests/test_wpart.py:90-92: ISA charges within 2e-3; the converged
+    parameters are the clipped spherical averages and the charges their
+    radial integrals."""
+    from common import WATER_IS_CHARGES
+
+    coords, nums, pseudo, grid, dens = load_water_fixture()
+    o = ho.OracleISA(coords, nums, pseudo, grid, dens)
+    c = o.do_partitioning()
+    assert np.abs(c["charges"] - WATER_IS_CHARGES).max() < 2e-3
+    for i in range(3):
+        g = o.rgrid(i)
+        sph = c[f"spherical_average_{i}"]
+        assert np.array_equal(o.get_proatom_rho(i, c["propars"]), sph) and sph.min() >= 1e-100
+        assert abs(pseudo[i] - g.integrate(4 * np.pi * g.points**2 * sph) - c["charges"][i]) < 1e-12
+
+
+@pytest.mark.parametrize("scheme", ["h", "hi", "is"])
+def test_monosilicic_acid_kats(scheme):
+    """tests/test_wpart.py:139-217: charges of HF/LANL2DZ monosilicic acid
+    (pseudo numbers of the LANL2DZ cores) within 4e-3."""
+    from common import MSA_H_CHARGES, MSA_HI_CHARGES, MSA_IS_CHARGES, load_msa_fixture
+
+    coords, nums, pseudo, grid, dens = load_msa_fixture()
+    if scheme == "is":
+        o, ref = ho.OracleISA(coords, nums, pseudo, grid, dens), MSA_IS_CHARGES
+    else:
+        db = _oracle_db([14, 8, 1], 4, -2, "hf_lan")
+        cls, ref = (ho.OracleHirshfeld, MSA_H_CHARGES) if scheme == "h" else (ho.OracleHI, MSA_HI_CHARGES)
+        o = cls(coords, nums, pseudo, grid, dens, proatomdb=db)
+    assert np.abs(o.do_partitioning()["charges"] - ref).max() < 4e-3
