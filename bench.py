@@ -552,6 +552,116 @@ def run_serving(args):
     print(json.dumps(out))
 
 
+# ---------------------------------------------------------------------------
+# tabulated pro-atoms (ISA, Hirshfeld): FP64 only
+# ---------------------------------------------------------------------------
+TAB_CHAIN_PROBS = {"C": 0.40, "H": 0.45, "O": 0.15}  # elements of the STO-3G pro-atom records
+
+
+def run_tabulated(args):
+    """ISA (``isa-chain20``: 200 outer iterations on the 20-atom chain, 150 x
+    302 grids) or Hirshfeld (``h-chain20``: one weights pass with the HF/STO-3G
+    neutral records of C, H and O on a C/H/O chain) timed as partitions/s
+    with inputs resident; roofline of the tabulated weights pass."""
+    import warnings
+
+    from horton_part_amd import HirshfeldWPart, ISAWPart, _native, workloads
+
+    warnings.simplefilter("ignore")
+    tests_dir = os.path.join(ROOT, "tests")
+    if tests_dir not in sys.path:
+        sys.path.insert(0, tests_dir)
+    isa = args.workload == "isa-chain20"
+    probs = None if isa else TAB_CHAIN_PROBS
+    w = workloads.chain(natom=20, seed=1001, nrad=150, nang=302, probs=probs, device=0)
+    maxiter = 200
+    if isa:
+        part = ISAWPart(w.coordinates, w.numbers, w.pseudo_numbers, w.grid, w.density, threshold=1e-12, maxiter=maxiter)
+        desc = f"ISA, {maxiter} outer iterations (threshold 1e-12 not reached)"
+    else:
+        from common import load_proatomdb
+
+        db = load_proatomdb([1, 6, 8], 0, 0, "hf_sto3g")
+        part = HirshfeldWPart(w.coordinates, w.numbers, w.pseudo_numbers, w.grid, w.density, db)
+        desc = "Hirshfeld, HF/STO-3G neutral records of C, H, O"
+    dev = part._ensure_device()
+    p0 = np.zeros(dev.npar)
+    lat = []
+    for i in range(args.warmup + args.steps):
+        t0 = time.perf_counter()
+        dev.set_propars(p0)
+        if isa:
+            rc = dev.run("fp64", 1)
+            if rc != 0:
+                raise RuntimeError(f"device run failed with {rc}")
+        else:
+            dev.weights(spherical=False)
+            dev.populations()
+        if i >= args.warmup:
+            lat.append(time.perf_counter() - t0)
+    lat = np.array(lat)
+    ms, n = dev.kernel_time(_native.KERNEL_WEIGHTS)
+    wk = dev.work()
+    t_w = ms / max(n, 1) * 1e-3
+    achieved = wk["bytes_per_pass"] / t_w / 1e9
+    evals = w.npoint * w.natom
+    peaks = measured_peaks()
+    out = {
+        "metric": "partitions/s (tabulated pro-atoms, FP64)",
+        "value": 1.0 / float(np.mean(lat)),
+        "unit": "partitions/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": float(np.mean(lat)) * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": f"{args.workload}: {desc}, {w.natom} atoms, {w.npoint} grid points"},
+        "niter": dev.niter if isa else 1,
+        "roofline": {
+            "kernel": "k_table_weights (spline pro-atoms: interval search and cubic per point and atom, promolecule, "
+                      "weights, per-shell sums)",
+            "bound": "hbm",
+            "achieved": achieved,
+            "peak": HBM_PEAK_GBPS,
+            "unit": "GB/s",
+            "frac": achieved / HBM_PEAK_GBPS,
+            "traffic": None,
+            "bytes_per_launch": wk["bytes_per_pass"],
+            "launch_us": t_w * 1e6,
+            "spline_evals_per_s": evals / t_w,
+        },
+        "measured": peaks,
+    }
+    if not args.no_cpu:
+        from oracle import horton_oracle as ho
+
+        log("CPU baseline (numpy oracle, 1 thread, bounded sample)")
+        from threadpoolctl import threadpool_limits
+
+        with threadpool_limits(1):
+            t0 = time.perf_counter()
+            if isa:
+                ho.OracleISA(w.coordinates, w.numbers, w.pseudo_numbers, w.grid, w.density, maxiter=2).do_partitioning()
+            else:
+                odb = ho.OracleProAtomDB([ho.load_record_npz(f) for f in __import__("common").proatom_files([1, 6, 8], 0, 0, "hf_sto3g")])
+                ho.OracleHirshfeld(w.coordinates, w.numbers, w.pseudo_numbers, w.grid, w.density, proatomdb=odb).do_partitioning()
+            dt = time.perf_counter() - t0
+        per = dt / 2 * maxiter if isa else dt
+        out["cpu_baseline"] = {
+            "value": 1.0 / per,
+            "unit": "partitions/s",
+            "cores": 1,
+            "kind": "port",
+            "sample": (f"2 outer iterations of the numpy oracle on the same workload ({dt:.2f} s), extrapolated to {maxiter}"
+                       if isa else f"the numpy oracle's Hirshfeld pass on the same workload ({dt:.2f} s)"),
+        }
+    print(json.dumps(out))
+
+
 def measured_peaks():
     """Device copy bandwidth and the exp throughput of both precisions."""
     from horton_part_amd import _native
@@ -581,7 +691,8 @@ def main():
     ap.add_argument("--switch-interval", type=float, default=0.0,
                     help="serving: interpreter thread switch interval in seconds (0: the default)")
     ap.add_argument("--workload", default="serving",
-                    choices=["serving", "water", "chain20", "chain20-gisa", "cluster200", "protein2000"])
+                    choices=["serving", "water", "chain20", "chain20-gisa", "cluster200", "protein2000", "isa-chain20",
+                             "h-chain20"])
     ap.add_argument("--ticks", type=int, default=8, help="serving: outer iterations per host step")
     ap.add_argument("--groups", type=int, default=0, help="serving: molecule groups interleaved per tick (1-4; 0: 3 with >= 8 hardware queues, else 2)")
     ap.add_argument("--inner-budget", type=int, default=0,
@@ -609,6 +720,8 @@ def main():
         return run_serving(args)
     args.steps = 5 if args.steps is None else args.steps
     args.warmup = 1 if args.warmup is None else args.warmup
+    if args.workload in ("isa-chain20", "h-chain20"):
+        return run_tabulated(args)
     log(f"building workload {args.workload} on the GPU")
     t0 = time.perf_counter()
     w, cls, kw, desc = build_workload(args.workload, device=0)

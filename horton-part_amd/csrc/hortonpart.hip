@@ -102,7 +102,8 @@ struct hp_part {
   // the current cubic pieces, HI's current knot densities and records, and
   // the per-shell atom-grid populations of the last weights pass
   int ntab = 0;
-  int *tab_off = nullptr, *rec_off = nullptr, *rec_charge = nullptr;
+  int *tab_off = nullptr, *rec_off = nullptr, *rec_charge = nullptr, *tab_guide = nullptr;
+  float2 *tab_gparam = nullptr;
   int64_t *rec_val = nullptr;
   double *tab_x = nullptr, *tab_w = nullptr, *tab_c = nullptr, *cur_rho = nullptr, *rec_rho = nullptr,
          *rec_c = nullptr, *pop_part = nullptr;
@@ -155,6 +156,8 @@ struct hp_part {
     f.tab_off = tab_off;
     f.tab_x = tab_x;
     f.tab_c = tab_c;
+    f.tab_guide = tab_guide;
+    f.tab_gparam = tab_gparam;
     return f;
   }
   DevRec rec() const {
@@ -191,7 +194,7 @@ void free_part(hp_part *p) {
                   p->hent,    p->hchg,    p->corr,      p->evals_done, p->grp_first, p->gl_G, p->gl_D,
                   p->gl_E,    p->gl_obj,  p->gl_grad,   p->gl_hess,  p->gl_out,    p->gl_coef, p->gl_pro,
                   p->gl_pairs, p->tab_off, p->rec_off, p->rec_charge, p->rec_val, p->tab_x, p->tab_w,
-                  p->tab_c,   p->cur_rho, p->rec_rho,   p->rec_c,    p->pop_part};
+                  p->tab_c,   p->cur_rho, p->rec_rho,   p->rec_c,    p->pop_part, p->tab_guide, p->tab_gparam};
   for (void *b : bufs)
     if (b) (void)hipFree(b);
   if (p->h_out) (void)hipHostFree(p->h_out);
@@ -875,8 +878,29 @@ int hp_part_create(int device, const hp_molecule_desc *md, const hp_scheme_desc 
   std::vector<int> fn_off(A + 1, 0);
   if (!table) fn_off.assign(sd->fn_offsets, sd->fn_offsets + A + 1);
   if ((rc = dupload(&p->fn_off, fn_off.data(), A + 1, s))) return bail(rc);
+  // host copies live until the stream synchronisation at the end of create
+  std::vector<int> toff, guide, roff, rch;
+  std::vector<float2> gparam;
   if (table) {
-    std::vector<int> toff(sd->tab_offsets, sd->tab_offsets + A + 1);
+    toff.assign(sd->tab_offsets, sd->tab_offsets + A + 1);
+    // interval guide per atom: bin j of log2 r starts at the interval that
+    // holds rmin 2^(j / bins-per-unit) (the kernels walk to the exact one)
+    guide.resize((size_t)A * HP_TAB_GUIDE);
+    gparam.resize(A);
+    for (int a = 0; a < A; ++a) {
+      const double *x = sd->tab_knots + toff[a];
+      const int n = toff[a + 1] - toff[a];
+      const double u0 = std::log2(x[0]), u1 = std::log2(x[n - 1]);
+      const double per = (u1 > u0) ? (HP_TAB_GUIDE - 1) / (u1 - u0) : 0.0;
+      gparam[a] = make_float2((float)u0, (float)per);
+      for (int j = 0; j < HP_TAB_GUIDE; ++j) {
+        const double r = per > 0.0 ? std::exp2(u0 + j / per) : x[0];
+        guide[(size_t)a * HP_TAB_GUIDE + j] = table_interval(x, n, r);
+      }
+    }
+    if ((rc = dupload(&p->tab_guide, guide.data(), guide.size(), s)) ||
+        (rc = dupload(&p->tab_gparam, gparam.data(), gparam.size(), s)))
+      return bail(rc);
     if ((rc = dupload(&p->tab_off, toff.data(), A + 1, s)) || (rc = dupload(&p->tab_x, sd->tab_knots, ntab, s)) ||
         (rc = dupload(&p->tab_w, sd->tab_rweights, ntab, s)) || (rc = dalloc(&p->tab_c, (size_t)4 * ntab)) ||
         (rc = dalloc(&p->cur_rho, ntab)) || (rc = dalloc(&p->pop_part, S)))
@@ -884,7 +908,8 @@ int hp_part_create(int device, const hp_molecule_desc *md, const hp_scheme_desc 
     if (sd->scheme != HP_SCHEME_ISA) {
       const int nrec = sd->rec_offsets[A];
       const int64_t nval = rec_val.empty() ? 0 : rec_val.back() + (sd->tab_offsets[A] - sd->tab_offsets[A - 1]);
-      std::vector<int> roff(sd->rec_offsets, sd->rec_offsets + A + 1), rch(sd->rec_charges, sd->rec_charges + nrec);
+      roff.assign(sd->rec_offsets, sd->rec_offsets + A + 1);
+      rch.assign(sd->rec_charges, sd->rec_charges + nrec);
       if ((rc = dupload(&p->rec_off, roff.data(), A + 1, s)) || (rc = dupload(&p->rec_charge, rch.data(), nrec, s)) ||
           (rc = dupload(&p->rec_val, rec_val.data(), rec_val.size(), s)) ||
           (rc = dupload(&p->rec_rho, sd->rec_rho, nval, s)) || (rc = dupload(&p->rec_c, sd->rec_coefs, 4 * nval, s)))
@@ -1444,7 +1469,10 @@ int hp_part_get_work(hp_part *p, hp_part_work *out) {
   HP_HIP(hipSetDevice(p->device));
   out->points = p->npoint;
   out->evals_per_pass = p->evals_per_pass;
-  out->bytes_per_pass = 48.0 * (double)p->npoint + 16.0 * (double)p->nshell;
+  // tabulated weights pass: x, y, z, rho and three weights per point (and
+  // the promolecule on the molecular grid), three sums per shell
+  out->bytes_per_pass = p->tabulated() ? (56.0 + (p->promol ? 8.0 : 0.0)) * (double)p->npoint + 24.0 * (double)p->nshell
+                                       : 48.0 * (double)p->npoint + 16.0 * (double)p->nshell;
   out->hessian_passes = 0;
   out->hessian_flops = 0.0;
   if (int rc = hp::read_eval_counters(p->evals_done, &out->evals_done_fp32, &out->evals_done_fp64)) return rc;

@@ -127,7 +127,13 @@ struct DevFn {
   const int *tab_off = nullptr;
   const double *tab_x = nullptr;
   const double *tab_c = nullptr;
+  // interval guide: per atom HP_TAB_GUIDE start intervals over uniform bins
+  // of log2 r, and (log2 of the first knot, bins per unit of log2 r)
+  const int *tab_guide = nullptr;
+  const float2 *tab_gparam = nullptr;
 };
+
+#define HP_TAB_GUIDE 256
 
 // Counters of performed function evaluations: 64 per precision, one per
 // 128-byte line, picked by workgroup, so that the workgroups of a launch do
@@ -350,9 +356,19 @@ template <typename R, int FK>
 __device__ __forceinline__ R proatom_value(const DevFn &fn, int b, R r, R r2) {
   if (FK == HP_FN_TABLE) {
     // the spline plus the 1e-100 of eval_proatom (core/stockholder.py:347-349,
-    // hirshfeld_i.py:158)
-    const int k0 = fn.tab_off[b];
-    return (R)(table_eval(fn.tab_x + k0, fn.tab_c + 4 * (int64_t)k0, fn.tab_off[b + 1] - k0, (double)r) + 1e-100);
+    // hirshfeld_i.py:158); the interval starts from the guide bin of log2 r
+    // and walks to the exact one (the largest i <= n - 2 with x_i <= r, or 0)
+    const int k0 = fn.tab_off[b], n = fn.tab_off[b + 1] - k0;
+    const double *x = fn.tab_x + k0;
+    const double rd = (double)r;
+    const float2 gp = fn.tab_gparam[b];
+    float t = (__log2f((float)rd) - gp.x) * gp.y;
+    t = t > 0.0f ? t : 0.0f;  // below the first knot, r = 0 and NaN
+    t = t < (float)(HP_TAB_GUIDE - 1) ? t : (float)(HP_TAB_GUIDE - 1);
+    int i = fn.tab_guide[b * HP_TAB_GUIDE + (int)t];
+    while (i < n - 2 && x[i + 1] <= rd) ++i;
+    while (i > 0 && x[i] > rd) --i;
+    return (R)(table_piece(fn.tab_c + 4 * (int64_t)(k0 + i), rd - x[i]) + 1e-100);
   }
   R pb = (R)0;
   const int f1 = fn.fn_off[b + 1];

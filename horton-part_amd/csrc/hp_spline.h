@@ -101,18 +101,15 @@ HP_HD inline int table_interval(const double *x, int n, double r) {
   return lo;
 }
 
-// Value of the table at r: c3 + c2 d + c1 d^2 + c0 d^3 with the powers of d
+// Value of a piece at d = r - x_i: c3 + c2 d + c1 d^2 + c0 d^3 with the powers of d
 // built by repeated products and every product and sum rounded on its own
 // (the operations of scipy's PPoly evaluation, without fused multiply-adds).
 // Across a wide last interval a density's cubic cancels many digits, so the
 // order of operations shows in its far tail.
-HP_HD inline double table_eval(const double *x, const double *coef, int n, double r) {
+HP_HD inline double table_piece(const double *c, double d) {
 #ifdef __clang__
 #pragma clang fp contract(off)
 #endif
-  const int i = table_interval(x, n, r);
-  const double d = r - x[i];
-  const double *c = coef + 4 * i;
   double z = d;
   double v = c[3] + c[2] * z;
   z *= d;
@@ -120,4 +117,9 @@ HP_HD inline double table_eval(const double *x, const double *coef, int n, doubl
   z *= d;
   v += c[0] * z;
   return v;
+}
+
+HP_HD inline double table_eval(const double *x, const double *coef, int n, double r) {
+  const int i = table_interval(x, n, r);
+  return table_piece(coef + 4 * i, r - x[i]);
 }

@@ -332,15 +332,15 @@ def partition_density(fn_in, fn_out, type="lisa", part_job_type="do_partitioning
         "threshold": opts["threshold"],
         "inner_threshold": opts["inner_threshold"] if type != "glisa" else np.nan,
         "solver": opts["solver"],
-        "charges": part.cache["charges"],
+        "charges": part["charges"],
         "time": elapsed,
-        "time_update_at_weights": part.cache["time_update_at_weights"],
-        "time_update_propars": part.cache["time_update_propars"],
-        "niter": part.cache["niter"],
-        "history_charges": part.cache["history_charges"],
-        "history_propars": part.cache["history_propars"],
-        "history_entropies": part.cache["history_entropies"],
-        "history_changes": part.cache["history_changes"],
+        "time_update_at_weights": part["time_update_at_weights"],
+        "time_update_propars": part["time_update_propars"],
+        "niter": part["niter"],
+        "history_charges": part["history_charges"],
+        "history_propars": part["history_propars"],
+        "history_entropies": part["history_entropies"],
+        "history_changes": part["history_changes"],
     }
     np.savez_compressed(fn_out, **out)
     return part
