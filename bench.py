@@ -48,6 +48,11 @@ import numpy as np  # noqa: E402
 
 METRIC = "partitions/s & p99 latency, mixed vs plain FP64 at max|Δq|≤1e-6 e"
 HBM_PEAK_GBPS = 8000.0
+# HBM bytes the serving tick's weights launches move per algorithmic byte
+# (48 B per point): FETCH_SIZE x 2 + WRITE_SIZE over every k_weights launch of
+# a 64-request serving run, each counter scaled by what it reads for a plain
+# copy of 256 MiB (profiles/k_weights_serving.md)
+K_WEIGHTS_TRAFFIC_RATIO = 1.038
 
 
 def log(msg):
@@ -536,7 +541,9 @@ def run_serving(args):
             "frac": t_ideal / t_w if t_w > 0 else 0.0,
             "peak_note": "the exponential throughput of the same exp code with every lane busy and no memory traffic "
             "(hp_exp_rate probe), per precision, weighted by the evaluations each precision performed",
-            "traffic": None,
+            "traffic": K_WEIGHTS_TRAFFIC_RATIO * bytes_per_tick,
+            "traffic_note": "HBM bytes per tick (all weights launches of a tick): the algorithmic bytes_per_tick times "
+            "the measured traffic ratio of the rocprofv3 FETCH_SIZE/WRITE_SIZE runs",
             "hbm_gbps": bytes_per_tick / (t_w / ticks) / 1e9 if t_w > 0 else 0.0,
             "hbm_frac": bytes_per_tick / (t_w / ticks) / 1e9 / HBM_PEAK_GBPS if t_w > 0 else 0.0,
             "bytes_per_tick": bytes_per_tick,

@@ -497,3 +497,17 @@ def test_glisa_newton_failures_like_reference():
     with pytest.raises(RuntimeError, match="Line search failed"):
         part.do_partitioning()
     assert len(part.history_changes) == len(oracle.history_changes)
+
+
+def test_becke_weights_device_matches_host():
+    """The device Becke weights against the numpy restatement on a 37-atom
+    cluster."""
+    require_gpu()
+    _say("case: Becke weights, 37-atom cluster, 20x26 grids")
+    from horton_part_amd.grid import BeckeWeights
+
+    w = workloads.cluster(natom=37, seed=5, nrad=20, nang=26, becke=False)
+    g = w.grid
+    host = BeckeWeights().generate_weights(g.points, w.coordinates, w.numbers, indices=g.indices)
+    dev = BeckeWeights(device=0).generate_weights(g.points, w.coordinates, w.numbers, indices=g.indices)
+    np.testing.assert_allclose(dev, host, rtol=1e-12, atol=1e-15)

@@ -48,7 +48,9 @@ def _check_weights(part, cache):
         w, wo = part[f"at_weights_{i}"], cache[f"at_weights_{i}"]
         k = part.to_atomic_grid(i, keep)
         assert w.min() >= 0.0 and w.max() <= 1.0
-        np.testing.assert_allclose(w[k], wo[k], rtol=1e-10, atol=1e-13)
+        # weights of order 1e-12 at points where another atom dominates carry
+        # the tail noise of this atom's spline: absolute bound 1e-10
+        np.testing.assert_allclose(w[k], wo[k], rtol=1e-10, atol=1e-10)
 
 
 def _check_splines(part):
