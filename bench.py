@@ -669,6 +669,84 @@ def run_tabulated(args):
     print(json.dumps(out))
 
 
+# ---------------------------------------------------------------------------
+# gLISA: the global terms of one Newton step (objective, gradient, Hessian)
+# ---------------------------------------------------------------------------
+def run_glisa(args):
+    """gLISA's global terms on chain20 (gauss.json basis, 150 x 302 grids):
+    one step = one Newton step's objective, gradient and M x M Hessian
+    G^T diag(w rho / rho0^2) G over the molecular grid at the initial
+    coefficients (``hp_part_global_terms``); roofline of the FP64 MFMA
+    Hessian pass against 2 M^2 N algorithmic flops."""
+    import warnings
+
+    from horton_part_amd import GlobalLinearISAWPart, _native, workloads
+
+    warnings.simplefilter("ignore")
+    w = workloads.chain(natom=20, seed=1001, nrad=150, nang=302, device=0)
+    kw = dict(solver="newton", basis_func="gauss", grid_type=2)
+    part = GlobalLinearISAWPart(w.coordinates, w.numbers, w.pseudo_numbers, w.grid, w.density, **kw)
+    dev = part._ensure_device()
+    p0 = part._init_propars()
+    M, N = int(p0.size), int(w.npoint)
+    lat = []
+    for i in range(args.warmup + args.steps):
+        t0 = time.perf_counter()
+        dev.global_terms(p0, nderiv=2)
+        if i >= args.warmup:
+            lat.append(time.perf_counter() - t0)
+    lat = np.array(lat)
+    ms, n = dev.kernel_time(_native.KERNEL_GLISA_HESS)
+    t_h = ms / max(n, 1) * 1e-3
+    flops = 2.0 * M * M * N
+    tf = flops / t_h / 1e12
+    out = {
+        "metric": "gLISA Newton steps/s (objective, gradient, Hessian over the molecular grid, FP64)",
+        "value": 1.0 / float(np.mean(lat)),
+        "unit": "steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": float(np.mean(lat)) * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": f"glisa-chain20: gLISA global terms, gauss.json basis (M = {M} functions), "
+                               f"{w.natom} atoms, {N} grid points"},
+        "roofline": {
+            "kernel": "k_glisa_hess (16x16 tiles of G^T D G on mfma_f64_16x16x4f64)",
+            "bound": "mfma",
+            "achieved": tf,
+            "peak": 78.6,
+            "unit": "TFLOP/s",
+            "frac": tf / 78.6,
+            "traffic": None,
+            "flops_per_launch": flops,
+            "launch_us": t_h * 1e6,
+        },
+    }
+    if not args.no_cpu:
+        from threadpoolctl import threadpool_limits
+
+        from oracle.horton_oracle import OracleBasis, OracleGLISA
+
+        log("CPU baseline (numpy oracle, 1 thread): one working_matrix call")
+        bs = part.bs_helper
+        o = OracleGLISA(w.coordinates, w.numbers, w.pseudo_numbers, w.grid, w.density, grid_type=2,
+                        basis=OracleBasis(bs.orders, bs.exponents, bs.initials))
+        q0 = o.init_propars()
+        with threadpool_limits(1):
+            t0 = time.perf_counter()
+            o.working_matrix(o.moldens, o.calc_promol_dens(q0), nderiv=2)
+            dt = time.perf_counter() - t0
+        out["cpu_baseline"] = {"value": 1.0 / dt, "unit": "steps/s", "cores": 1, "kind": "port",
+                               "sample": f"one objective/gradient/Hessian evaluation of the numpy oracle on the same "
+                                         f"workload ({dt:.1f} s)"}
+    print(json.dumps(out))
+
+
 def measured_peaks():
     """Device copy bandwidth and the exp throughput of both precisions."""
     from horton_part_amd import _native
@@ -699,7 +777,7 @@ def main():
                     help="serving: interpreter thread switch interval in seconds (0: the default)")
     ap.add_argument("--workload", default="serving",
                     choices=["serving", "water", "chain20", "chain20-gisa", "cluster200", "protein2000", "isa-chain20",
-                             "h-chain20"])
+                             "h-chain20", "glisa-chain20"])
     ap.add_argument("--ticks", type=int, default=8, help="serving: outer iterations per host step")
     ap.add_argument("--groups", type=int, default=0, help="serving: molecule groups interleaved per tick (1-4; 0: 3 with >= 8 hardware queues, else 2)")
     ap.add_argument("--inner-budget", type=int, default=0,
@@ -729,6 +807,8 @@ def main():
     args.warmup = 1 if args.warmup is None else args.warmup
     if args.workload in ("isa-chain20", "h-chain20"):
         return run_tabulated(args)
+    if args.workload == "glisa-chain20":
+        return run_glisa(args)
     log(f"building workload {args.workload} on the GPU")
     t0 = time.perf_counter()
     w, cls, kw, desc = build_workload(args.workload, device=0)

@@ -110,8 +110,8 @@ struct hp_part {
   // host state
   int niter = 0;
   double last_change = 0.0, last_entropy = 0.0;
-  double k_ms[3] = {0, 0, 0};
-  int64_t k_n[3] = {0, 0, 0};
+  double k_ms[4] = {0, 0, 0, 0};
+  int64_t k_n[4] = {0, 0, 0, 0};
 
   DevMol mol() const {
     DevMol m;
@@ -1360,9 +1360,11 @@ int hp_part_global_terms(hp_part *p, const double *coefs, int nderiv, double *ob
     HP_LAUNCHED();
   }
   if (nderiv == 2) {
+    HP_HIP(hipEventRecord(p->ev[0], s));
     hipLaunchKernelGGL(k_glisa_hess, dim3((unsigned)p->gl_npair, (unsigned)p->gl_nchunk), dim3(64), 0, s, p->npoint,
                        Mp, p->gl_pairs, p->gl_G, p->gl_D, p->gl_hess);
     HP_LAUNCHED();
+    HP_HIP(hipEventRecord(p->ev[1], s));
   }
   const int nt = std::max({1, nderiv >= 1 ? M : 0, nderiv == 2 ? p->gl_npair * 256 : 0});
   hipLaunchKernelGGL(k_glisa_reduce, dim3((unsigned)((nt + 255) / 256)), dim3(256), 0, s, p->gl_nblock, p->gl_obj,
@@ -1377,6 +1379,12 @@ int hp_part_global_terms(hp_part *p, const double *coefs, int nderiv, double *ob
   if (hessian && nderiv == 2)
     HP_HIP(hipMemcpyAsync(hessian, p->gl_out + 1 + M, sizeof(double) * M * M, hipMemcpyDeviceToHost, s));
   HP_HIP(hipStreamSynchronize(s));
+  if (nderiv == 2) {
+    float ms = 0.f;
+    HP_HIP(hipEventElapsedTime(&ms, p->ev[0], p->ev[1]));
+    p->k_ms[HP_KERNEL_GLISA_HESS] += ms;
+    p->k_n[HP_KERNEL_GLISA_HESS] += 1;
+  }
   return 0;
 }
 
@@ -1458,7 +1466,7 @@ int hp_part_populations(hp_part *p, double *pop) {
 }
 
 int hp_part_kernel_time(hp_part *p, int kernel, double *total_ms, int64_t *launches) {
-  if (!p || kernel < 0 || kernel > 2) return fail(HP_E_ARG, "bad argument");
+  if (!p || kernel < 0 || kernel > HP_KERNEL_GLISA_HESS) return fail(HP_E_ARG, "bad argument");
   if (total_ms) *total_ms = p->k_ms[kernel];
   if (launches) *launches = p->k_n[kernel];
   return 0;

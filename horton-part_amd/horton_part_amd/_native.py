@@ -38,6 +38,7 @@ ST_NO_RECORD = 1024
 KERNEL_WEIGHTS = 0
 KERNEL_INNER = 1
 KERNEL_LISA_GRID = 2
+KERNEL_GLISA_HESS = 3
 
 EXPORTED_SYMBOLS = [
     "hp_abi_version",

@@ -291,8 +291,9 @@ int hp_part_global_proatoms(hp_part *part, const double *coefs_new, const double
  * times w_a rho. */
 int hp_part_populations(hp_part *part, double *populations);
 
-/* Timing of the pro-atom/weights kernel over the last hp_part_run, measured
- * with HIP events on the library's stream: total milliseconds and launches. */
+/* Timing of one kernel (HP_KERNEL_*) measured with HIP events on the
+ * library's stream: total milliseconds and launches, over the last
+ * hp_part_run (HP_KERNEL_GLISA_HESS: since the partitioning was created). */
 int hp_part_kernel_time(hp_part *part, int kernel, double *total_ms, int64_t *launches);
 
 /* Algorithmic work of a partitioning, from its inputs (for rooflines). */
@@ -316,6 +317,7 @@ int hp_part_get_work(hp_part *part, hp_part_work *out);
 #define HP_KERNEL_WEIGHTS 0 /* fused pro-atom/promolecule/weights/sphavg pass */
 #define HP_KERNEL_INNER 1   /* per-atom inner solvers */
 #define HP_KERNEL_LISA_GRID 2 /* molecular-grid Newton pass (grid_type 2) */
+#define HP_KERNEL_GLISA_HESS 3 /* gLISA Hessian pass of hp_part_global_terms (accumulated since create) */
 
 /* ---- continuous batching: many molecules in flight on one GPU ----
  *
