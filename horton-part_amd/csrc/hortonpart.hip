@@ -73,6 +73,7 @@ struct hp_part {
   double *par = nullptr, *old_par = nullptr, *rhobar = nullptr, *ent_part = nullptr, *promol = nullptr,
          *charges = nullptr, *change_part = nullptr, *out = nullptr, *mom_part = nullptr, *mom_out = nullptr,
          *scratch_pts = nullptr;
+  size_t mom_part_cap = 0, mom_out_cap = 0;  // doubles held by mom_part, mom_out
   int *inner_niter = nullptr, *status = nullptr, *status_all = nullptr;
   // grid_type 2: per-atom solver states, chunk partials, active-atom counter
   Gt2Atom *gt2 = nullptr;
@@ -597,6 +598,76 @@ int launch_moments(hp_part *p, double *part_buf, double *out_buf, bool molgrid) 
     HP_LAUNCHED();
     hipLaunchKernelGGL(k_moments_reduce, dim3(p->natom), dim3(64), 0, p->stream, p->mol(), part_buf, out_buf);
   }
+  HP_LAUNCHED();
+  return 0;
+}
+
+// The moments of order lmax as rows of the per-atom sums: lmax <= 3 takes a
+// prefix of each family from the 40 sums of k_moments (cart 20 | pure 16 |
+// radial 4); higher orders have rows of nq = ncart + npure + nrad from
+// k_moments_any.
+struct MomShape {
+  int ncart, npure, nrad;
+  int stride, src_cart, src_pure, src_rad;
+  bool fast;
+};
+
+MomShape moment_shape(int lmax) {
+  MomShape s;
+  s.ncart = (lmax + 1) * (lmax + 2) * (lmax + 3) / 6;
+  s.npure = (lmax + 1) * (lmax + 1);
+  s.nrad = lmax + 1;
+  s.fast = lmax <= 3;
+  if (s.fast) {
+    s.stride = HP_NMOM;
+    s.src_cart = 0;
+    s.src_pure = 20;
+    s.src_rad = 36;
+  } else {
+    s.stride = s.ncart + s.npure + s.nrad;
+    s.src_cart = 0;
+    s.src_pure = s.ncart;
+    s.src_rad = s.ncart + s.npure;
+  }
+  return s;
+}
+
+// The moments passes for order lmax: partials into part_buf
+// (moments_partials rows of ms.stride), per-atom sums into out_buf (natom
+// rows of ms.stride).
+int launch_moments_lmax(hp_part *p, int lmax, const MomShape &ms, double *part_buf, double *out_buf, bool molgrid) {
+  if (ms.fast) return launch_moments(p, part_buf, out_buf, molgrid);
+  DevFn f = p->fn();
+  const int nq = ms.stride;
+  const unsigned ngrp = (unsigned)((nq + HP_MOM_GROUP - 1) / HP_MOM_GROUP);
+  const int nchunk = molgrid ? (int)((p->npoint + HP_MOM_CHUNK - 1) / HP_MOM_CHUNK) : 0;
+  const dim3 g = molgrid ? dim3((unsigned)nchunk, (unsigned)p->natom, ngrp) : dim3((unsigned)p->nshell, 1, ngrp);
+  const double *pm = p->promol;
+#define HP_MOM_ANY(FK, MOL) \
+  hipLaunchKernelGGL((k_moments_any<FK, MOL>), g, dim3(256), 0, p->stream, p->mol(), f, pm, lmax, nq, part_buf)
+  if (molgrid) {
+    if (p->fnkind == HP_FN_TABLE)
+      HP_MOM_ANY(HP_FN_TABLE, true);
+    else if (p->fnkind == HP_FN_SLATER)
+      HP_MOM_ANY(HP_FN_SLATER, true);
+    else if (p->fnkind == HP_FN_GAUSS)
+      HP_MOM_ANY(HP_FN_GAUSS, true);
+    else
+      HP_MOM_ANY(HP_FN_GENERAL, true);
+  } else {
+    if (p->fnkind == HP_FN_TABLE)
+      HP_MOM_ANY(HP_FN_TABLE, false);
+    else if (p->fnkind == HP_FN_SLATER)
+      HP_MOM_ANY(HP_FN_SLATER, false);
+    else if (p->fnkind == HP_FN_GAUSS)
+      HP_MOM_ANY(HP_FN_GAUSS, false);
+    else
+      HP_MOM_ANY(HP_FN_GENERAL, false);
+  }
+#undef HP_MOM_ANY
+  HP_LAUNCHED();
+  hipLaunchKernelGGL(k_moments_reduce_any, dim3(p->natom), dim3(128), 0, p->stream, p->mol(), molgrid ? 1 : 0, nchunk,
+                     nq, part_buf, out_buf);
   HP_LAUNCHED();
   return 0;
 }
@@ -1250,59 +1321,81 @@ int hp_part_promoldens(hp_part *p, double *out, int64_t n) {
 
 int hp_part_moments(hp_part *p, int lmax, double *cart, double *pure, double *radial) {
   if (!p) return fail(HP_E_ARG, "NULL part");
-  if (lmax != 3) return fail(HP_E_UNSUPP, "the device moments are implemented for lmax = 3");
+  if (lmax < 0) return fail(HP_E_ARG, "lmax must be >= 0");
   if (p->niter == 0 && !p->weights_ready) return fail(HP_E_ARG, "run the partitioning first");
   HP_HIP(hipSetDevice(p->device));
   int rc = 0;
-  const size_t npart = moments_partials(p, p->only_molgrid);
-  if (!p->mom_part && ((rc = dalloc(&p->mom_part, npart * HP_NMOM)) ||
-                       (rc = dalloc(&p->mom_out, (size_t)p->natom * HP_NMOM))))
-    return rc;
-  if ((rc = launch_moments(p, p->mom_part, p->mom_out, p->only_molgrid))) return rc;
-  std::vector<double> h((size_t)p->natom * HP_NMOM);
+  const MomShape ms = moment_shape(lmax);
+  const size_t npart = moments_partials(p, p->only_molgrid) * (size_t)ms.stride;
+  const size_t nout = (size_t)p->natom * (size_t)ms.stride;
+  if (p->mom_part_cap < npart) {
+    if (p->mom_part) HP_HIP(hipFree(p->mom_part));
+    p->mom_part = nullptr;
+    p->mom_part_cap = 0;
+    if ((rc = dalloc(&p->mom_part, npart))) return rc;
+    p->mom_part_cap = npart;
+  }
+  if (p->mom_out_cap < nout) {
+    if (p->mom_out) HP_HIP(hipFree(p->mom_out));
+    p->mom_out = nullptr;
+    p->mom_out_cap = 0;
+    if ((rc = dalloc(&p->mom_out, nout))) return rc;
+    p->mom_out_cap = nout;
+  }
+  if ((rc = launch_moments_lmax(p, lmax, ms, p->mom_part, p->mom_out, p->only_molgrid))) return rc;
+  std::vector<double> h(nout);
   std::vector<double> z(p->natom);
   HP_HIP(hipMemcpyAsync(h.data(), p->mom_out, sizeof(double) * h.size(), hipMemcpyDeviceToHost, p->stream));
   HP_HIP(hipMemcpyAsync(z.data(), p->pseudo, sizeof(double) * p->natom, hipMemcpyDeviceToHost, p->stream));
   HP_HIP(hipStreamSynchronize(p->stream));
   for (int a = 0; a < p->natom; ++a) {
-    const double *v = &h[(size_t)a * HP_NMOM];
+    const double *v = &h[(size_t)a * ms.stride];
     if (cart) {
-      for (int i = 0; i < 20; ++i) cart[a * 20 + i] = -v[i];
-      cart[a * 20] += z[a];
+      double *c = cart + (size_t)a * ms.ncart;
+      for (int i = 0; i < ms.ncart; ++i) c[i] = -v[ms.src_cart + i];
+      c[0] += z[a];
     }
     if (pure) {
-      for (int i = 0; i < 16; ++i) pure[a * 16 + i] = -v[20 + i];
-      pure[a * 16] += z[a];
+      double *u = pure + (size_t)a * ms.npure;
+      for (int i = 0; i < ms.npure; ++i) u[i] = -v[ms.src_pure + i];
+      u[0] += z[a];
     }
-    if (radial)
-      for (int i = 0; i < 4; ++i) radial[a * 4 + i] = v[36 + i];
+    if (radial) {
+      double *r = radial + (size_t)a * ms.nrad;
+      for (int i = 0; i < ms.nrad; ++i) r[i] = v[ms.src_rad + i];
+    }
   }
   return 0;
 }
 
 int hp_part_moments_static(hp_part *p, int lmax, double *cart, double *pure, double *radial) {
   if (!p) return fail(HP_E_ARG, "NULL part");
-  if (lmax != 3) return fail(HP_E_UNSUPP, "the device moments are implemented for lmax = 3");
+  if (lmax < 0) return fail(HP_E_ARG, "lmax must be >= 0");
   if (p->niter == 0 && !p->weights_ready) return fail(HP_E_ARG, "run the partitioning first");
   // partials, per-atom sums, then cart | pure | radial of every atom
-  const size_t npart = moments_partials(p, p->only_molgrid);
-  const size_t need = (npart + 2 * (size_t)p->natom) * HP_NMOM;
+  const MomShape ms = moment_shape(lmax);
+  const size_t A = (size_t)p->natom;
+  const size_t npart = moments_partials(p, p->only_molgrid) * (size_t)ms.stride;
+  const size_t nlay = A * (size_t)(ms.ncart + ms.npure + ms.nrad);
+  const size_t need = npart + A * (size_t)ms.stride + nlay;
   if (need > (size_t)HP_STATIC_WORKSPACE_DOUBLES)
     return fail(HP_E_ARG, "moments of this molecule need " + std::to_string(need) + " doubles of workspace, beyond the " +
                               std::to_string(HP_STATIC_WORKSPACE_DOUBLES) +
-                              " of the static workspace ((partial rows + 2 natom) x 40 doubles); use hp_part_moments");
+                              " of the static workspace (partial rows and natom rows of " +
+                              std::to_string(ms.stride) + " sums, and the natom x " +
+                              std::to_string(ms.ncart + ms.npure + ms.nrad) + " output); use hp_part_moments");
   HP_HIP(hipSetDevice(p->device));
   double *ws = nullptr;
   if (int rc = static_workspace(p->device, &ws)) return rc;
-  double *part_buf = ws, *out_buf = ws + npart * HP_NMOM, *lay = out_buf + (size_t)p->natom * HP_NMOM;
-  if (int rc = launch_moments(p, part_buf, out_buf, p->only_molgrid)) return rc;
-  hipLaunchKernelGGL(k_moments_layout, dim3((p->natom + 63) / 64), dim3(64), 0, p->stream, p->natom, out_buf,
-                     p->pseudo, lay);
+  double *part_buf = ws, *out_buf = ws + npart, *lay = out_buf + A * (size_t)ms.stride;
+  if (int rc = launch_moments_lmax(p, lmax, ms, part_buf, out_buf, p->only_molgrid)) return rc;
+  hipLaunchKernelGGL(k_moments_layout, dim3((p->natom + 63) / 64), dim3(64), 0, p->stream, p->natom, ms.stride,
+                     ms.src_cart, ms.ncart, ms.src_pure, ms.npure, ms.src_rad, ms.nrad, out_buf, p->pseudo, lay);
   HP_LAUNCHED();
-  const size_t A = (size_t)p->natom;
-  if (cart) HP_HIP(hipMemcpyAsync(cart, lay, sizeof(double) * A * 20, hipMemcpyDeviceToHost, p->stream));
-  if (pure) HP_HIP(hipMemcpyAsync(pure, lay + A * 20, sizeof(double) * A * 16, hipMemcpyDeviceToHost, p->stream));
-  if (radial) HP_HIP(hipMemcpyAsync(radial, lay + A * 36, sizeof(double) * A * 4, hipMemcpyDeviceToHost, p->stream));
+  const double *lay_pure = lay + A * ms.ncart, *lay_rad = lay_pure + A * ms.npure;
+  if (cart) HP_HIP(hipMemcpyAsync(cart, lay, sizeof(double) * A * ms.ncart, hipMemcpyDeviceToHost, p->stream));
+  if (pure) HP_HIP(hipMemcpyAsync(pure, lay_pure, sizeof(double) * A * ms.npure, hipMemcpyDeviceToHost, p->stream));
+  if (radial) HP_HIP(hipMemcpyAsync(radial, lay_rad, sizeof(double) * A * ms.nrad, hipMemcpyDeviceToHost, p->stream));
   HP_HIP(hipStreamSynchronize(p->stream));
   return 0;
 }

@@ -24,6 +24,7 @@
 #define HP_HD __host__ __device__
 #include "hp_exp_tab.h"
 #include "hp_spline.h"
+#include "hp_moments.h"
 
 #define HP_KMAX 16      // functions per atom handled by the per-atom kernels
 
@@ -1930,7 +1931,7 @@ static __global__ __launch_bounds__(256) void k_finalize(int *__restrict__ state
 }
 
 // ---------------------------------------------------------------------------
-// moments (lmax <= 3): one workgroup per radial shell, 40 partial sums
+// moments up to lmax = 3: one workgroup per radial shell, 40 partial sums
 // ---------------------------------------------------------------------------
 #define HP_NMOM 40
 #define HP_MOM_CHUNK 4096  // molecular-grid moments: points per workgroup
@@ -2041,21 +2042,23 @@ static __global__ void k_moments_reduce_mol(int nchunk, const double *__restrict
   out[(int64_t)a * HP_NMOM + i] = v;
 }
 
-// Final layout of the moments from the per-atom sums: cart (natom x 20),
-// pure (natom x 16), radial (natom x 4) one after the other, the electron
-// sign applied and pseudo_numbers added to the monopoles (core/base.py:374-402).
-static __global__ void k_moments_layout(int natom, const double *__restrict__ mom, const double *__restrict__ pseudo,
-                                        double *__restrict__ out) {
+// Final layout of the moments from the per-atom sums (rows of `stride`):
+// cart (natom x ncart), pure (natom x npure), radial (natom x nrad) one after
+// the other, taken from the row offsets src_*; the electron sign applied and
+// pseudo_numbers added to the monopoles (core/base.py:374-402).
+static __global__ void k_moments_layout(int natom, int stride, int src_cart, int ncart, int src_pure, int npure,
+                                        int src_rad, int nrad, const double *__restrict__ mom,
+                                        const double *__restrict__ pseudo, double *__restrict__ out) {
   const int a = blockIdx.x * blockDim.x + threadIdx.x;
   if (a >= natom) return;
-  const double *v = mom + (int64_t)a * HP_NMOM;
-  double *cart = out + (int64_t)a * 20, *pure = out + (int64_t)natom * 20 + (int64_t)a * 16,
-         *rad = out + (int64_t)natom * 36 + (int64_t)a * 4;
-  for (int i = 0; i < 20; ++i) cart[i] = -v[i];
+  const double *v = mom + (int64_t)a * stride;
+  double *cart = out + (int64_t)a * ncart, *pure = out + (int64_t)natom * ncart + (int64_t)a * npure,
+         *rad = out + (int64_t)natom * (ncart + npure) + (int64_t)a * nrad;
+  for (int i = 0; i < ncart; ++i) cart[i] = -v[src_cart + i];
   cart[0] += pseudo[a];
-  for (int i = 0; i < 16; ++i) pure[i] = -v[20 + i];
+  for (int i = 0; i < npure; ++i) pure[i] = -v[src_pure + i];
   pure[0] += pseudo[a];
-  for (int i = 0; i < 4; ++i) rad[i] = v[36 + i];
+  for (int i = 0; i < nrad; ++i) rad[i] = v[src_rad + i];
 }
 
 // Sum of the per-shell partials (one workgroup of 256 threads): the entropy
@@ -2076,6 +2079,100 @@ static __global__ void k_moments_reduce(DevMol m, const double *__restrict__ par
   double v = 0.0;
   for (int s = m.atom_shell_off[a]; s < m.atom_shell_off[a + 1]; ++s) v += part[(int64_t)s * HP_NMOM + i];
   out[(int64_t)a * HP_NMOM + i] = v;
+}
+
+// ---------------------------------------------------------------------------
+// moments of any order: one lane per quantity
+// ---------------------------------------------------------------------------
+// The quantities of order lmax (decode_moment_quantity, moment_quantity): a
+// workgroup covers HP_MOM_GROUP of them (blockIdx.z), two per lane; its
+// points are staged in LDS 256 at a time and each wave takes every fourth
+// staged point.
+#define HP_MOM_GROUP 128
+
+// Moments of order lmax (any): MOL = false, one workgroup per radial shell
+// (the shell atom's AIM density with the atom-grid weights, as k_moments);
+// MOL = true, workgroup (point chunk, atom) with the molecular weights and the
+// stored promolecule (as k_moments_mol).  Row of part: the shell, or
+// atom * nchunk + chunk; nq quantities per row.
+template <int FK, bool MOL>
+__global__ __launch_bounds__(256) void k_moments_any(DevMol m, DevFn fn, const double *__restrict__ promol, int lmax,
+                                                     int nq, double *__restrict__ part) {
+  __shared__ double s_w[256], s_x[256], s_y[256], s_z[256];
+  __shared__ double red[4][HP_MOM_GROUP];
+  int a;
+  int64_t p0, p1, row;
+  if (MOL) {
+    a = blockIdx.y;
+    p0 = (int64_t)blockIdx.x * HP_MOM_CHUNK;
+    p1 = p0 + HP_MOM_CHUNK < m.npoint ? p0 + HP_MOM_CHUNK : m.npoint;
+    row = (int64_t)a * gridDim.x + blockIdx.x;
+  } else {
+    a = m.shell_atom[blockIdx.x];
+    p0 = m.shell_off[blockIdx.x];
+    p1 = m.shell_off[blockIdx.x + 1];
+    row = blockIdx.x;
+  }
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const int qa = (int)blockIdx.z * HP_MOM_GROUP + lane, qb = qa + 64;
+  const MomQuantity da = decode_moment_quantity(qa < nq ? qa : -1, lmax);
+  const MomQuantity db = decode_moment_quantity(qb < nq ? qb : -1, lmax);
+  DevFn f2 = fn;
+  f2.nb_list = nullptr;
+  const bool cull = fn.rc > 0.0;
+  double acc_a = 0.0, acc_b = 0.0;
+  for (int64_t base = p0; base < p1; base += 256) {
+    const int64_t p = base + threadIdx.x;
+    double wf = 0.0, x = 0.0, y = 0.0, z = 0.0;
+    if (p < p1) {
+      x = m.px[p] - m.cx[a];
+      y = m.py[p] - m.cy[a];
+      z = m.pz[p] - m.cz[a];
+      if (MOL) {
+        const double r2 = x * x + y * y + z * z;
+        const double r = sqrt(r2);
+        const double own = (!cull || r <= fn.rc) ? proatom_value<double, FK>(fn, a, r, r2) : 0.0;
+        wf = m.wmol[p] * (m.rho[p] * aim_weight(own, promol[p]));
+      } else {
+        double pm, own;
+        promol_at<double, FK>(m, f2, a, m.natom, nullptr, m.px[p], m.py[p], m.pz[p], pm, own);
+        wf = m.wat[p] * (m.rho[p] * aim_weight(own, pm));
+      }
+    }
+    s_w[threadIdx.x] = wf;
+    s_x[threadIdx.x] = x;
+    s_y[threadIdx.x] = y;
+    s_z[threadIdx.x] = z;
+    __syncthreads();
+    const int cnt = p1 - base < 256 ? (int)(p1 - base) : 256;
+    for (int j = wv; j < cnt; j += 4) {
+      const double w = s_w[j], xj = s_x[j], yj = s_y[j], zj = s_z[j];
+      if (da.kind >= 0) acc_a += w * moment_quantity(da, xj, yj, zj);
+      if (db.kind >= 0) acc_b += w * moment_quantity(db, xj, yj, zj);
+    }
+    __syncthreads();
+  }
+  red[wv][lane] = acc_a;
+  red[wv][lane + 64] = acc_b;
+  __syncthreads();
+  if (threadIdx.x < HP_MOM_GROUP) {
+    const int q = (int)blockIdx.z * HP_MOM_GROUP + threadIdx.x;
+    if (q < nq) part[row * nq + q] = ((red[0][threadIdx.x] + red[1][threadIdx.x]) + red[2][threadIdx.x]) + red[3][threadIdx.x];
+  }
+}
+
+// Per-atom sums of the partial rows (nq per row): over the atom's shells
+// (atom grids) or its nchunk point chunks (molecular grid), in row order.
+static __global__ void k_moments_reduce_any(DevMol m, int molgrid, int nchunk, int nq, const double *__restrict__ part,
+                                            double *__restrict__ out) {
+  const int a = blockIdx.x;
+  const int64_t r0 = molgrid ? (int64_t)a * nchunk : m.atom_shell_off[a];
+  const int64_t r1 = molgrid ? r0 + nchunk : m.atom_shell_off[a + 1];
+  for (int i = threadIdx.x; i < nq; i += blockDim.x) {
+    double v = 0.0;
+    for (int64_t r = r0; r < r1; ++r) v += part[r * nq + i];
+    out[(int64_t)a * nq + i] = v;
+  }
 }
 
 // ---------------------------------------------------------------------------

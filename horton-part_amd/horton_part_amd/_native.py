@@ -424,9 +424,12 @@ class DevicePartition:
         return out
 
     def moments(self, lmax=3):
-        cart = np.empty((self.natom, 20))
-        pure = np.empty((self.natom, 16))
-        rad = np.empty((self.natom, 4))
+        lmax = int(lmax)
+        if lmax < 0:
+            raise ValueError("lmax must be >= 0")
+        cart = np.empty((self.natom, (lmax + 1) * (lmax + 2) * (lmax + 3) // 6))
+        pure = np.empty((self.natom, (lmax + 1) ** 2))
+        rad = np.empty((self.natom, lmax + 1))
         check(self.lib.hp_part_moments(self.handle, int(lmax), ptr(cart), ptr(pure), ptr(rad)))
         return cart, pure, rad
 

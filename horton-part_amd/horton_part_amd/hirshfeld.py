@@ -166,8 +166,6 @@ class HirshfeldWPart(ProatomSplines, WPart):
             populations[:] = pseudo + self.numbers - self.pseudo_numbers
 
     def _device_moments(self):
-        if self.lmax != 3:
-            raise NotImplementedError("device moments are implemented for lmax = 3")
         return self._dev.moments(self.lmax)
 
     @just_once

@@ -651,8 +651,6 @@ class AbstractISAWPart(WPart):
         self.cache.dump("time_update_propars", t_i * 1e-3, tags="o")
 
     def _device_moments(self):
-        if self.lmax != 3:
-            raise NotImplementedError("device moments are implemented for lmax = 3")
         return self._dev.moments(self.lmax)
 
     @property

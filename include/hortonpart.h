@@ -239,9 +239,10 @@ int hp_part_at_weights(hp_part *part, int atom, double *out, int64_t n);
 /* Promolecule density of the last update ("promoldens", stockholder.py:355). */
 int hp_part_promoldens(hp_part *part, double *out, int64_t n);
 
-/* AIM multipole moments (core/base.py:329-402) for lmax <= 3 in HORTON order:
- * cartesian natom*20, pure natom*16, radial natom*4 (pseudo_numbers added to
- * the monopoles, electron sign applied as the reference does).  Grid types
+/* AIM multipole moments (core/base.py:329-402) of order lmax >= 0 in HORTON
+ * order: cartesian natom*(lmax+1)(lmax+2)(lmax+3)/6, pure natom*(lmax+1)^2,
+ * radial natom*(lmax+1) (pseudo_numbers added to the monopoles, electron sign
+ * applied as the reference does).  Any of the three may be NULL.  Grid types
  * 1 and 2 integrate each atom's AIM density on its own atom grid with the
  * atom-grid weights; grid_type 3 on the whole molecular grid with the
  * molecular weights (local = False, core/base.py:455-459). */
@@ -251,9 +252,11 @@ int hp_part_moments(hp_part *part, int lmax, double *cartesian, double *pure, do
  * sums in a static device workspace of 2^20 doubles (8 MiB) that each GPU
  * holds until the process ends.  No counterpart in the reference: it is for
  * callers that cannot allocate device memory (code under stream capture,
- * serving loops).  Molecules whose moments need more than the workspace,
- * (partial rows + 2 natom) x 40 doubles with one partial row per radial
- * shell (grid_type 3: per atom and 4096-point chunk), get HP_E_ARG.  Every
+ * serving loops).  Molecules whose moments need more than the workspace get
+ * HP_E_ARG: (partial rows + natom) x S sums plus the natom x Q outputs, with
+ * one partial row per radial shell (grid_type 3: per atom and 4096-point
+ * chunk), S = 40 for lmax <= 3 and S = Q otherwise, Q the number of
+ * quantities of lmax (cartesian + pure + radial).  Every
  * call on one GPU uses that GPU's workspace: calls that overlap on a GPU
  * (several host threads) race, so serialise them. */
 int hp_part_moments_static(hp_part *part, int lmax, double *cartesian, double *pure, double *radial);

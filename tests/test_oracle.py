@@ -367,6 +367,127 @@ def test_oracle_moments_closed_forms():
     np.testing.assert_allclose(rad0[0], N * np.array([1.0, 3 / a, m2, 60 / a**3]), rtol=1e-7)
 
 
+def _slater_radial_moment(n, a):
+    """<r^n> of the normalised Slater density a^3/(8 pi) e^{-a r}: (n+2)!/(2 a^n)."""
+    from math import factorial
+
+    return factorial(n + 2) / (2 * a**n)
+
+
+def _isotropic_moment(i, j, k, a):
+    """E[u_x^i u_y^j u_z^k] of a spherical density: zero unless all even, else
+    <r^n> (i-1)!!(j-1)!!(k-1)!!/(n+1)!! with n = i + j + k."""
+    if i % 2 or j % 2 or k % 2:
+        return 0.0
+
+    def dfact(m):
+        out = 1
+        while m > 1:
+            out *= m
+            m -= 2
+        return out
+
+    n = i + j + k
+    return _slater_radial_moment(n, a) * dfact(i - 1) * dfact(j - 1) * dfact(k - 1) / dfact(n + 1)
+
+
+def _solid_harmonics_lpmv(lmax, s):
+    """Racah-normalised real solid harmonics at s from scipy's associated
+    Legendre function (lpmv carries the Condon-Shortley phase (-1)^m, removed
+    here): R_lm = sqrt(2 (l-m)!/(l+m)!) r^l P_l^m(cos t) (cos | sin)(m phi)."""
+    from math import factorial
+
+    from scipy.special import lpmv
+
+    r = np.linalg.norm(s)
+    ct, phi = s[2] / r, np.arctan2(s[1], s[0])
+    out = []
+    for l in range(lmax + 1):
+        out.append(r**l * lpmv(0, l, ct))
+        for m in range(1, l + 1):
+            f = np.sqrt(2.0 * factorial(l - m) / factorial(l + m)) * r**l * (-1) ** m * lpmv(m, l, ct)
+            out.extend([f * np.cos(m * phi), f * np.sin(m * phi)])
+    return np.array(out)
+
+
+def test_oracle_moments_closed_forms_any_order():
+    """``OracleWPart.do_moments`` at lmax = 5 on the displaced Slater density
+    of the lmax = 3 test: cartesian moments from the multinomial expansion of
+    (s + u)^p over the isotropic moments of u; pure moments -N R_lm(s) (the
+    mean-value property of harmonic polynomials) with R from lpmv; radial
+    moments N (n+2)!/(2 a^n) about the density's own centre."""
+    from math import comb
+
+    a, N, Z, lmax = 2.0, 1.5, 3, 5
+    s = np.array([0.05, -0.03, 0.04])
+    o = _one_atom_oracle(s, alpha=a, pop=N, z=Z)
+    o.lmax = lmax
+    cart, pure, rad = o.do_moments()
+    powers = [(nx, ny, l - nx - ny) for l in range(lmax + 1) for nx in range(l, -1, -1) for ny in range(l - nx, -1, -1)]
+    assert cart.shape == (1, 56) and pure.shape == (1, 36) and rad.shape == (1, 6)
+    expect = []
+    for px, py, pz in powers:
+        v = 0.0
+        for i in range(px + 1):
+            for j in range(py + 1):
+                for k in range(pz + 1):
+                    v += (comb(px, i) * comb(py, j) * comb(pz, k) * s[0] ** (px - i) * s[1] ** (py - j)
+                          * s[2] ** (pz - k) * _isotropic_moment(i, j, k, a))
+        expect.append(v)
+    expect = -N * np.array(expect)
+    expect[0] += Z
+    np.testing.assert_allclose(cart[0], expect, rtol=1e-7, atol=1e-9)
+    pure_expect = -N * _solid_harmonics_lpmv(lmax, s)
+    pure_expect[0] += Z
+    np.testing.assert_allclose(pure[0], pure_expect, rtol=1e-7, atol=1e-9)
+    o0 = _one_atom_oracle(np.zeros(3), alpha=a, pop=N, z=Z)
+    o0.lmax = lmax
+    _, _, rad0 = o0.do_moments()
+    np.testing.assert_allclose(rad0[0], N * np.array([_slater_radial_moment(n, a) for n in range(lmax + 1)]), rtol=1e-7)
+
+
+def test_moment_quantities_of_the_device(tmp_path):
+    """The device's moment integrands (hp_moments.h, host-compiled with g++):
+    decoding of the row index into cartesian powers (HORTON order), solid
+    harmonics (l, m, cos | sin) and radial powers, and their values, against
+    the textbook forms above at lmax = 0, 3, 6 and 9."""
+    import shutil
+    import subprocess
+
+    gxx = shutil.which("g++")
+    if gxx is None:
+        pytest.skip("g++ not available")
+    src = tmp_path / "q.cpp"
+    src.write_text(
+        '#include <cstdio>\n#include <cstdlib>\n#include "hp_moments.h"\n'
+        "int main(int c, char **v) {\n"
+        "  int L = atoi(v[1]); double x = atof(v[2]), y = atof(v[3]), z = atof(v[4]);\n"
+        "  int nq = (L + 1) * (L + 2) * (L + 3) / 6 + (L + 1) * (L + 1) + L + 1;\n"
+        "  for (int q = 0; q < nq + 2; ++q) { MomQuantity d = decode_moment_quantity(q, L);\n"
+        '    printf("%d %d %d %d %.17g\\n", d.kind, d.i0, d.i1, d.i2, moment_quantity(d, x, y, z)); }\n'
+        "}\n")
+    exe = tmp_path / "q"
+    inc = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "horton-part_amd", "csrc")
+    subprocess.run([gxx, "-O2", "-I", inc, "-o", str(exe), str(src)], check=True)
+    for lmax in (0, 3, 6, 9):
+        for pt in ((0.3, -0.7, 0.45), (1.2, 0.1, -2.0), (-0.4, 0.9, 0.0)):
+            out = subprocess.run([str(exe), str(lmax)] + [repr(v) for v in pt], capture_output=True, text=True,
+                                 check=True).stdout.split("\n")
+            rows = [line.split() for line in out if line]
+            powers = [(nx, ny, l - nx - ny) for l in range(lmax + 1) for nx in range(l, -1, -1)
+                      for ny in range(l - nx, -1, -1)]
+            d = np.array(pt)
+            expect = [d[0] ** p * d[1] ** q * d[2] ** r for p, q, r in powers]
+            expect += list(_solid_harmonics_lpmv(lmax, d))
+            expect += [np.linalg.norm(d) ** n for n in range(lmax + 1)]
+            nq = len(expect)
+            assert len(rows) == nq + 2
+            assert [int(r[0]) for r in rows] == [0] * len(powers) + [1] * (lmax + 1) ** 2 + [2] * (lmax + 1) + [-1, -1]
+            assert [tuple(int(v) for v in r[1:4]) for r in rows[: len(powers)]] == powers
+            got = np.array([float(r[4]) for r in rows[:nq]])
+            np.testing.assert_allclose(got, expect, rtol=1e-12, atol=1e-12)
+
+
 def test_oracle_moments_fail_on_planted_convention_errors():
     """The closed forms above separate the conventions: a Condon-Shortley
     phase (-1)^m or a swapped (cos, sin) order in the pure moments moves them

@@ -2,7 +2,8 @@
 
 The static variant runs the same kernels with their partial sums in a static
 device workspace, so its moments equal the allocating entry point's bit for
-bit.  Two molecules in a row, the second smaller, check that the workspace's
+bit, for lmax 3 (40 sums per row), 1 (a prefix of them) and 5 (one sum per
+quantity).  Two molecules in a row, the second smaller, check that the workspace's
 leftovers from the first call do not leak into the second; a molecule beyond
 the workspace gets a clean HP_E_ARG.
 """
@@ -25,13 +26,13 @@ def _say(msg):
     sys.stdout.flush()
 
 
-def _static_moments(dev, natom):
+def _static_moments(dev, natom, lmax=3):
     from horton_part_amd import _native
 
-    cart = np.full((natom, 20), np.nan)
-    pure = np.full((natom, 16), np.nan)
-    rad = np.full((natom, 4), np.nan)
-    rc = dev.lib.hp_part_moments_static(dev.handle, 3, _native.ptr(cart), _native.ptr(pure), _native.ptr(rad))
+    cart = np.full((natom, (lmax + 1) * (lmax + 2) * (lmax + 3) // 6), np.nan)
+    pure = np.full((natom, (lmax + 1) ** 2), np.nan)
+    rad = np.full((natom, lmax + 1), np.nan)
+    rc = dev.lib.hp_part_moments_static(dev.handle, lmax, _native.ptr(cart), _native.ptr(pure), _native.ptr(rad))
     return rc, (cart, pure, rad)
 
 
@@ -46,11 +47,12 @@ def test_moments_static_equals_allocating(grid_type):
         part = MBISWPart(w.coordinates, w.numbers, w.pseudo_numbers, w.grid, w.density, maxiter=8, grid_type=grid_type)
         part.do_partitioning()
         dev = part._dev
-        ref = dev.moments(3)
-        rc, got = _static_moments(dev, natom)
-        assert rc == 0
-        for r, g in zip(ref, got):
-            np.testing.assert_array_equal(g, r)
+        for lmax in (3, 1, 5):  # the 40-sum kernel, a prefix of it, the any-order kernel
+            ref = dev.moments(lmax)
+            rc, got = _static_moments(dev, natom, lmax)
+            assert rc == 0
+            for r, g in zip(ref, got):
+                np.testing.assert_array_equal(g, r)
         part.close()
 
 
