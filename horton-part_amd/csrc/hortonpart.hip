@@ -1691,21 +1691,39 @@ int hp_becke_weights(int device, int natom, const double *coordinates, const dou
       if (d == 0.0) return fail(HP_E_ARG, "two atoms at the same position");
       invr[(size_t)i * natom + j] = 1.0 / d;
     }
+  // per atom o: every atom by distance from o (o first), ties by index
+  std::vector<int> ord((size_t)natom * natom);
+  {
+    std::vector<double> d2(natom);
+    for (int o = 0; o < natom; ++o) {
+      for (int j = 0; j < natom; ++j) {
+        const double dx = coordinates[3 * o] - coordinates[3 * j], dy = coordinates[3 * o + 1] - coordinates[3 * j + 1],
+                     dz = coordinates[3 * o + 2] - coordinates[3 * j + 2];
+        d2[j] = dx * dx + dy * dy + dz * dz;
+      }
+      int *row = ord.data() + (size_t)o * natom;
+      for (int j = 0; j < natom; ++j) row[j] = j;
+      std::sort(row, row + natom, [&](int u, int v) { return d2[u] < d2[v] || (d2[u] == d2[v] && u < v); });
+      if (row[0] != o) return fail(HP_E_ARG, "two atoms at the same position");
+    }
+  }
   GridScratch *g = nullptr;
   double *c = nullptr, *p = nullptr;
-  void *dtab = nullptr, *doff = nullptr, *dout = nullptr;
+  void *dtab = nullptr, *doff = nullptr, *dout = nullptr, *dord = nullptr;
   int rc = grid_scratch(device, &g);
   if (!rc) rc = upload_points(g, natom, coordinates, npoint, points, &c, &p);
   if (!rc) rc = grid_buf(g, 2, tab.size() * sizeof(double), &dtab);
   if (!rc) rc = grid_buf(g, 3, (natom + 1) * sizeof(int64_t), &doff);
   if (!rc) rc = grid_buf(g, 4, npoint * sizeof(double), &dout);
+  if (!rc) rc = grid_buf(g, 5, ord.size() * sizeof(int), &dord);
   if (rc) return rc;
   HP_HIP(hipMemcpyAsync(dtab, tab.data(), tab.size() * sizeof(double), hipMemcpyHostToDevice, g->stream));
   HP_HIP(hipMemcpyAsync(doff, atom_offsets, (natom + 1) * sizeof(int64_t), hipMemcpyHostToDevice, g->stream));
+  HP_HIP(hipMemcpyAsync(dord, ord.data(), ord.size() * sizeof(int), hipMemcpyHostToDevice, g->stream));
   const double *dadj = (const double *)dtab, *dinvr = (const double *)dtab + (size_t)natom * natom;
   hipLaunchKernelGGL(k_becke, dim3((unsigned)((npoint + 255) / 256)), dim3(256), 0, g->stream, natom, c, c + natom,
-                     c + 2 * natom, dadj, dinvr, npoint, p, p + npoint, p + 2 * npoint, (const int64_t *)doff, 3,
-                     (double *)dout);
+                     c + 2 * natom, dadj, dinvr, (const int *)dord, npoint, p, p + npoint, p + 2 * npoint,
+                     (const int64_t *)doff, 3, (double *)dout);
   HP_LAUNCHED();
   HP_HIP(hipMemcpyAsync(out, dout, sizeof(double) * npoint, hipMemcpyDeviceToHost, g->stream));
   HP_HIP(hipStreamSynchronize(g->stream));

@@ -2181,14 +2181,106 @@ static __global__ void k_moments_reduce_any(DevMol m, int molgrid, int nchunk, i
 
 // Becke fuzzy-cell weight of each point for the atom whose grid holds it
 // (A. D. Becke, J. Chem. Phys. 88 (1988) 2547; size-adjusted, order-3 step).
-// adj[i*A+j] = a_ij, invr[i*A+j] = 1/|R_i - R_j|.
+// adj[i*A+j] = a_ij, invr[i*A+j] = 1/|R_i - R_j|, ord[o*A + k] = the atoms
+// by distance from atom o (o itself first).
+//
+// Per point the cell products P_i = prod_{j != i} s(nu_ij) run over the atoms
+// in the order of the point's owner: the atoms nearest the point come first,
+// so the product of an atom far from the point falls fast.  Every factor is
+// at most 1, so once P_i is at most 2^-64 of the sum of the products before
+// it, P_i stays there: the product stops and the atom adds nothing.  Each
+// omitted term is below 2^-64 of the final sum (natom of them move the
+// weight by less than natom x 2^-64 relative).  The owner comes first, with
+// an empty sum, and its product stops only at 0, as every product did before.
+// The distances of the first HP_BECKE_NC atoms of the order are formed once
+// per point.
+#define HP_BECKE_NC 8
+
+__device__ __forceinline__ double becke_factor(double di, double dj, double invr_ij, double adj_ij, int order) {
+  const double mu = (di - dj) * invr_ij;
+  double f = mu + adj_ij * (1.0 - mu * mu);
+  for (int k = 0; k < order; ++k) f = 1.5 * f - 0.5 * (f * f * f);
+  return 0.5 * (1.0 - f);
+}
+
+// One point's weight.  UNI: the owner is the same on every lane of the
+// wavefront (passed as a wave-uniform value), so the order, the coordinates
+// and the table rows are read with scalar loads.
+template <bool UNI>
+__device__ __forceinline__ double becke_point(int natom, const double *__restrict__ cx, const double *__restrict__ cy,
+                                              const double *__restrict__ cz, const double *__restrict__ adj,
+                                              const double *__restrict__ invr, const int *__restrict__ ord, int lo,
+                                              double x, double y, double z, int order) {
+  constexpr int NC = HP_BECKE_NC;
+  const int *o = ord + (int64_t)lo * natom;
+  double dc[NC];
+#pragma unroll
+  for (int k = 0; k < NC; ++k) {
+    dc[k] = 0.0;
+    if (k < natom) {
+      const int j = UNI ? __builtin_amdgcn_readfirstlane(o[k]) : o[k];
+      const double dx = x - cx[j], dy = y - cy[j], dz = z - cz[j];
+      dc[k] = sqrt(dx * dx + dy * dy + dz * dz);
+    }
+  }
+  double sum = 0.0, own = 0.0;
+  for (int ki = 0; ki < natom; ++ki) {
+    const int i = UNI ? __builtin_amdgcn_readfirstlane(o[ki]) : o[ki];
+    double di;
+    if (ki < NC) {
+      di = dc[0];
+#pragma unroll
+      for (int k = 1; k < NC; ++k) di = ki == k ? dc[k] : di;
+    } else {
+      const double dx = x - cx[i], dy = y - cy[i], dz = z - cz[i];
+      di = sqrt(dx * dx + dy * dy + dz * dz);
+    }
+    const double thr = 0x1p-64 * sum;
+    const double *ai = adj + (int64_t)i * natom, *ri = invr + (int64_t)i * natom;
+    double P = 1.0;
+    bool stop = false;
+#pragma unroll
+    for (int kj = 0; kj < NC; ++kj) {
+      if (kj < natom && !stop) {
+        const int j = UNI ? __builtin_amdgcn_readfirstlane(o[kj]) : o[kj];
+        if (j != i) {
+          P *= becke_factor(di, dc[kj], ri[j], ai[j], order);
+          stop = P <= thr;
+        }
+      }
+    }
+    if (!stop) {
+      for (int kj = NC; kj < natom; ++kj) {
+        const int j = UNI ? __builtin_amdgcn_readfirstlane(o[kj]) : o[kj];
+        if (j == i) continue;
+        const double dx = x - cx[j], dy = y - cy[j], dz = z - cz[j];
+        const double dj = sqrt(dx * dx + dy * dy + dz * dz);
+        P *= becke_factor(di, dj, ri[j], ai[j], order);
+        if (P <= thr) {
+          stop = true;
+          break;
+        }
+      }
+    }
+    if (ki == 0) {
+      own = P;
+      sum = P;
+    } else if (!stop) {
+      sum += P;
+    }
+  }
+  return own / sum;
+}
+
 static __global__ void k_becke(int natom, const double *__restrict__ cx, const double *__restrict__ cy,
-                        const double *__restrict__ cz, const double *__restrict__ adj, const double *__restrict__ invr,
-                        int64_t npoint, const double *__restrict__ px, const double *__restrict__ py,
-                        const double *__restrict__ pz, const int64_t *__restrict__ atom_off, int order,
-                        double *__restrict__ out) {
-  const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (p >= npoint) return;
+                               const double *__restrict__ cz, const double *__restrict__ adj,
+                               const double *__restrict__ invr, const int *__restrict__ ord, int64_t npoint,
+                               const double *__restrict__ px, const double *__restrict__ py,
+                               const double *__restrict__ pz, const int64_t *__restrict__ atom_off, int order,
+                               double *__restrict__ out) {
+  const int64_t p0 = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const bool ok = p0 < npoint;
+  const int64_t p = ok ? p0 : npoint - 1;  // idle lanes follow the last point, so they stay on its owner
   int lo = 0, hi = natom;  // owner: atom_off[lo] <= p < atom_off[lo+1]
   while (hi - lo > 1) {
     const int mid = (lo + hi) >> 1;
@@ -2198,24 +2290,15 @@ static __global__ void k_becke(int natom, const double *__restrict__ cx, const d
       hi = mid;
   }
   const double x = px[p], y = py[p], z = pz[p];
-  double sum = 0.0, own = 0.0;
-  for (int i = 0; i < natom; ++i) {
-    const double dxi = x - cx[i], dyi = y - cy[i], dzi = z - cz[i];
-    const double di = sqrt(dxi * dxi + dyi * dyi + dzi * dzi);
-    double P = 1.0;
-    for (int j = 0; j < natom && P != 0.0; ++j) {
-      if (j == i) continue;
-      const double dxj = x - cx[j], dyj = y - cy[j], dzj = z - cz[j];
-      const double dj = sqrt(dxj * dxj + dyj * dyj + dzj * dzj);
-      const double mu = (di - dj) * invr[(int64_t)i * natom + j];
-      double f = mu + adj[(int64_t)i * natom + j] * (1.0 - mu * mu);
-      for (int k = 0; k < order; ++k) f = 1.5 * f - 0.5 * (f * f * f);
-      P *= 0.5 * (1.0 - f);
-    }
-    sum += P;
-    if (i == lo) own = P;
+  double w = 1.0;
+  if (natom > 1) {
+    const int lo_u = __builtin_amdgcn_readfirstlane(lo);
+    if (__all(lo == lo_u))
+      w = becke_point<true>(natom, cx, cy, cz, adj, invr, ord, lo_u, x, y, z, order);
+    else
+      w = becke_point<false>(natom, cx, cy, cz, adj, invr, ord, lo, x, y, z, order);
   }
-  out[p] = natom == 1 ? 1.0 : own / sum;
+  if (ok) out[p] = w;
 }
 
 // sum_a sum_k coef_ak exp(-alpha_ak |p - R_a|) at every point.

@@ -499,14 +499,21 @@ def test_glisa_newton_failures_like_reference():
     assert len(part.history_changes) == len(oracle.history_changes)
 
 
-def test_becke_weights_device_matches_host():
-    """The device Becke weights against the numpy restatement on a 37-atom
-    cluster."""
+@pytest.mark.parametrize("shape", ["cluster37", "chain60"])
+def test_becke_weights_device_matches_host(shape):
+    """The device Becke weights against the numpy restatement (every cell
+    product in full).  The device visits the atoms nearest-first from each
+    point's owner and stops a product below 2^-64 of the running sum: on the
+    60-atom chain most products stop after a few factors."""
     require_gpu()
-    _say("case: Becke weights, 37-atom cluster, 20x26 grids")
     from horton_part_amd.grid import BeckeWeights
 
-    w = workloads.cluster(natom=37, seed=5, nrad=20, nang=26, becke=False)
+    if shape == "cluster37":
+        _say("case: Becke weights, 37-atom cluster seed 5, 20x26 grids")
+        w = workloads.cluster(natom=37, seed=5, nrad=20, nang=26, becke=False)
+    else:
+        _say("case: Becke weights, 60-atom chain seed 9, 20x26 grids")
+        w = workloads.chain(natom=60, seed=9, nrad=20, nang=26, becke=False)
     g = w.grid
     host = BeckeWeights().generate_weights(g.points, w.coordinates, w.numbers, indices=g.indices)
     dev = BeckeWeights(device=0).generate_weights(g.points, w.coordinates, w.numbers, indices=g.indices)
