@@ -236,6 +236,17 @@ __global__ __launch_bounds__(256) void k_gt2_pass(const int *__restrict__ done, 
 // negative cutoff (alisa.py:836-850).  Each lane keeps a 32-bit mask (bit t
 // set = candidate t fails somewhere); masks are OR-ed over lanes, waves and
 // chunks, and the update takes the first candidate whose bit is clear.
+//
+// The candidate's value is linear in x: with P0 = sum_k B_k c_k and
+// P1 = sum_k B_k delta_k, v_t = P0 + x_t P1 up to rounding.  The value the
+// test is defined on is the FMA chain sum_k B_k cand_tk over the rounded
+// candidates cand_tk = fl(c_k + fl(x_t delta_k)); it differs from
+// fl(P0 + x_t P1) by less than (2 + 2 K + 1) u A (u = 2^-53, K <= 8 terms,
+// A = sum_k |B_k| (|c_k| + xm |delta_k|), xm >= |x_t|), plus at most a few
+// units of 2^-1074 where products fall below the normal range.  A candidate
+// whose linear value is farther than 64 u A + 1e-300 from the cutoff is
+// decided by it; the others run the chain itself, so every decision is the
+// chain's.
 __device__ __forceinline__ int wave_or(int v) {
 #pragma unroll
   for (int o = 32; o > 0; o >>= 1) v |= __shfl_xor(v, o, 64);
@@ -253,6 +264,7 @@ __global__ __launch_bounds__(256) void k_gt2_linesearch(const int *__restrict__ 
   constexpr int KM = HP_GT2_KM, NV = HP_GT2_NV, NL = HP_GT2_NLIN;
   __shared__ int s_mask[4];
   __shared__ double cand[NL][KM];  // c + x_t delta (alisa.py:838-840)
+  __shared__ double s_x[NL];       // x_t
   if (done && *done) return;
   const int a = blockIdx.y;
   const Gt2Atom &S = st[a];
@@ -266,6 +278,16 @@ __global__ __launch_bounds__(256) void k_gt2_linesearch(const int *__restrict__ 
     const double step = (0.0 - tau) / (double)(nlin - 1);
     const double x = (t == nlin - 1) ? 0.0 : (double)t * step + tau;
     cand[t][k] = gt2_step(S.c[k], x, S.delta[k]);
+    if (k == 0) s_x[t] = x;
+  }
+  // the linear form's coefficients and its error scale
+  const double xm = fabs(tau) > 1.0 ? fabs(tau) : 1.0;
+  double c[KM], d[KM], wgt[KM];
+#pragma unroll
+  for (int k = 0; k < KM; ++k) {
+    c[k] = k < K ? S.c[k] : 0.0;
+    d[k] = k < K ? S.delta[k] : 0.0;
+    wgt[k] = fabs(c[k]) + xm * fabs(d[k]);
   }
   __syncthreads();
   int bad = 0;
@@ -276,12 +298,30 @@ __global__ __launch_bounds__(256) void k_gt2_linesearch(const int *__restrict__ 
     const R r = (R)sqrt(dx * dx + dy * dy + dz * dz);
     R T[KM];
     gt2_terms<R>(false, K, nullptr, norm, alpha, order, r, T);
-    for (int t = 0; t < nlin; ++t) {
-      double v = 0.0;
+    double P0 = 0.0, P1 = 0.0, A = 0.0;
 #pragma unroll
-      for (int k = 0; k < KM; ++k)
-        if (k < K) v += (double)T[k] * cand[t][k];
-      if (!(v > neg_cutoff)) bad |= 1 << t;
+    for (int k = 0; k < KM; ++k) {
+      if (k < K) {
+        const double tk = (double)T[k];
+        P0 += tk * c[k];
+        P1 += tk * d[k];
+        A += fabs(tk) * wgt[k];
+      }
+    }
+    const double e = 64.0 * 0x1p-53 * A + 1e-300;  // + the rounding of products in the subnormal range
+    for (int t = 0; t < nlin; ++t) {
+      if (bad & (1 << t)) continue;
+      const double va = P0 + s_x[t] * P1;
+      if (va - e > neg_cutoff) continue;  // above the cutoff in the chain too
+      bool fails = true;                  // at or below it in the chain too
+      if (!(va + e <= neg_cutoff)) {
+        double v = 0.0;
+#pragma unroll
+        for (int k = 0; k < KM; ++k)
+          if (k < K) v += (double)T[k] * cand[t][k];
+        fails = !(v > neg_cutoff);
+      }
+      if (fails) bad |= 1 << t;
     }
   }
   bad = wave_or(bad);
