@@ -272,8 +272,8 @@ template <int PPL>
 int launch_inner_ppl(hp_part *p) {
   const dim3 grid((p->natom + 3) / 4), block(256);
   if (p->scheme == HP_SCHEME_MBIS) {
-    // one wavefront per atom with two radial points per lane (up to 128
-    // points), else one workgroup per atom with one point per lane
+    // one wavefront per atom with two to four radial points per lane (up to
+    // 256 points), else one workgroup per atom with one point per lane
     const int ppl = hp::mbis_ppl(p->maxR);
     const dim3 g1(p->natom), b1(ppl > 1 ? 64 : ((p->maxR + 63) / 64) * 64);
 #define HP_MBIS(KMV, PPLV)                                                                                            \
@@ -284,6 +284,10 @@ int launch_inner_ppl(hp_part *p) {
   do {                    \
     if (ppl == 2)         \
       HP_MBIS(KMV, 2);    \
+    else if (ppl == 3)    \
+      HP_MBIS(KMV, 3);    \
+    else if (ppl == 4)    \
+      HP_MBIS(KMV, 4);    \
     else                  \
       HP_MBIS(KMV, 1);    \
   } while (0)

@@ -542,7 +542,7 @@ bool plan_group(hp_batch *b, const std::vector<int> &active, std::vector<Launch>
   }
   // MBIS inner solver: one launch per layout (a molecule's own, as alone);
   // GISA QP: one launch per points-per-lane
-  for (int ppl : {2, 1}) {
+  for (int ppl : {2, 3, 4, 1}) {
     Launch L{};
     L.kind = 1;
     L.off = w;
@@ -696,6 +696,10 @@ int enqueue_tick(hp_batch *b, int t) {
   do {                     \
     if (L.ppl == 2)        \
       HP_BMBIS(KMV, 2);    \
+    else if (L.ppl == 3)   \
+      HP_BMBIS(KMV, 3);    \
+    else if (L.ppl == 4)   \
+      HP_BMBIS(KMV, 4);    \
     else                   \
       HP_BMBIS(KMV, 1);    \
   } while (0)

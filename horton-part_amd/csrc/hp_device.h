@@ -1010,6 +1010,8 @@ __device__ __forceinline__ void block_allsum(double *v, double *lds, int parity)
 #define HP_INV_8PI 0.039788735772973836  // 1 / (8 pi), rounded to the nearest double
 
 // Per-block context of one atom's MBIS solve.
+#define HP_MBIS_PPL_MAX 4  // one wavefront per atom up to 256 radial points
+
 struct MbisCtx {
   const DevMol *m;
   double *par;
@@ -1020,8 +1022,8 @@ struct MbisCtx {
   double *charges, *change_part, *ist_oldpro;
   double thr, pop;
   // this lane's radial points i + 64 j, j < PPL (the solver's template)
-  double r[2], wr[2], w[2], rho[2];
-  bool valid[2];
+  double r[HP_MBIS_PPL_MAX], wr[HP_MBIS_PPL_MAX], w[HP_MBIS_PPL_MAX], rho[HP_MBIS_PPL_MAX];
+  bool valid[HP_MBIS_PPL_MAX];
   int a, f0, s0, i, maxit, budget;
   bool resume;
 };
@@ -1181,8 +1183,9 @@ __device__ __forceinline__ void mbis_dispatch(int K, const MbisCtx &c, int slot)
 }
 
 // PPL = 1: one workgroup per atom, one radial point per lane (ceil(R/64)
-// wavefronts).  PPL = 2: one wavefront per atom, radial points i and i + 64
-// on lane i (R <= 128): its sums need no LDS exchange and no barrier.
+// wavefronts).  PPL = 2, 3, 4: one wavefront per atom, radial points i,
+// i + 64, ... on lane i (R <= 64 PPL): its sums need no LDS exchange and no
+// barrier.
 template <int KM, int PPL>
 __global__ __launch_bounds__(512) void k_mbis_inner(const int *__restrict__ done, DevMol m,
                                                     const double *__restrict__ rhobar, double *__restrict__ par,
@@ -1230,7 +1233,7 @@ __global__ __launch_bounds__(512) void k_mbis_inner(const int *__restrict__ done
   c.thr = thr;
   double pv = 0.0;
 #pragma unroll
-  for (int j = 0; j < 2; ++j) {
+  for (int j = 0; j < HP_MBIS_PPL_MAX; ++j) {
     const int q = i + 64 * j;
     const bool valid = j < PPL && q < R;
     c.valid[j] = valid;

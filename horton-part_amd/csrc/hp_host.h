@@ -91,7 +91,7 @@ inline size_t weights_lds(size_t records, const WeightsShape &w) {
 // else 1 for one workgroup per atom with one point per lane (three points per
 // lane measured no faster at 150).  It depends on the molecule alone, so that
 // its sums are the same alone and in any batch.
-inline int mbis_ppl(int maxR) { return maxR <= 128 ? 2 : 1; }
+inline int mbis_ppl(int maxR) { return maxR <= 128 ? 2 : (maxR <= 192 ? 3 : (maxR <= 256 ? 4 : 1)); }
 
 // Radial points per lane of the one-wave-per-atom solvers.
 inline int pick_ppl(int R) {

@@ -93,13 +93,14 @@ def test_mbis_synthetic_water_degenerate_shells():
     _compare(part, cache, oracle, qtol=1e-10, ptol=1e-8)
 
 
-def test_mbis_workgroup_layout():
-    """Atoms of more than 128 radial points: the MBIS solver runs one
-    workgroup per atom (one point per lane, several wavefronts) instead of
-    one wavefront per atom."""
+@pytest.mark.parametrize("nrad", [150, 240, 300])
+def test_mbis_solver_layouts(nrad):
+    """The MBIS solver's layouts beyond two radial points per lane: one
+    wavefront per atom with three (150 points) or four (240) points per lane,
+    and one workgroup per atom, one point per lane, beyond 256 (300)."""
     require_gpu()
-    _say("case: MBIS chain 4 atoms seed 5, 150x26 (solver: one workgroup per atom)")
-    w = workloads.chain(natom=4, seed=5, nrad=150, nang=26)
+    _say(f"case: MBIS chain 4 atoms seed 5, {nrad}x26 (solver layout of {nrad} radial points)")
+    w = workloads.chain(natom=4, seed=5, nrad=nrad, nang=26)
     kw = dict(maxiter=60)
     part = MBISWPart(w.coordinates, w.numbers, w.pseudo_numbers, w.grid, w.density, **kw)
     part.do_partitioning()
