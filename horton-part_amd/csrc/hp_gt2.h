@@ -343,35 +343,52 @@ __global__ __launch_bounds__(128) void k_gt2_update(const int *__restrict__ done
                                                     double tau, int nlin, int *__restrict__ active,
                                                     int *__restrict__ nhess) {
   constexpr int KM = HP_GT2_KM, NS = HP_GT2_NS, NV = HP_GT2_NV;
+  constexpr int UB = 8;  // chunk values loaded together before they are added in order
   __shared__ double v[NV];
   __shared__ double M[(HP_KMAX + 1) * (HP_KMAX + 2)];
+  __shared__ int s_or[4];
   if (done && *done) return;
   const int a = blockIdx.x;
   Gt2Atom &S = st[a];
   const int phase = S.phase;
   if (phase == 3) return;
   const bool newton = scheme != 0 && (solver == 3 || solver == 4);
-  if (threadIdx.x < NV) {
+  if (phase == 1) {
+    // the candidates' failure masks: an OR, in any order, over every thread
+    unsigned mask = 0u;
+    for (int c = threadIdx.x; c < nchunk; c += blockDim.x)
+      mask |= (unsigned)partial[((int64_t)a * nchunk + c) * NV + 4];
+    const int wm = wave_or((int)mask);
+    if ((threadIdx.x & 63) == 0) s_or[threadIdx.x >> 6] = wm;
+  } else if (threadIdx.x < NV) {
+    // sums (and the minimum in slot 2) over the chunks in chunk order
     const int q = threadIdx.x;
-    if (phase == 1) {
-      unsigned mask = 0u;
-      if (q == 4)
-        for (int c = 0; c < nchunk; ++c) mask |= (unsigned)partial[((int64_t)a * nchunk + c) * NV + 4];
-      v[q] = (double)mask;
-    } else {
-      const bool is_min = q == 2;
-      double acc = is_min ? 1e300 : 0.0;
-      if (q < NS || (newton && phase == 0)) {
-        for (int c = 0; c < nchunk; ++c) {
-          const double pv = partial[((int64_t)a * nchunk + c) * NV + q];
-          acc = is_min ? fmin(acc, pv) : acc + pv;
-        }
+    const bool is_min = q == 2;
+    double acc = is_min ? 1e300 : 0.0;
+    if (q < NS || (newton && phase == 0)) {
+      const double *col = partial + (int64_t)a * nchunk * NV + q;
+      int c = 0;
+      for (; c + UB <= nchunk; c += UB) {
+        double pv[UB];
+#pragma unroll
+        for (int u = 0; u < UB; ++u) pv[u] = col[(int64_t)(c + u) * NV];
+#pragma unroll
+        for (int u = 0; u < UB; ++u) acc = is_min ? fmin(acc, pv[u]) : acc + pv[u];
       }
-      v[q] = acc;
+      for (; c < nchunk; ++c) {
+        const double pv = col[(int64_t)c * NV];
+        acc = is_min ? fmin(acc, pv) : acc + pv;
+      }
     }
+    v[q] = acc;
   }
   __syncthreads();
   if (threadIdx.x != 0) return;
+  if (phase == 1) {
+    unsigned mask = 0u;
+    for (int w = 0; w < (int)(blockDim.x >> 6); ++w) mask |= (unsigned)s_or[w];
+    v[4] = (double)mask;
+  }
   const int K = fn_off[a + 1] - fn_off[a];
   if (newton && phase == 0) nhess[a] += 1;  // this round's pass formed the Hessian
   bool keep = true;
