@@ -129,10 +129,24 @@ __global__ __launch_bounds__(256) void k_glisa_reduce(int nblock_obj, const doub
                                                       const double *__restrict__ hess_part, double *__restrict__ out) {
   // out: [0] objective, [1 .. 1+M) gradient, [1+M .. 1+M+M^2) Hessian, then
   // the smallest promolecule value
+  // every sum runs in block (or chunk) order; UB values are loaded together
+  // before they are added, so the loads' latency is paid once per UB
+  constexpr int UB = 8;
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
   if (t == 0) {
     double s = 0.0, mn = 1.0e308;
-    for (int b = 0; b < nblock_obj; ++b) {
+    int b = 0;
+    for (; b + UB <= nblock_obj; b += UB) {
+      double v[2 * UB];
+#pragma unroll
+      for (int u = 0; u < 2 * UB; ++u) v[u] = obj_part[2 * b + u];
+#pragma unroll
+      for (int u = 0; u < UB; ++u) {
+        s += v[2 * u];
+        mn = fmin(mn, v[2 * u + 1]);
+      }
+    }
+    for (; b < nblock_obj; ++b) {
       s += obj_part[2 * b];
       mn = fmin(mn, obj_part[2 * b + 1]);
     }
@@ -141,7 +155,15 @@ __global__ __launch_bounds__(256) void k_glisa_reduce(int nblock_obj, const doub
   }
   if (do_grad && t < M) {
     double s = 0.0;
-    for (int c = 0; c < nchunk; ++c) s += grad_part[(int64_t)c * M + t];
+    int c = 0;
+    for (; c + UB <= nchunk; c += UB) {
+      double v[UB];
+#pragma unroll
+      for (int u = 0; u < UB; ++u) v[u] = grad_part[(int64_t)(c + u) * M + t];
+#pragma unroll
+      for (int u = 0; u < UB; ++u) s += v[u];
+    }
+    for (; c < nchunk; ++c) s += grad_part[(int64_t)c * M + t];
     out[1 + t] = -s;
   }
   if (hess_part && t < npair * 256) {
@@ -149,8 +171,17 @@ __global__ __launch_bounds__(256) void k_glisa_reduce(int nblock_obj, const doub
     const int2 pr = pairs[pi];
     const int ii = 16 * pr.x + e / 16, jj = 16 * pr.y + e % 16;
     if (ii < M && jj < M && ii <= jj) {
+      const double *col = hess_part + (int64_t)pi * nchunk * 256 + e;
       double s = 0.0;
-      for (int c = 0; c < nchunk; ++c) s += hess_part[((int64_t)pi * nchunk + c) * 256 + e];
+      int c = 0;
+      for (; c + UB <= nchunk; c += UB) {
+        double v[UB];
+#pragma unroll
+        for (int u = 0; u < UB; ++u) v[u] = col[(int64_t)(c + u) * 256];
+#pragma unroll
+        for (int u = 0; u < UB; ++u) s += v[u];
+      }
+      for (; c < nchunk; ++c) s += col[(int64_t)c * 256];
       out[1 + M + (int64_t)ii * M + jj] = s;
       out[1 + M + (int64_t)jj * M + ii] = s;
     }
@@ -201,10 +232,23 @@ __global__ void k_glisa_proatoms_reduce(int natom, int nchunk, const double *__r
                                         double *__restrict__ out) {
   const int a = blockIdx.x * blockDim.x + threadIdx.x;
   if (a >= natom) return;
+  constexpr int UB = 8;  // loaded together, added in chunk order
+  const double *row = part + (int64_t)a * nchunk * 2;
   double mn = 1.0e308, msd = 0.0;
-  for (int c = 0; c < nchunk; ++c) {
-    mn = fmin(mn, part[((int64_t)a * nchunk + c) * 2]);
-    msd += part[((int64_t)a * nchunk + c) * 2 + 1];
+  int c = 0;
+  for (; c + UB <= nchunk; c += UB) {
+    double v[2 * UB];
+#pragma unroll
+    for (int u = 0; u < 2 * UB; ++u) v[u] = row[2 * c + u];
+#pragma unroll
+    for (int u = 0; u < UB; ++u) {
+      mn = fmin(mn, v[2 * u]);
+      msd += v[2 * u + 1];
+    }
+  }
+  for (; c < nchunk; ++c) {
+    mn = fmin(mn, row[2 * c]);
+    msd += row[2 * c + 1];
   }
   out[2 * a] = mn;
   out[2 * a + 1] = msd;
