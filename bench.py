@@ -677,7 +677,9 @@ def run_glisa(args):
     one step = one Newton step's objective, gradient and M x M Hessian
     G^T diag(w rho / rho0^2) G over the molecular grid at the initial
     coefficients (``hp_part_global_terms``); roofline of the FP64 MFMA
-    Hessian pass against 2 M^2 N algorithmic flops."""
+    Hessian pass against M (M + 1) N algorithmic flops: the symmetric
+    product's M (M + 1) / 2 distinct entries, a multiply and an add per entry
+    and point (the kernel forms the upper 16 x 16 tiles only)."""
     import warnings
 
     from horton_part_amd import GlobalLinearISAWPart, _native, workloads
@@ -698,7 +700,10 @@ def run_glisa(args):
     lat = np.array(lat)
     ms, n = dev.kernel_time(_native.KERNEL_GLISA_HESS)
     t_h = ms / max(n, 1) * 1e-3
-    flops = 2.0 * M * M * N
+    flops = float(M) * (M + 1) * N
+    Mp = (M + 15) // 16 * 16
+    T = Mp // 16
+    executed = 2.0 * 256 * (T * (T + 1) // 2) * N  # the MFMA work of the upper tiles, padding included
     tf = flops / t_h / 1e12
     out = {
         "metric": "gLISA Newton steps/s (objective, gradient, Hessian over the molecular grid, FP64)",
@@ -716,7 +721,9 @@ def run_glisa(args):
         "config": {"workload": f"glisa-chain20: gLISA global terms, gauss.json basis (M = {M} functions), "
                                f"{w.natom} atoms, {N} grid points"},
         "roofline": {
-            "kernel": "k_glisa_hess (16x16 tiles of G^T D G on mfma_f64_16x16x4f64)",
+            "kernel": ("k_glisa_hess_lds (every upper 16x16 tile of G^T D G per point chunk on mfma_f64_16x16x4f64, "
+                       "rows of G staged in LDS)" if Mp <= 112 else
+                       "k_glisa_hess (one wavefront per upper 16x16 tile pair and point chunk, mfma_f64_16x16x4f64)"),
             "bound": "mfma",
             "achieved": tf,
             "peak": 78.6,
@@ -724,6 +731,9 @@ def run_glisa(args):
             "frac": tf / 78.6,
             "traffic": None,
             "flops_per_launch": flops,
+            "flops_note": "M (M + 1) N: the distinct entries of the symmetric M x M product over N points",
+            "mfma_flops_per_launch": executed,
+            "mfma_frac": executed / t_h / 1e12 / 78.6,
             "launch_us": t_h * 1e6,
         },
     }

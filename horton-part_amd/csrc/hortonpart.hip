@@ -94,7 +94,7 @@ struct hp_part {
   int params_nonneg = 1;      // the parameters last set are all >= 0 (tail skipping needs it)
   unsigned long long *evals_done = nullptr;  // [FP32, FP64] evaluations of the last run
   // gLISA global terms (allocated at the first call)
-  int gl_M = 0, gl_Mp = 0, gl_npair = 0, gl_nchunk = 0, gl_nblock = 0;
+  int gl_M = 0, gl_Mp = 0, gl_npair = 0, gl_nchunk = 0, gl_hnchunk = 0, gl_nblock = 0;
   double *gl_G = nullptr, *gl_D = nullptr, *gl_E = nullptr, *gl_obj = nullptr, *gl_grad = nullptr,
          *gl_hess = nullptr, *gl_out = nullptr, *gl_coef = nullptr, *gl_pro = nullptr;
   int2 *gl_pairs = nullptr;
@@ -1411,6 +1411,7 @@ static int glisa_setup(hp_part *p) {
   p->gl_Mp = Mp;
   p->gl_npair = T * (T + 1) / 2;
   p->gl_nchunk = (int)((p->npoint + HP_GLISA_CHUNK - 1) / HP_GLISA_CHUNK);
+  p->gl_hnchunk = (int)((p->npoint + HP_GLISA_HCHUNK - 1) / HP_GLISA_HCHUNK);
   p->gl_nblock = (int)((p->npoint + 255) / 256);
   std::vector<int2> pairs;
   for (int I = 0; I < T; ++I)
@@ -1419,7 +1420,7 @@ static int glisa_setup(hp_part *p) {
   if ((rc = dalloc(&p->gl_G, (size_t)p->npoint * Mp)) || (rc = dalloc(&p->gl_D, (size_t)p->npoint)) ||
       (rc = dalloc(&p->gl_E, (size_t)p->npoint)) || (rc = dalloc(&p->gl_obj, 2 * (size_t)p->gl_nblock)) ||
       (rc = dalloc(&p->gl_grad, (size_t)p->gl_nchunk * M)) ||
-      (rc = dalloc(&p->gl_hess, (size_t)p->gl_npair * p->gl_nchunk * 256)) ||
+      (rc = dalloc(&p->gl_hess, (size_t)p->gl_npair * p->gl_hnchunk * 256)) ||
       (rc = dalloc(&p->gl_out, 2 + (size_t)M + (size_t)M * M)) || (rc = dalloc(&p->gl_coef, 2 * (size_t)M)) ||
       (rc = dalloc(&p->gl_pro, 2 * (size_t)p->natom * (p->gl_nchunk + 1))) ||
       (rc = dupload(&p->gl_pairs, pairs.data(), pairs.size(), p->stream)))
@@ -1458,15 +1459,22 @@ int hp_part_global_terms(hp_part *p, const double *coefs, int nderiv, double *ob
   }
   if (nderiv == 2) {
     HP_HIP(hipEventRecord(p->ev[0], s));
-    hipLaunchKernelGGL(k_glisa_hess, dim3((unsigned)p->gl_npair, (unsigned)p->gl_nchunk), dim3(64), 0, s, p->npoint,
-                       Mp, p->gl_pairs, p->gl_G, p->gl_D, p->gl_hess);
+    // every tile pair of a chunk in one workgroup while the pairs fit its
+    // wavefronts' registers (Mp <= 112: at most 28 pairs, 7 per wavefront),
+    // else one wavefront per pair and chunk
+    if (Mp <= 112)
+      hipLaunchKernelGGL((k_glisa_hess_lds<32, 112, 7>), dim3((unsigned)p->gl_hnchunk), dim3(256), 0, s, p->npoint,
+                         Mp, p->gl_npair, p->gl_pairs, p->gl_G, p->gl_D, p->gl_hess);
+    else
+      hipLaunchKernelGGL(k_glisa_hess, dim3((unsigned)p->gl_npair, (unsigned)p->gl_hnchunk), dim3(64), 0, s,
+                         p->npoint, Mp, p->gl_pairs, p->gl_G, p->gl_D, p->gl_hess);
     HP_LAUNCHED();
     HP_HIP(hipEventRecord(p->ev[1], s));
   }
   const int nt = std::max({1, nderiv >= 1 ? M : 0, nderiv == 2 ? p->gl_npair * 256 : 0});
   hipLaunchKernelGGL(k_glisa_reduce, dim3((unsigned)((nt + 255) / 256)), dim3(256), 0, s, p->gl_nblock, p->gl_obj,
                      p->gl_nchunk, M, nderiv >= 1 ? 1 : 0, p->gl_grad, nderiv == 2 ? p->gl_npair : 0, p->gl_pairs,
-                     nderiv == 2 ? p->gl_hess : nullptr, p->gl_out);
+                     p->gl_hnchunk, nderiv == 2 ? p->gl_hess : nullptr, p->gl_out);
   HP_LAUNCHED();
   if (objective) HP_HIP(hipMemcpyAsync(objective, p->gl_out, sizeof(double), hipMemcpyDeviceToHost, s));
   if (min_promol)

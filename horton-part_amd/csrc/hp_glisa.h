@@ -20,7 +20,8 @@
 
 typedef double hp_gd4 __attribute__((ext_vector_type(4)));
 
-#define HP_GLISA_CHUNK 4096  // points per workgroup of the reductions
+#define HP_GLISA_CHUNK 4096   // points per workgroup of the reductions
+#define HP_GLISA_HCHUNK 1024  // points per workgroup of the Hessian passes
 
 // Basis value of function f at distance r (and r^2) from its atom.
 template <int FK>
@@ -79,17 +80,114 @@ __global__ __launch_bounds__(256) void k_glisa_points(DevMol m, DevFn fn, const 
 }
 
 // Gradient partials: workgroup c sums E g_i over its chunk for every i.
+// With Mp <= 256 the workgroup is ng = 256 / Mp groups of Mp threads; thread
+// i of group h adds E(p) g_i(p) over the points p0 + h, p0 + h + ng, ... (row
+// p of G is read by a group as one contiguous run), and the groups' sums are
+// added in group order.  Wider tables take columns i, i + 256, ... per thread.
 __global__ __launch_bounds__(256) void k_glisa_grad(int64_t npoint, int M, int Mp, const double *__restrict__ G,
                                                     const double *__restrict__ E, double *__restrict__ grad_part) {
-  __shared__ double lds[32];
+  __shared__ double s_part[256];
   const int64_t p0 = (int64_t)blockIdx.x * HP_GLISA_CHUNK;
   const int64_t p1 = p0 + HP_GLISA_CHUNK < npoint ? p0 + HP_GLISA_CHUNK : npoint;
-  for (int i = 0; i < M; ++i) {
-    double s = 0.0, dummy = 0.0;
-    for (int64_t p = p0 + threadIdx.x; p < p1; p += blockDim.x) s += E[p] * G[p * Mp + i];
-    block_sum2(s, dummy, lds);
-    if (threadIdx.x == 0) grad_part[(int64_t)blockIdx.x * M + i] = s;
+  if (Mp <= 256) {
+    const int ng = 256 / Mp;
+    const int h = threadIdx.x / Mp, col = threadIdx.x - h * Mp;
+    double s = 0.0;
+    if (h < ng) {
+#pragma unroll 8
+      for (int64_t p = p0 + h; p < p1; p += ng) s += E[p] * G[p * Mp + col];
+    }
+    s_part[threadIdx.x] = s;
     __syncthreads();
+    if (threadIdx.x < M) {
+      double v = 0.0;
+      for (int g = 0; g < ng; ++g) v += s_part[g * Mp + threadIdx.x];
+      grad_part[(int64_t)blockIdx.x * M + threadIdx.x] = v;
+    }
+  } else {
+    for (int col = threadIdx.x; col < M; col += blockDim.x) {
+      double s = 0.0;
+#pragma unroll 8
+      for (int64_t p = p0; p < p1; ++p) s += E[p] * G[p * Mp + col];
+      grad_part[(int64_t)blockIdx.x * M + col] = s;
+    }
+  }
+}
+
+// Hessian tile partials, every tile pair of a point chunk (HP_GLISA_HCHUNK
+// points) in one workgroup (Mp <= MPMAX <= 128, so at most 36 pairs); several
+// workgroups share a compute unit, so that one's loads of G overlap
+// another's products: the chunk's rows of G pass
+// through LDS SC at a time (one contiguous run of SC Mp doubles, loaded into
+// registers one round ahead), and wavefront w accumulates the pairs w, w + 4,
+// ... with the same rank-4 steps, in the same point order, as k_glisa_hess,
+// reading each row of G once instead of once per tile pair.
+// PW: tile pairs per wavefront (at most 4 PW pairs in all).
+template <int SC, int MPMAX, int PW>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k_glisa_hess_lds(
+    int64_t npoint, int Mp, int npair, const int2 *__restrict__ pairs, const double *__restrict__ G,
+    const double *__restrict__ D, double *__restrict__ hess_part) {
+  constexpr int RPT = SC * MPMAX / 256;
+  __shared__ double s_g[SC * MPMAX];
+  __shared__ double s_d[SC];
+  const int c = blockIdx.x, nchunk = gridDim.x;
+  const int64_t p0 = (int64_t)c * HP_GLISA_HCHUNK;
+  const int64_t p1 = p0 + HP_GLISA_HCHUNK < npoint ? p0 + HP_GLISA_HCHUNK : npoint;
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6, i = lane & 15, k = lane >> 4;
+  int pI[PW], pJ[PW];
+#pragma unroll
+  for (int j = 0; j < PW; ++j) {
+    const int pi = wv + 4 * j;
+    const int2 pr = pi < npair ? pairs[pi] : make_int2(0, 0);
+    pI[j] = 16 * pr.x + i;
+    pJ[j] = 16 * pr.y + i;
+  }
+  const int np = npair > wv ? (npair - wv + 3) / 4 : 0;  // pairs of this wavefront
+  hp_gd4 acc[PW];
+#pragma unroll
+  for (int j = 0; j < PW; ++j) acc[j] = hp_gd4{0.0, 0.0, 0.0, 0.0};
+  const int nround = (int)((p1 - p0 + SC - 1) / SC);
+  double reg[RPT], dreg = 0.0;
+  // round r's values: rows p0 + r SC ..., zero beyond the chunk
+  auto fetch = [&](int r) {
+    const int64_t pb = p0 + (int64_t)r * SC;
+    const int64_t lim = (p1 - pb < SC ? p1 - pb : SC) * Mp;
+    const double *src = G + pb * Mp;
+#pragma unroll
+    for (int u = 0; u < RPT; ++u) {
+      const int e = threadIdx.x + 256 * u;
+      reg[u] = e < lim ? src[e] : 0.0;
+    }
+    dreg = (threadIdx.x < SC && pb + threadIdx.x < p1) ? D[pb + threadIdx.x] : 0.0;
+  };
+  fetch(0);
+  for (int r = 0; r < nround; ++r) {
+    __syncthreads();  // the previous round's rows are no longer read
+#pragma unroll
+    for (int u = 0; u < RPT; ++u) s_g[threadIdx.x + 256 * u] = reg[u];
+    if (threadIdx.x < SC) s_d[threadIdx.x] = dreg;
+    __syncthreads();
+    if (r + 1 < nround) fetch(r + 1);
+    for (int q = 0; q < SC; q += 4) {
+      const double dv = s_d[q + k];
+      const double *row = s_g + (q + k) * Mp;
+#pragma unroll
+      for (int j = 0; j < PW; ++j) {
+        if (j < np) {
+          const double av = dv * row[pI[j]];
+          const double bv = row[pJ[j]];
+          acc[j] = __builtin_amdgcn_mfma_f64_16x16x4f64(av, bv, acc[j], 0, 0, 0);
+        }
+      }
+    }
+  }
+#pragma unroll
+  for (int j = 0; j < PW; ++j) {
+    if (j < np) {
+      double *out = hess_part + ((int64_t)(wv + 4 * j) * nchunk + c) * 256;
+#pragma unroll
+      for (int v = 0; v < 4; ++v) out[(4 * v + k) * 16 + i] = acc[j][v];
+    }
   }
 }
 
@@ -104,8 +202,8 @@ __global__ __launch_bounds__(64) void k_glisa_hess(int64_t npoint, int Mp, const
   const int2 pr = pairs[blockIdx.x];
   const int lane = threadIdx.x;
   const int i = lane & 15, k = lane >> 4;
-  const int64_t p0 = (int64_t)blockIdx.y * HP_GLISA_CHUNK;
-  const int64_t p1 = p0 + HP_GLISA_CHUNK < npoint ? p0 + HP_GLISA_CHUNK : npoint;
+  const int64_t p0 = (int64_t)blockIdx.y * HP_GLISA_HCHUNK;
+  const int64_t p1 = p0 + HP_GLISA_HCHUNK < npoint ? p0 + HP_GLISA_HCHUNK : npoint;
   hp_gd4 acc = {0.0, 0.0, 0.0, 0.0};
   for (int64_t q = p0; q < p1; q += 4) {
     const int64_t p = q + k;
@@ -125,7 +223,7 @@ __global__ __launch_bounds__(64) void k_glisa_hess(int64_t npoint, int Mp, const
 // symmetric Hessian.
 __global__ __launch_bounds__(256) void k_glisa_reduce(int nblock_obj, const double *__restrict__ obj_part, int nchunk,
                                                       int M, int do_grad, const double *__restrict__ grad_part, int npair,
-                                                      const int2 *__restrict__ pairs,
+                                                      const int2 *__restrict__ pairs, int hnchunk,
                                                       const double *__restrict__ hess_part, double *__restrict__ out) {
   // out: [0] objective, [1 .. 1+M) gradient, [1+M .. 1+M+M^2) Hessian, then
   // the smallest promolecule value
@@ -171,17 +269,17 @@ __global__ __launch_bounds__(256) void k_glisa_reduce(int nblock_obj, const doub
     const int2 pr = pairs[pi];
     const int ii = 16 * pr.x + e / 16, jj = 16 * pr.y + e % 16;
     if (ii < M && jj < M && ii <= jj) {
-      const double *col = hess_part + (int64_t)pi * nchunk * 256 + e;
+      const double *col = hess_part + (int64_t)pi * hnchunk * 256 + e;
       double s = 0.0;
       int c = 0;
-      for (; c + UB <= nchunk; c += UB) {
+      for (; c + UB <= hnchunk; c += UB) {
         double v[UB];
 #pragma unroll
         for (int u = 0; u < UB; ++u) v[u] = col[(int64_t)(c + u) * 256];
 #pragma unroll
         for (int u = 0; u < UB; ++u) s += v[u];
       }
-      for (; c < nchunk; ++c) s += col[(int64_t)c * 256];
+      for (; c < hnchunk; ++c) s += col[(int64_t)c * 256];
       out[1 + M + (int64_t)ii * M + jj] = s;
       out[1 + M + (int64_t)jj * M + ii] = s;
     }

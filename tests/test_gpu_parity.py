@@ -432,13 +432,16 @@ def _glisa_pair(w, basis_name, solver, **kw):
     return part, oracle
 
 
-def test_glisa_global_terms_against_oracle():
+@pytest.mark.parametrize("natom,nrad,nang", [(5, 40, 50), (22, 16, 26), (24, 16, 26)])
+def test_glisa_global_terms_against_oracle(natom, nrad, nang):
     """The device objective, gradient and MFMA Hessian of the global problem
     (glisa.py:411-479) against the oracle's at the initial coefficients and at
-    perturbed ones."""
+    perturbed ones.  M = 22 and 112 take the Hessian kernel that forms every
+    tile pair of a point chunk in one workgroup (3 and 28 pairs); M = 120
+    (padded to 128) the one with a wavefront per tile pair."""
     require_gpu()
-    _say("case: gLISA global terms, chain 5 atoms seed 11, 40x50 (gauss basis, M = 22)")
-    w = workloads.chain(natom=5, seed=11, nrad=40, nang=50)
+    _say(f"case: gLISA global terms, chain {natom} atoms seed 11, {nrad}x{nang} (gauss basis)")
+    w = workloads.chain(natom=natom, seed=11, nrad=nrad, nang=nang)
     part, oracle = _glisa_pair(w, "gauss", "sc")
     c = part._init_propars()
     oc = oracle.init_propars()
