@@ -1472,7 +1472,7 @@ int hp_part_global_terms(hp_part *p, const double *coefs, int nderiv, double *ob
     HP_HIP(hipEventRecord(p->ev[1], s));
   }
   const int nt = std::max({1, nderiv >= 1 ? M : 0, nderiv == 2 ? p->gl_npair * 256 : 0});
-  hipLaunchKernelGGL(k_glisa_reduce, dim3((unsigned)((nt + 255) / 256)), dim3(256), 0, s, p->gl_nblock, p->gl_obj,
+  hipLaunchKernelGGL(k_glisa_reduce, dim3((unsigned)(1 + (nt + 255) / 256)), dim3(256), 0, s, p->gl_nblock, p->gl_obj,
                      p->gl_nchunk, M, nderiv >= 1 ? 1 : 0, p->gl_grad, nderiv == 2 ? p->gl_npair : 0, p->gl_pairs,
                      p->gl_hnchunk, nderiv == 2 ? p->gl_hess : nullptr, p->gl_out);
   HP_LAUNCHED();

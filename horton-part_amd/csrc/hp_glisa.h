@@ -38,31 +38,56 @@ __device__ __forceinline__ double glisa_basis(const DevFn &fn, const double *nor
   return norm[f] * exp(-fn.alpha[f] * arg);
 }
 
-// One thread per point: G row, D, E and the objective's block partial.
+// One thread per point: G row, D, E and the objective's block partial.  The
+// workgroup's 256 rows of G pass through LDS 16 columns at a time, so that G
+// is written in whole 128-byte runs of each row (columns M .. Mp are zero).
 template <int FK>
 __global__ __launch_bounds__(256) void k_glisa_points(DevMol m, DevFn fn, const double *__restrict__ norm,
                                                       const double *__restrict__ coef, int M, int Mp, double dc,
                                                       double *__restrict__ G, double *__restrict__ D,
                                                       double *__restrict__ E, double *__restrict__ obj_part) {
+  constexpr int LD = 17;  // LDS row stride (doubles) of the column blocks
   __shared__ double lds[32];
   __shared__ double s_min[4];
-  const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  double fo = 0.0, dummy = 0.0, mn = 1.0e308;
-  if (p < m.npoint) {
-    const double x = m.px[p], y = m.py[p], z = m.pz[p];
-    double rho0 = 0.0;
-    double *row = G + p * Mp;
-    for (int a = 0; a < m.natom; ++a) {
-      const double dx = x - m.cx[a], dy = y - m.cy[a], dz = z - m.cz[a];
-      const double r2 = dx * dx + dy * dy + dz * dz;
-      const double r = sqrt(r2);
-      for (int f = fn.fn_off[a]; f < fn.fn_off[a + 1]; ++f) {
-        const double g = glisa_basis<FK>(fn, norm, f, r, r2);
-        row[f] = g;
-        rho0 += coef[f] * g;
+  __shared__ double s_t[256 * LD];
+  const int64_t pb = (int64_t)blockIdx.x * blockDim.x;
+  const int64_t p = pb + threadIdx.x;
+  const bool ok = p < m.npoint;
+  const int64_t pc = ok ? p : m.npoint - 1;  // idle threads follow the last point (no output)
+  const int nrow = (int)(m.npoint - pb < (int64_t)blockDim.x ? m.npoint - pb : (int64_t)blockDim.x);
+  // columns [16 blk, 16 blk + 16) of the workgroup's rows, from LDS to G
+  auto flush = [&](int blk) {
+    __syncthreads();
+    for (int q = threadIdx.x; q < 256 * 8; q += blockDim.x) {
+      const int row = q >> 3, pr = q & 7;
+      if (row < nrow) {
+        double2 v;
+        v.x = s_t[row * LD + 2 * pr];
+        v.y = s_t[row * LD + 2 * pr + 1];
+        *reinterpret_cast<double2 *>(G + (pb + row) * Mp + 16 * blk + 2 * pr) = v;
       }
     }
-    for (int f = M; f < Mp; ++f) row[f] = 0.0;
+    __syncthreads();
+  };
+  const double x = m.px[pc], y = m.py[pc], z = m.pz[pc];
+  double rho0 = 0.0;
+  for (int a = 0; a < m.natom; ++a) {
+    const double dx = x - m.cx[a], dy = y - m.cy[a], dz = z - m.cz[a];
+    const double r2 = dx * dx + dy * dy + dz * dz;
+    const double r = sqrt(r2);
+    for (int f = fn.fn_off[a]; f < fn.fn_off[a + 1]; ++f) {
+      const double g = glisa_basis<FK>(fn, norm, f, r, r2);
+      s_t[threadIdx.x * LD + (f & 15)] = g;
+      rho0 += coef[f] * g;
+      if ((f & 15) == 15) flush(f >> 4);
+    }
+  }
+  if (M & 15) {
+    for (int c = M & 15; c < 16; ++c) s_t[threadIdx.x * LD + c] = 0.0;
+    flush(M >> 4);
+  }
+  double fo = 0.0, dummy = 0.0, mn = 1.0e308;
+  if (ok) {
     const double rho = m.rho[p], w = m.wmol[p];
     const bool sick = (rho < dc) || (rho0 < dc);
     D[p] = sick ? 0.0 : w * rho / (rho0 * rho0);
@@ -220,37 +245,43 @@ __global__ __launch_bounds__(64) void k_glisa_hess(int64_t npoint, int Mp, const
 }
 
 // Fixed-order sums over the chunks: objective, gradient (negated) and the
-// symmetric Hessian.
+// symmetric Hessian.  Workgroup 0 forms the objective: thread t adds the
+// blocks of its contiguous range in block order, then thread 0 adds the 256
+// ranges in order.  The other workgroups take one gradient or Hessian entry
+// per thread (entry (blockIdx.x - 1) 256 + threadIdx.x), summed in chunk
+// order with UB values loaded together before they are added, so the
+// loads' latency is paid once per UB.
 __global__ __launch_bounds__(256) void k_glisa_reduce(int nblock_obj, const double *__restrict__ obj_part, int nchunk,
                                                       int M, int do_grad, const double *__restrict__ grad_part, int npair,
                                                       const int2 *__restrict__ pairs, int hnchunk,
                                                       const double *__restrict__ hess_part, double *__restrict__ out) {
   // out: [0] objective, [1 .. 1+M) gradient, [1+M .. 1+M+M^2) Hessian, then
   // the smallest promolecule value
-  // every sum runs in block (or chunk) order; UB values are loaded together
-  // before they are added, so the loads' latency is paid once per UB
-  constexpr int UB = 8;
-  const int t = blockIdx.x * blockDim.x + threadIdx.x;
-  if (t == 0) {
+  constexpr int UB = 16;
+  if (blockIdx.x == 0) {
+    __shared__ double s_sum[256], s_min[256];
+    const int seg = (nblock_obj + 255) / 256;
+    const int b0 = threadIdx.x * seg, b1 = b0 + seg < nblock_obj ? b0 + seg : nblock_obj;
     double s = 0.0, mn = 1.0e308;
-    int b = 0;
-    for (; b + UB <= nblock_obj; b += UB) {
-      double v[2 * UB];
-#pragma unroll
-      for (int u = 0; u < 2 * UB; ++u) v[u] = obj_part[2 * b + u];
-#pragma unroll
-      for (int u = 0; u < UB; ++u) {
-        s += v[2 * u];
-        mn = fmin(mn, v[2 * u + 1]);
-      }
-    }
-    for (; b < nblock_obj; ++b) {
+    for (int b = b0; b < b1; ++b) {
       s += obj_part[2 * b];
       mn = fmin(mn, obj_part[2 * b + 1]);
     }
-    out[0] = s;
-    out[1 + M + (int64_t)M * M] = mn;
+    s_sum[threadIdx.x] = s;
+    s_min[threadIdx.x] = mn;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      double tot = 0.0, tmn = 1.0e308;
+      for (int u = 0; u < 256; ++u) {
+        tot += s_sum[u];
+        tmn = fmin(tmn, s_min[u]);
+      }
+      out[0] = tot;
+      out[1 + M + (int64_t)M * M] = tmn;
+    }
+    return;
   }
+  const int t = (blockIdx.x - 1) * blockDim.x + threadIdx.x;
   if (do_grad && t < M) {
     double s = 0.0;
     int c = 0;
