@@ -679,7 +679,8 @@ def run_glisa(args):
     coefficients (``hp_part_global_terms``); roofline of the FP64 MFMA
     Hessian pass against M (M + 1) N algorithmic flops: the symmetric
     product's M (M + 1) / 2 distinct entries, a multiply and an add per entry
-    and point (the kernel forms the upper 16 x 16 tiles only)."""
+    and point (the kernel forms the upper 16 x 16 tiles only), and 2 M N for
+    the gradient when the same kernel forms it."""
     import warnings
 
     from horton_part_amd import GlobalLinearISAWPart, _native, workloads
@@ -700,9 +701,10 @@ def run_glisa(args):
     lat = np.array(lat)
     ms, n = dev.kernel_time(_native.KERNEL_GLISA_HESS)
     t_h = ms / max(n, 1) * 1e-3
-    flops = float(M) * (M + 1) * N
     Mp = (M + 15) // 16 * 16
     T = Mp // 16
+    fused = Mp <= 112  # the Hessian kernel also forms the gradient partials
+    flops = float(M) * (M + 1) * N + (2.0 * M * N if fused else 0.0)
     executed = 2.0 * 256 * (T * (T + 1) // 2) * N  # the MFMA work of the upper tiles, padding included
     tf = flops / t_h / 1e12
     out = {
@@ -722,7 +724,7 @@ def run_glisa(args):
                                f"{w.natom} atoms, {N} grid points"},
         "roofline": {
             "kernel": ("k_glisa_hess_lds (every upper 16x16 tile of G^T D G per point chunk on mfma_f64_16x16x4f64, "
-                       "rows of G staged in LDS)" if Mp <= 112 else
+                       "rows of G staged in LDS, with the gradient partials E G from the same rows)" if fused else
                        "k_glisa_hess (one wavefront per upper 16x16 tile pair and point chunk, mfma_f64_16x16x4f64)"),
             "bound": "mfma",
             "achieved": tf,
@@ -731,7 +733,8 @@ def run_glisa(args):
             "frac": tf / 78.6,
             "traffic": None,
             "flops_per_launch": flops,
-            "flops_note": "M (M + 1) N: the distinct entries of the symmetric M x M product over N points",
+            "flops_note": "M (M + 1) N: the distinct entries of the symmetric M x M product over N points"
+                          + (", + 2 M N for the gradient formed in the same launch" if fused else ""),
             "mfma_flops_per_launch": executed,
             "mfma_frac": executed / t_h / 1e12 / 78.6,
             "launch_us": t_h * 1e6,
@@ -855,23 +858,30 @@ def main():
 
     wk = fp64["work"]
     t_w = per_launch(fp64, "weights") * 1e-3
-    achieved = wk["bytes_per_pass"] / t_w / 1e9
+    hbm = wk["bytes_per_pass"] / t_w / 1e9
+    # exponentials performed (after culling and tail skipping) per second of
+    # the FP64 weights launches, against the exp probe's rate for the same code
+    exps = wk["evals_done_fp64"] / (fp64["kernels"]["weights"][0] * 1e-3) / 1e12
+    exp_peak = peaks["exp_fp64_per_s"] / 1e12
     roof = {
         "kernel": "k_weights<double> (fused pro-atom/promolecule/weights/spherical average)",
-        "bound": "hbm",
-        "achieved": achieved,
-        "peak": HBM_PEAK_GBPS,
-        "unit": "GB/s",
-        "frac": achieved / HBM_PEAK_GBPS,
-        "traffic": None,
+        "bound": "valu",
+        "achieved": exps,
+        "peak": exp_peak,
+        "unit": "Texp/s",
+        "frac": exps / exp_peak,
+        "peak_note": "the exponential throughput of the same exp code with every lane busy and no memory traffic "
+                     "(hp_exp_rate probe)",
+        "traffic": wk["bytes_per_pass"] * K_WEIGHTS_TRAFFIC_RATIO,
+        "traffic_note": "HBM bytes per launch: the algorithmic 48 B per point times the measured traffic ratio of "
+                        "the weights launches",
         "bytes_per_launch": wk["bytes_per_pass"],
+        "hbm_gbps": hbm,
+        "hbm_frac": hbm / HBM_PEAK_GBPS,
         "launch_us": t_w * 1e6,
         "evals_per_launch": wk["evals_per_pass"],
-        "evals_per_s": wk["evals_per_pass"] / t_w,
-        # evaluations performed (after culling and tail skipping) at the exp probe's rate
         "evals_done_frac": wk["evals_done_fp64"] / max(wk["evals_per_pass"] * fp64["niter"], 1.0),
-        "exp_frac": wk["evals_done_fp64"] / peaks["exp_fp64_per_s"] / (fp64["kernels"]["weights"][0] * 1e-3),
-        "copy_frac": achieved / peaks["copy_gbps"],
+        "copy_frac": hbm / peaks["copy_gbps"],
     }
     if fp64["kernels"]["grid"][1]:
         # molecular-grid Newton passes: the Hessian B diag(w) B^T on MFMA
@@ -885,6 +895,9 @@ def main():
             "unit": "TFLOP/s",
             "frac": tf / 78.6,
             "traffic": None,
+            "bound_note": "the Hessian of K = 6 functions fills 6 x 6 of each 16 x 16 tile; SQ counters of this "
+                          "workload show the vector ALUs issuing 56% of the pass's time and the matrix pipe busy "
+                          "20% (exponentials, divisions and the per-point sums bound it)",
             "hessian_flops": wk["hessian_flops"],
             "hessian_passes": wk["hessian_passes"],
             "pass_launches": n,

@@ -1419,7 +1419,7 @@ static int glisa_setup(hp_part *p) {
   int rc = 0;
   if ((rc = dalloc(&p->gl_G, (size_t)p->npoint * Mp)) || (rc = dalloc(&p->gl_D, (size_t)p->npoint)) ||
       (rc = dalloc(&p->gl_E, (size_t)p->npoint)) || (rc = dalloc(&p->gl_obj, 2 * (size_t)p->gl_nblock)) ||
-      (rc = dalloc(&p->gl_grad, (size_t)p->gl_nchunk * M)) ||
+      (rc = dalloc(&p->gl_grad, (size_t)std::max(p->gl_nchunk, p->gl_hnchunk) * M)) ||
       (rc = dalloc(&p->gl_hess, (size_t)p->gl_npair * p->gl_hnchunk * 256)) ||
       (rc = dalloc(&p->gl_out, 2 + (size_t)M + (size_t)M * M)) || (rc = dalloc(&p->gl_coef, 2 * (size_t)M)) ||
       (rc = dalloc(&p->gl_pro, 2 * (size_t)p->natom * (p->gl_nchunk + 1))) ||
@@ -1452,7 +1452,11 @@ int hp_part_global_terms(hp_part *p, const double *coefs, int nderiv, double *ob
     HP_GL_POINTS(HP_FN_GENERAL);
 #undef HP_GL_POINTS
   HP_LAUNCHED();
-  if (nderiv >= 1) {
+  // with the Hessian of the LDS kernel the gradient comes from its staged
+  // rows (partials per Hessian chunk), else from its own pass over G
+  const bool fused_grad = nderiv == 2 && Mp <= 112;
+  const int gchunk = fused_grad ? p->gl_hnchunk : p->gl_nchunk;
+  if (nderiv >= 1 && !fused_grad) {
     hipLaunchKernelGGL(k_glisa_grad, dim3((unsigned)p->gl_nchunk), dim3(256), 0, s, p->npoint, M, Mp, p->gl_G, p->gl_E,
                        p->gl_grad);
     HP_LAUNCHED();
@@ -1463,8 +1467,8 @@ int hp_part_global_terms(hp_part *p, const double *coefs, int nderiv, double *ob
     // wavefronts' registers (Mp <= 112: at most 28 pairs, 7 per wavefront),
     // else one wavefront per pair and chunk
     if (Mp <= 112)
-      hipLaunchKernelGGL((k_glisa_hess_lds<32, 112, 7>), dim3((unsigned)p->gl_hnchunk), dim3(256), 0, s, p->npoint,
-                         Mp, p->gl_npair, p->gl_pairs, p->gl_G, p->gl_D, p->gl_hess);
+      hipLaunchKernelGGL((k_glisa_hess_lds<16, 112, 7>), dim3((unsigned)p->gl_hnchunk), dim3(256), 0, s, p->npoint,
+                         Mp, p->gl_npair, p->gl_pairs, p->gl_G, p->gl_D, p->gl_hess, M, p->gl_E, p->gl_grad);
     else
       hipLaunchKernelGGL(k_glisa_hess, dim3((unsigned)p->gl_npair, (unsigned)p->gl_hnchunk), dim3(64), 0, s,
                          p->npoint, Mp, p->gl_pairs, p->gl_G, p->gl_D, p->gl_hess);
@@ -1473,7 +1477,7 @@ int hp_part_global_terms(hp_part *p, const double *coefs, int nderiv, double *ob
   }
   const int nt = std::max({1, nderiv >= 1 ? M : 0, nderiv == 2 ? p->gl_npair * 256 : 0});
   hipLaunchKernelGGL(k_glisa_reduce, dim3((unsigned)(1 + (nt + 255) / 256)), dim3(256), 0, s, p->gl_nblock, p->gl_obj,
-                     p->gl_nchunk, M, nderiv >= 1 ? 1 : 0, p->gl_grad, nderiv == 2 ? p->gl_npair : 0, p->gl_pairs,
+                     gchunk, M, nderiv >= 1 ? 1 : 0, p->gl_grad, nderiv == 2 ? p->gl_npair : 0, p->gl_pairs,
                      p->gl_hnchunk, nderiv == 2 ? p->gl_hess : nullptr, p->gl_out);
   HP_LAUNCHED();
   if (objective) HP_HIP(hipMemcpyAsync(objective, p->gl_out, sizeof(double), hipMemcpyDeviceToHost, s));

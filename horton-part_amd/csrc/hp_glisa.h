@@ -140,21 +140,25 @@ __global__ __launch_bounds__(256) void k_glisa_grad(int64_t npoint, int M, int M
 }
 
 // Hessian tile partials, every tile pair of a point chunk (HP_GLISA_HCHUNK
-// points) in one workgroup (Mp <= MPMAX <= 128, so at most 36 pairs); several
-// workgroups share a compute unit, so that one's loads of G overlap
-// another's products: the chunk's rows of G pass
-// through LDS SC at a time (one contiguous run of SC Mp doubles, loaded into
-// registers one round ahead), and wavefront w accumulates the pairs w, w + 4,
-// ... with the same rank-4 steps, in the same point order, as k_glisa_hess,
-// reading each row of G once instead of once per tile pair.
-// PW: tile pairs per wavefront (at most 4 PW pairs in all).
+// points) in one workgroup (Mp <= MPMAX; PW tile pairs per wavefront, at most
+// 4 PW pairs).  Two workgroups share a compute unit, so that one's loads of G
+// overlap the other's products.  The chunk's rows of G pass through LDS SC at
+// a time (one contiguous run of SC Mp doubles, loaded into registers one
+// round ahead).  Wavefront w accumulates
+// the pairs w, w + 4, ... with the same rank-4 steps, in the same point
+// order, as k_glisa_hess, reading each row of G once instead of once per
+// tile pair.  With grad_part it also forms the gradient partials of the
+// chunk from the staged rows (the even and the odd points of the chunk by
+// two thread groups, then their sum), in place of k_glisa_grad's second pass
+// over G.
 template <int SC, int MPMAX, int PW>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k_glisa_hess_lds(
     int64_t npoint, int Mp, int npair, const int2 *__restrict__ pairs, const double *__restrict__ G,
-    const double *__restrict__ D, double *__restrict__ hess_part) {
+    const double *__restrict__ D, double *__restrict__ hess_part, int M, const double *__restrict__ E,
+    double *__restrict__ grad_part) {
   constexpr int RPT = SC * MPMAX / 256;
   __shared__ double s_g[SC * MPMAX];
-  __shared__ double s_d[SC];
+  __shared__ double s_d[SC], s_e[SC];
   const int c = blockIdx.x, nchunk = gridDim.x;
   const int64_t p0 = (int64_t)c * HP_GLISA_HCHUNK;
   const int64_t p1 = p0 + HP_GLISA_HCHUNK < npoint ? p0 + HP_GLISA_HCHUNK : npoint;
@@ -172,7 +176,14 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k
 #pragma unroll
   for (int j = 0; j < PW; ++j) acc[j] = hp_gd4{0.0, 0.0, 0.0, 0.0};
   const int nround = (int)((p1 - p0 + SC - 1) / SC);
-  double reg[RPT], dreg = 0.0;
+  double reg[RPT], dreg = 0.0, ereg = 0.0;
+  const bool grad = grad_part != nullptr;
+  // gradient: thread t < 2 Mp adds E g_c over the rows q = h, h + 2, ... of
+  // every round (c = t mod Mp, h = t / Mp); the two halves are added in
+  // order at the end
+  const int gh = (int)threadIdx.x / Mp, gc = (int)threadIdx.x - gh * Mp;
+  const bool gthread = grad && gh < 2 && gc < M;
+  double gacc = 0.0;
   // round r's values: rows p0 + r SC ..., zero beyond the chunk
   auto fetch = [&](int r) {
     const int64_t pb = p0 + (int64_t)r * SC;
@@ -183,15 +194,22 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k
       const int e = threadIdx.x + 256 * u;
       reg[u] = e < lim ? src[e] : 0.0;
     }
-    dreg = (threadIdx.x < SC && pb + threadIdx.x < p1) ? D[pb + threadIdx.x] : 0.0;
+    const bool in = threadIdx.x < SC && pb + threadIdx.x < p1;
+    dreg = in ? D[pb + threadIdx.x] : 0.0;
+    ereg = (in && grad) ? E[pb + threadIdx.x] : 0.0;
   };
   fetch(0);
   for (int r = 0; r < nround; ++r) {
     __syncthreads();  // the previous round's rows are no longer read
 #pragma unroll
     for (int u = 0; u < RPT; ++u) s_g[threadIdx.x + 256 * u] = reg[u];
-    if (threadIdx.x < SC) s_d[threadIdx.x] = dreg;
+    if (threadIdx.x < SC) {
+      s_d[threadIdx.x] = dreg;
+      s_e[threadIdx.x] = ereg;
+    }
     __syncthreads();
+    if (gthread)
+      for (int q = gh; q < SC; q += 2) gacc += s_e[q] * s_g[q * Mp + gc];
     if (r + 1 < nround) fetch(r + 1);
     for (int q = 0; q < SC; q += 4) {
       const double dv = s_d[q + k];
@@ -213,6 +231,12 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k
 #pragma unroll
       for (int v = 0; v < 4; ++v) out[(4 * v + k) * 16 + i] = acc[j][v];
     }
+  }
+  if (grad) {
+    __syncthreads();  // s_g is free: the halves meet there
+    if (gthread) s_g[gh * Mp + gc] = gacc;
+    __syncthreads();
+    if ((int)threadIdx.x < M) grad_part[(int64_t)c * M + threadIdx.x] = s_g[threadIdx.x] + s_g[Mp + threadIdx.x];
   }
 }
 
