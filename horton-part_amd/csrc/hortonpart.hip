@@ -217,7 +217,7 @@ void free_part(hp_part *p) {
 
 template <typename R, int FK, bool GT2, int PT, int KM>
 void launch_weights_km(hp_part *p, int bs, int tile, int corr_mode) {
-  const size_t lds = hp::weights_lds(weights_lds_per_atom<R>(KM) * tile, p->wshape);
+  const size_t lds = hp::weights_lds(weights_lds_records<R>(KM, tile, FK), p->wshape);
   hipLaunchKernelGGL((k_weights<R, FK, GT2, PT, KM>), dim3(p->ngroup), dim3(bs), lds, p->stream, p->state, p->mol(),
                      p->fn(), p->rhobar, p->ent_part, p->promol, p->dc, tile, p->corr, corr_mode);
 }

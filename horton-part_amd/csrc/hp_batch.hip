@@ -536,7 +536,8 @@ bool plan_group(hp_batch *b, const std::vector<int> &active, std::vector<Launch>
     L.len = w - L.off;
     L.km = weights_km(kmax, L.fk);
     L.tile = L.fp64 ? weights_tile_size<double>(L.km, amax) : weights_tile_size<float>(L.km, amax);
-    L.lds = hp::weights_lds((L.fp64 ? weights_lds_per_atom<double>(L.km) : weights_lds_per_atom<float>(L.km)) * L.tile,
+    L.lds = hp::weights_lds(L.fp64 ? weights_lds_records<double>(L.km, L.tile, L.fk)
+                                   : weights_lds_records<float>(L.km, L.tile, L.fk),
                             hp::WeightsShape{L.pt, L.bs, 1});
     plan.push_back(L);
   }

@@ -439,6 +439,17 @@ __host__ __device__ constexpr size_t weights_lds_per_atom(int km) {
   return (size_t)weights_record<R>(km) * sizeof(R) + sizeof(int);
 }
 
+// Record bytes of a weights launch's staging rounds: `tile` records in R
+// and, for the FP64 pass of Slater or Gaussian pro-atoms (which can take the
+// tail path), `tile` FP32 records of the FP32 tier as well.
+template <typename R>
+__host__ __device__ constexpr size_t weights_lds_records(int km, int tile, int fk) {
+  return weights_lds_per_atom<R>(km) * (size_t)tile +
+         (sizeof(R) == sizeof(double) && fk != HP_FN_GENERAL
+              ? (size_t)weights_record<float>(km) * sizeof(float) * (size_t)tile
+              : 0);
+}
+
 // Functions per atom record: the smallest of 2, 4, 8, 16 that holds the
 // largest atom (general orders always use 16).
 inline __host__ int weights_km(int maxK, int fk) {
@@ -659,18 +670,71 @@ __device__ __forceinline__ float tail_log2_own(const DevFn &fn, int a, float rhi
   return m;
 }
 
-// The test for one candidate (its values are consumed as they arrive, so
-// few registers stay live beside the points').
+// The bound log2 U_b for one candidate (its values are consumed as they
+// arrive, so few registers stay live beside the points').
 template <int FK>
-__device__ __forceinline__ bool tail_negligible(const DevMol &m, const DevFn &fn, int b, double ax, double ay,
-                                                double az, float rhi, float log2_thr) {
+__device__ __forceinline__ float tail_log2_bound(const DevMol &m, const DevFn &fn, int b, double ax, double ay,
+                                                 double az, float rhi) {
   const float dx = (float)(m.cx[b] - ax), dy = (float)(m.cy[b] - ay), dz = (float)(m.cz[b] - az);
   const float d = __builtin_amdgcn_sqrtf(dx * dx + dy * dy + dz * dz);
   const float x = fmaxf(0.0f, d * (1.0f - 1e-6f) - rhi * (1.0f + 1e-6f));
   const int f0 = fn.fn_off[b], nf = fn.fn_off[b + 1] - f0;
   float u = -__builtin_inff();
   for (int k = 0; k < nf; ++k) u = fmaxf(u, tail_log2_term<FK>(fn.coef[f0 + k], fn.alpha[f0 + k], x));
-  return u + __builtin_amdgcn_logf((float)nf) < log2_thr;
+  return u + __builtin_amdgcn_logf((float)nf);
+}
+
+// The FP32 tier of the FP64 pass (no cut-off radius).  An atom whose bound
+// is below 2^-42 L adds less than 2^-42 of the promolecule at every point of
+// the workgroup.  Evaluated in FP32 (offsets from the shell's atom formed in
+// FP64 and rounded once, v_sqrt_f32, v_exp_f32 with the exponent scaled by
+// log2 e), a term that FP32 represents carries a relative error below 3e-4:
+// the distance is within 1.8e-7 (r + 2 r_max) of the FP64 one, and the
+// exponent alpha log2(e) r is below 136 for such a term, so with alpha up to
+// 40 per bohr and r_max up to 20 bohr the exponent moves by less than 4.4e-4.
+// The promolecule then moves by less than 3e-4 x 2^-42 < 2^-53 relative,
+// under one FP64 rounding of it.  Terms below the FP32 range (1.2e-38) are
+// lost; the tier is used only where L >= 2^-60, so that their sum over a few
+// hundred atoms stays below 2^-53 L as well.  Floors are added in FP64 as for
+// every atom.
+#define HP_TAIL_LOG2_TIER32 (-42.0f)
+#define HP_TAIL_LOG2_TIER32_MIN_L (-60.0f)
+
+template <int FK, int PT, int NP, int KM>
+__device__ __forceinline__ int weights_tile_f32(const float *__restrict__ rec, int tn, const double (&px)[PT],
+                                                const double (&py)[PT], const double (&pz)[PT], double ax, double ay,
+                                                double az, double (&promol)[PT]) {
+  constexpr int RS = weights_record<float>(KM);
+  // the points' offsets from the shell's atom, formed in FP64 and rounded once
+  float qx[NP], qy[NP], qz[NP];
+#pragma unroll
+  for (int j = 0; j < NP; ++j) {
+    qx[j] = (float)(px[j] - ax);
+    qy[j] = (float)(py[j] - ay);
+    qz[j] = (float)(pz[j] - az);
+  }
+  int nfn = 0;
+  for (int i = 0; i < tn; ++i) {
+    const float *q = rec + (size_t)i * RS;
+    float v[RS];
+#pragma unroll
+    for (int u = 0; u < RS; ++u) v[u] = q[u];
+    const int nf = __builtin_amdgcn_readfirstlane((int)v[3]);
+    nfn += nf;
+#pragma unroll
+    for (int j = 0; j < NP; ++j) {
+      const float dx = qx[j] - v[0], dy = qy[j] - v[1], dz = qz[j] - v[2];
+      const float r2 = dx * dx + dy * dy + dz * dz;
+      const float arg = FK == HP_FN_GAUSS ? r2 : __builtin_amdgcn_sqrtf(r2);
+      float pb = 0.0f;
+#pragma unroll
+      for (int k = 0; k < KM; ++k)
+        if (k < nf) pb += v[4 + k] * __builtin_amdgcn_exp2f(-v[4 + KM + k] * arg);
+      promol[j] += (double)pb;
+      promol[j] += 1e-100;
+    }
+  }
+  return nfn;
 }
 
 // Defect correction of the FP32 pass (corr_mode):
@@ -687,7 +751,7 @@ __device__ __forceinline__ bool tail_negligible(const DevMol &m, const DevFn &fn
 #define HP_CORR_SLOT -1
 
 template <typename R, int FK, bool GT2, int PT, int KM>
-__global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(KM <= 2 ? 6 : (KM <= 4 ? 5 : 1)))) void k_weights(const int *__restrict__ done, DevMol m, DevFn fn,
+__global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(sizeof(R) == sizeof(double) ? (KM <= 2 ? 5 : (KM <= 4 ? 4 : 1)) : (KM <= 2 ? 6 : (KM <= 4 ? 5 : 1))))) void k_weights(const int *__restrict__ done, DevMol m, DevFn fn,
                                                  double *__restrict__ rhobar, double *__restrict__ ent_part,
                                                  double *__restrict__ promol_out, double dc, int tile,
                                                  double *__restrict__ corr, int corr_mode) {
@@ -697,11 +761,14 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(KM <= 2 ? 6
   __shared__ double s_t64[64];  // 2^(j/64) for the FP64 exponential
   __shared__ float s_rhi[8];    // per wave: largest point distance from the shell's atom
   __shared__ int s_cnt[8];      // per wave: candidates kept in this staging round
+  __shared__ int s_cnt32[8];    // per wave: of those, in the FP32 tier
   __shared__ int s_own_tile, s_own_pos;  // tail path: staging round and slot of the own atom
   if (done && *done) return;
   constexpr int RS = weights_record<R>(KM);
   R *s_rec = reinterpret_cast<R *>(smem_d);
   int *s_idx = reinterpret_cast<int *>(s_rec + (size_t)tile * RS);
+  float *s_rec32 = reinterpret_cast<float *>(s_idx + tile);  // FP64 pass: the FP32 tier's records
+  constexpr int RS32 = weights_record<float>(KM);
 
   int s = m.grp_first ? m.grp_first[blockIdx.x] : (int)blockIdx.x;  // first radial shell of the group
   int a_begin = 0, a_end = m.natom;  // atoms of this shell's molecule
@@ -786,15 +853,17 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(KM <= 2 ? 6
     if (lane == 0) s_rhi[wv] = rhi;
   }
   const bool tail = tail_req;
-  float tail_rhi = 0.0f, tail_log2_thr = 0.0f;
+  float tail_rhi = 0.0f, tail_log2_thr = 0.0f, tail_log2_thr32 = 0.0f;
+  bool tier32 = false;  // FP64 pass: the FP32 tier in use for this workgroup
   int n_cut = 0;  // candidates skipped by the tail bound
   int n_fn = 0;  // functions evaluated on every point of this wavefront (uniform in it)
+  int n_fn32 = 0;  // of the FP32 tier
   // the tail path stages one candidate per thread and round
   const int step = tail_req ? min(tile, (int)blockDim.x) : tile;
   for (int t0 = 0, round = 0; t0 < nb_n; t0 += step, ++round) {
     const int tn = min(step, nb_n - t0);
     __syncthreads();
-    int tk = tn, own_i;
+    int tk = tn, tk32 = 0, own_i;
     if (!tail) {
       for (int i = threadIdx.x; i < tn; i += blockDim.x) {
         const int b = nb ? nb[t0 + i] : a_begin + t0 + i;
@@ -819,20 +888,38 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(KM <= 2 ? 6
                                       ? __builtin_amdgcn_logf((float)(a_end - a_begin)) - 332.19281f
                                       : -__builtin_inff();
         const float log2_eps = sizeof(R) == sizeof(double) ? HP_TAIL_LOG2_EPS64 : HP_TAIL_LOG2_EPS32;
-        tail_log2_thr = log2_eps - 2.0f + tail_log2_own<FK>(fn, a, rhi, log2_floors);
+        const float log2_own = tail_log2_own<FK>(fn, a, rhi, log2_floors);
+        tail_log2_thr = log2_eps - 2.0f + log2_own;
+        tail_log2_thr32 = HP_TAIL_LOG2_TIER32 - 2.0f + log2_own;
+        tier32 = sizeof(R) == sizeof(double) && !cull && log2_own >= HP_TAIL_LOG2_TIER32_MIN_L;
       }
-      if (i < tn) keep = b == a || !tail_negligible<FK>(m, fn, b, ax, ay, az, tail_rhi, tail_log2_thr);
-      const unsigned long long bal = __ballot(keep);
-      if (lane == 0) s_cnt[wv] = __popcll(bal);
+      bool keep32 = false;
+      if (i < tn) {
+        if (b == a) {
+          keep = true;
+        } else {
+          const float ub = tail_log2_bound<FK>(m, fn, b, ax, ay, az, tail_rhi);
+          keep = !(ub < tail_log2_thr);
+          keep32 = keep && tier32 && ub < tail_log2_thr32;
+        }
+      }
+      const bool keep64 = keep && !keep32;
+      const unsigned long long bal = __ballot(keep64), bal32 = __ballot(keep32);
+      if (lane == 0) {
+        s_cnt[wv] = __popcll(bal);
+        s_cnt32[wv] = __popcll(bal32);
+      }
       __syncthreads();
-      int base = 0;
+      int base = 0, base32 = 0;
       tk = 0;
       for (int w = 0; w < nw; ++w) {
-        const int c = s_cnt[w];
+        const int c = s_cnt[w], c32 = s_cnt32[w];
         base += w < wv ? c : 0;
+        base32 += w < wv ? c32 : 0;
         tk += c;
+        tk32 += c32;
       }
-      if (keep) {
+      if (keep64) {
         const int pos =
             base + (int)__builtin_amdgcn_mbcnt_hi((unsigned)(bal >> 32), __builtin_amdgcn_mbcnt_lo((unsigned)bal, 0u));
         stage_record<R, KM>(s_rec + (size_t)pos * RS, s_idx + pos, b, m, fn, ax, ay, az);
@@ -841,11 +928,18 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(KM <= 2 ? 6
           s_own_pos = pos;
         }
       }
+      if (keep32) {
+        const int pos = base32 + (int)__builtin_amdgcn_mbcnt_hi((unsigned)(bal32 >> 32),
+                                                                __builtin_amdgcn_mbcnt_lo((unsigned)bal32, 0u));
+        int unused;
+        stage_record<float, KM>(s_rec32 + (size_t)pos * RS32, &unused, b, m, fn, ax, ay, az);
+      }
       __syncthreads();
       // uniform across the block: scalar registers keep the atom loop uniform
       tk = __builtin_amdgcn_readfirstlane(tk);
+      tk32 = __builtin_amdgcn_readfirstlane(tk32);
       own_i = __builtin_amdgcn_readfirstlane(s_own_tile == round ? s_own_pos : -1);
-      n_cut += tn - tk;
+      n_cut += tn - tk - tk32;
     }
     // every point slot in use, or only the first (the shell fits one pass of the block)
     if (cull) {
@@ -858,6 +952,11 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(KM <= 2 ? 6
                                                            pz, promol, own)
                   : weights_tile<R, FK, PT, 1, KM, false>(s_rec, s_idx, s_t64, tk, own_i, fn, rc, qx, qy, qz, px, py,
                                                           pz, promol, own);
+      if constexpr (sizeof(R) == sizeof(double)) {
+        if (tk32 > 0)
+          n_fn32 += two ? weights_tile_f32<FK, PT, PT, KM>(s_rec32, tk32, px, py, pz, ax, ay, az, promol)
+                        : weights_tile_f32<FK, PT, 1, KM>(s_rec32, tk32, px, py, pz, ax, ay, az, promol);
+      }
     }
   }
   // per point: its spherical-average and entropy terms, staged in LDS over
@@ -922,8 +1021,11 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(KM <= 2 ? 6
 #pragma unroll
     for (int j = 0; j < PT; ++j) npts_w += (unsigned long long)__popcll(__ballot(pidx[j] < p1));
     if (lane == 0 && npts_w) {
-      const int slot = (sizeof(R) == sizeof(double) ? HP_EVAL_LANES : 0) + (int)((blockIdx.x * 8u + wv) & (HP_EVAL_LANES - 1));
+      const int lane_slot = (int)((blockIdx.x * 8u + wv) & (HP_EVAL_LANES - 1));
+      const int slot = (sizeof(R) == sizeof(double) ? HP_EVAL_LANES : 0) + lane_slot;
       atomicAdd(fn.evals_done + (size_t)slot * HP_EVAL_STRIDE, (unsigned long long)n_fn * npts_w);
+      // the FP32 tier's exponentials count with the FP32 ones
+      if (n_fn32) atomicAdd(fn.evals_done + (size_t)lane_slot * HP_EVAL_STRIDE, (unsigned long long)n_fn32 * npts_w);
     }
   }
 }
