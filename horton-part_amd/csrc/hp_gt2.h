@@ -175,11 +175,13 @@ __global__ __launch_bounds__(256) void k_gt2_pass(const int *__restrict__ done, 
 #pragma unroll
       for (int k = 0; k < KM; ++k) acc[4 + k] += wmol * (((double)T[k] * S.c[k]) * ratio);
     } else if (KIND == 2) {
+      // the Hessian's weight w rho_a / pro^2, one division per point
+      const double hw = sick ? 0.0 : ratio / pro * wmol;
 #pragma unroll
       for (int k = 0; k < KM; ++k) {
         const double integ = (double)T[k] * ratio;
         acc[4 + k] += integ * wmol;
-        s_g[wv][k][lane] = sick ? (R)0 : (R)(integ / pro * wmol);
+        s_g[wv][k][lane] = (R)((double)T[k] * hw);
         s_b[wv][k][lane] = (R)T[k];
       }
     }
