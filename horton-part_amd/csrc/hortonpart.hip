@@ -1731,7 +1731,7 @@ int hp_becke_weights(int device, int natom, const double *coordinates, const dou
   if (!rc) rc = grid_buf(g, 2, tab.size() * sizeof(double), &dtab);
   if (!rc) rc = grid_buf(g, 3, (natom + 1) * sizeof(int64_t), &doff);
   if (!rc) rc = grid_buf(g, 4, npoint * sizeof(double), &dout);
-  if (!rc) rc = grid_buf(g, 5, ord.size() * sizeof(int), &dord);
+  if (!rc) rc = grid_buf(g, 7, ord.size() * sizeof(int), &dord);
   if (rc) return rc;
   HP_HIP(hipMemcpyAsync(dtab, tab.data(), tab.size() * sizeof(double), hipMemcpyHostToDevice, g->stream));
   HP_HIP(hipMemcpyAsync(doff, atom_offsets, (natom + 1) * sizeof(int64_t), hipMemcpyHostToDevice, g->stream));
@@ -1758,6 +1758,9 @@ int hp_slater_density(int device, int natom, const double *coordinates, const in
   std::vector<double> tab(2 * (size_t)nfn);
   std::copy(coef, coef + nfn, tab.begin());
   std::copy(alpha, alpha + nfn, tab.begin() + nfn);
+  // the workgroups' bounds need every term to decrease with distance
+  bool cull = true;
+  for (int f = 0; f < nfn; ++f) cull = cull && coef[f] >= 0.0 && alpha[f] >= 0.0;
   GridScratch *g = nullptr;
   double *c = nullptr, *p = nullptr;
   void *dtab = nullptr, *doff = nullptr, *dout = nullptr;
@@ -1771,7 +1774,7 @@ int hp_slater_density(int device, int natom, const double *coordinates, const in
   HP_HIP(hipMemcpyAsync(doff, fn_offsets, (natom + 1) * sizeof(int32_t), hipMemcpyHostToDevice, g->stream));
   hipLaunchKernelGGL(k_slater_sum, dim3((unsigned)((npoint + 255) / 256)), dim3(256), 0, g->stream, natom, c, c + natom,
                      c + 2 * natom, (const int *)doff, (const double *)dtab, (const double *)dtab + nfn, npoint, p,
-                     p + npoint, p + 2 * npoint, (double *)dout);
+                     p + npoint, p + 2 * npoint, cull ? 1 : 0, (double *)dout);
   HP_LAUNCHED();
   HP_HIP(hipMemcpyAsync(out, dout, sizeof(double) * npoint, hipMemcpyDeviceToHost, g->stream));
   HP_HIP(hipStreamSynchronize(g->stream));

@@ -2406,22 +2406,127 @@ static __global__ void k_becke(int natom, const double *__restrict__ cx, const d
   if (ok) out[p] = w;
 }
 
-// sum_a sum_k coef_ak exp(-alpha_ak |p - R_a|) at every point.
-static __global__ void k_slater_sum(int natom, const double *__restrict__ cx, const double *__restrict__ cy,
-                             const double *__restrict__ cz, const int *__restrict__ fn_off,
-                             const double *__restrict__ coef, const double *__restrict__ alpha, int64_t npoint,
-                             const double *__restrict__ px, const double *__restrict__ py,
-                             const double *__restrict__ pz, double *__restrict__ out) {
+// sum_a sum_k coef_ak exp(-alpha_ak |p - R_a|) at every point, one
+// workgroup per 256 consecutive points.  With cull (every coefficient and
+// exponent non-negative, so every term decreases with distance) the
+// workgroup bounds each atom over its points' bounding box first: atom a adds
+// at least lo_a = max_k c_ak e^(-alpha_ak dmax_a) and at most
+// hi_a = nf_a max_k c_ak e^(-alpha_ak dmin_a) at every point of the box
+// (dmin, dmax the nearest and farthest distances from R_a to the box).  An
+// atom with hi_a < 2^-64 max_b lo_b adds less than 2^-64 of the density at
+// every point and is skipped.  The test runs in the log2 domain in FP32 with
+// two units of margin and 1e-6 relative margins on the distances; the kept
+// atoms are summed in atom order with the FP64 terms of the plain sum.
+static __global__ __launch_bounds__(256) void k_slater_sum(int natom, const double *__restrict__ cx,
+                                                           const double *__restrict__ cy, const double *__restrict__ cz,
+                                                           const int *__restrict__ fn_off,
+                                                           const double *__restrict__ coef,
+                                                           const double *__restrict__ alpha, int64_t npoint,
+                                                           const double *__restrict__ px, const double *__restrict__ py,
+                                                           const double *__restrict__ pz, int cull,
+                                                           double *__restrict__ out) {
+  __shared__ float s_red[6][4];
+  __shared__ float s_lo[4];
+  __shared__ int s_cnt[4];
+  __shared__ int s_keep[256];
   const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (p >= npoint) return;
-  const double x = px[p], y = py[p], z = pz[p];
+  const bool ok = p < npoint;
+  const int64_t pc = ok ? p : npoint - 1;
+  const double x = px[pc], y = py[pc], z = pz[pc];
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6, nw = blockDim.x >> 6;
   double acc = 0.0;
-  for (int a = 0; a < natom; ++a) {
-    const double dx = x - cx[a], dy = y - cy[a], dz = z - cz[a];
-    const double r = sqrt(dx * dx + dy * dy + dz * dz);
-    for (int f = fn_off[a]; f < fn_off[a + 1]; ++f) acc += coef[f] * exp(-alpha[f] * r);
+  if (!cull) {
+    for (int a = 0; a < natom; ++a) {
+      const double dx = x - cx[a], dy = y - cy[a], dz = z - cz[a];
+      const double r = sqrt(dx * dx + dy * dy + dz * dz);
+      for (int f = fn_off[a]; f < fn_off[a + 1]; ++f) acc += coef[f] * exp(-alpha[f] * r);
+    }
+    if (ok) out[p] = acc;
+    return;
   }
-  out[p] = acc;
+  // the points' bounding box (idle threads repeat the last point)
+  float bx[6] = {(float)x, (float)y, (float)z, -(float)x, -(float)y, -(float)z};  // min, then -max
+#pragma unroll
+  for (int c = 0; c < 6; ++c) {
+    const float v = (float)-wave_min(-(double)-bx[c]);  // wave maximum of -bx = -(wave minimum of bx)
+    if (lane == 0) s_red[c][wv] = -v;
+  }
+  __syncthreads();
+  float bmin[3], bmax[3];
+#pragma unroll
+  for (int c = 0; c < 3; ++c) {
+    float lo = s_red[c][0], nhi = s_red[c + 3][0];
+    for (int w = 1; w < nw; ++w) {
+      lo = fminf(lo, s_red[c][w]);
+      nhi = fminf(nhi, s_red[c + 3][w]);
+    }
+    bmin[c] = lo;
+    bmax[c] = -nhi;
+  }
+  auto dist_box = [&](int a, float &dmin, float &dmax) {
+    const float ac[3] = {(float)cx[a], (float)cy[a], (float)cz[a]};
+    float n2 = 0.0f, f2 = 0.0f;
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      const float lo = bmin[c] - ac[c], hi = ac[c] - bmax[c];
+      const float dn = fmaxf(0.0f, fmaxf(lo, hi));
+      const float df = fmaxf(fabsf(ac[c] - bmin[c]), fabsf(ac[c] - bmax[c]));
+      n2 += dn * dn;
+      f2 += df * df;
+    }
+    dmin = __builtin_amdgcn_sqrtf(n2) * (1.0f - 1e-6f);
+    dmax = __builtin_amdgcn_sqrtf(f2) * (1.0f + 1e-6f) + 1e-6f;
+  };
+  constexpr float log2e = 1.4426950408889634f;
+  // L = max_a log2 lo_a
+  float L = -__builtin_inff();
+  for (int a = threadIdx.x; a < natom; a += blockDim.x) {
+    float dmin, dmax;
+    dist_box(a, dmin, dmax);
+    for (int f = fn_off[a]; f < fn_off[a + 1]; ++f)
+      L = fmaxf(L, __builtin_amdgcn_logf((float)coef[f]) - (float)alpha[f] * log2e * dmax);
+  }
+  {
+    const float wl = (float)-wave_min(-(double)L);
+    if (lane == 0) s_lo[wv] = wl;
+  }
+  __syncthreads();
+  for (int w = 0; w < nw; ++w) L = fmaxf(L, s_lo[w]);
+  const float thr = L - 64.0f - 2.0f;
+  // the kept atoms, 256 candidates at a time, in atom order
+  for (int a0 = 0; a0 < natom; a0 += blockDim.x) {
+    const int a = a0 + (int)threadIdx.x;
+    bool keep = false;
+    if (a < natom) {
+      float dmin, dmax;
+      dist_box(a, dmin, dmax);
+      const int f0 = fn_off[a], nf = fn_off[a + 1] - f0;
+      float u = -__builtin_inff();
+      for (int f = f0; f < f0 + nf; ++f)
+        u = fmaxf(u, __builtin_amdgcn_logf((float)coef[f]) - (float)alpha[f] * log2e * dmin);
+      keep = nf > 0 && !(u + __builtin_amdgcn_logf((float)nf) < thr);
+    }
+    const unsigned long long bal = __ballot(keep);
+    if (lane == 0) s_cnt[wv] = __popcll(bal);
+    __syncthreads();
+    int base = 0, nk = 0;
+    for (int w = 0; w < nw; ++w) {
+      base += w < wv ? s_cnt[w] : 0;
+      nk += s_cnt[w];
+    }
+    if (keep)
+      s_keep[base + (int)__builtin_amdgcn_mbcnt_hi((unsigned)(bal >> 32), __builtin_amdgcn_mbcnt_lo((unsigned)bal, 0u))] = a;
+    __syncthreads();
+    nk = __builtin_amdgcn_readfirstlane(nk);
+    for (int i = 0; i < nk; ++i) {
+      const int b = s_keep[i];
+      const double dx = x - cx[b], dy = y - cy[b], dz = z - cz[b];
+      const double r = sqrt(dx * dx + dy * dy + dz * dz);
+      for (int f = fn_off[b]; f < fn_off[b + 1]; ++f) acc += coef[f] * exp(-alpha[f] * r);
+    }
+    __syncthreads();  // s_keep and s_cnt are rewritten by the next round
+  }
+  if (ok) out[p] = acc;
 }
 
 // ---------------------------------------------------------------------------

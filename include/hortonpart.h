@@ -458,7 +458,9 @@ int hp_becke_weights(int device, int natom, const double *coordinates, const dou
 /* sum_a sum_k coef_ak exp(-alpha_ak |p - R_a|) at every point: the
  * Slater-shell densities of the synthetic workloads (no reference
  * counterpart; the reference reads densities from wavefunction files,
- * scripts/generate_density.py:119-204). */
+ * scripts/generate_density.py:119-204).  With non-negative coefficients and
+ * exponents, atoms whose terms are below 2^-64 of the density at every point
+ * of a block of 256 consecutive points are skipped. */
 int hp_slater_density(int device, int natom, const double *coordinates, const int32_t *fn_offsets,
                       const double *coef, const double *alpha, int64_t npoint, const double *points, double *out);
 

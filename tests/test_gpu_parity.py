@@ -503,6 +503,31 @@ def test_glisa_newton_failures_like_reference():
     assert len(part.history_changes) == len(oracle.history_changes)
 
 
+@pytest.mark.parametrize("natom", [3, 70])
+def test_slater_density_device_matches_host(natom):
+    """The device Slater-shell density (hp_slater_density) against the plain
+    sum over every atom in numpy.  On the 70-atom chain the workgroups skip
+    atoms bounded below 2^-64 of the density over their points' box."""
+    require_gpu()
+    from horton_part_amd import _native
+
+    _say(f"case: Slater density, chain {natom} atoms seed 9, 20x26 grids")
+    w = workloads.chain(natom=natom, seed=9, nrad=20, nang=26, becke=False)
+    rng = np.random.default_rng(4)
+    nf = rng.integers(1, 4, size=natom)
+    off = np.concatenate([[0], np.cumsum(nf)]).astype(np.int32)
+    coef = rng.uniform(0.1, 50.0, size=off[-1])
+    alpha = rng.uniform(0.8, 8.0, size=off[-1])
+    pts = w.grid.points
+    dev = _native.slater_density(w.coordinates, off, coef, alpha, pts)
+    host = np.zeros(len(pts))
+    for a in range(natom):
+        r = np.linalg.norm(pts - w.coordinates[a], axis=1)
+        for f in range(off[a], off[a + 1]):
+            host += coef[f] * np.exp(-alpha[f] * r)
+    np.testing.assert_allclose(dev, host, rtol=1e-13, atol=0)
+
+
 @pytest.mark.parametrize("shape", ["cluster37", "chain60"])
 def test_becke_weights_device_matches_host(shape):
     """The device Becke weights against the numpy restatement (every cell
