@@ -381,6 +381,11 @@ def serving_cpu_baseline(items, fp64, budget_s=8.0):
         "per scheme (partitions / summed CPU time, no queueing)",
         "host_cpus": ncpu,
         "cpu_model": platform.processor() or platform.machine(),
+        # Requests are independent, so one molecule per core is the CPU's
+        # best case: the one-core figure times the host's cores, assuming
+        # perfect scaling (not measured).
+        "p_process_value": len(items) / total * ncpu,
+        "p_process_note": f"one request per core on {ncpu} cores, perfect scaling assumed (extrapolated, not measured)",
     }
 
 
