@@ -123,14 +123,28 @@ def chain(natom=20, seed=1001, nrad=150, nang=302, probs=None, becke=True, devic
     rng = np.random.default_rng(seed)
     probs = probs or {"C": 0.35, "H": 0.45, "N": 0.10, "O": 0.10}
     numbers = _pick_elements(rng, natom, probs)
-    pos = [np.zeros(3)]
-    while len(pos) < natom:
+    pos = np.zeros((natom, 3))
+    n = 1
+    while n < natom:
         v = rng.normal(size=3)
         v /= np.linalg.norm(v)
-        cand = pos[-1] + v * rng.uniform(1.0, 1.6)
-        if min(np.linalg.norm(cand - q) for q in pos) >= 0.9:
-            pos.append(cand)
-    coords = np.array(pos) * ANGSTROM
+        cand = pos[n - 1] + v * rng.uniform(1.0, 1.6)
+        # The acceptance test is min_q |cand - q| >= 0.9 with each distance
+        # formed by np.linalg.norm.  The squared distances of all accepted
+        # atoms at once decide it unless the nearest lies within 1e-12 of
+        # the threshold, where the per-atom norms decide as before.
+        d = pos[:n] - cand
+        m = float(np.min(np.einsum("ij,ij->i", d, d)))
+        if m > 0.81 * (1.0 + 1e-12):
+            ok = True
+        elif m < 0.81 * (1.0 - 1e-12):
+            ok = False
+        else:
+            ok = min(np.linalg.norm(cand - q) for q in pos[:n]) >= 0.9
+        if ok:
+            pos[n] = cand
+            n += 1
+    coords = pos * ANGSTROM
     return _build(f"chain{natom}", coords, numbers, nrad, nang, rng, lambda z: int(rng.integers(1, 4)), becke=becke, device=device)
 
 
