@@ -788,7 +788,8 @@ def main():
     ap.add_argument("--requests", type=int, default=4096, help="serving: requests of the stream (4096: the whole stream)")
     ap.add_argument("--builders", type=int, default=8, help="serving: host threads that build the requests")
     ap.add_argument("--lookahead", type=int, default=24, help="serving: requests built ahead of submission")
-    ap.add_argument("--arena-points", type=int, default=96_000_000, help="serving: grid points resident at once")
+    ap.add_argument("--arena-points", type=int, default=320_000_000,
+                    help="serving: grid points resident at once (48 B each: 15.4 GB at the default)")
     ap.add_argument("--max-inflight", type=int, default=256, help="serving: molecules resident at once")
     ap.add_argument("--fp64-every", type=int, default=64, help="serving: FP64 one-at-a-time sample: every n-th request")
     ap.add_argument("--switch-interval", type=float, default=0.0,
