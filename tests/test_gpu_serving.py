@@ -215,3 +215,47 @@ def test_batch_rejects_what_it_cannot_run():
             srv.submit(_part("mbis", "water", inner_threshold=1e-10))
         assert e.value.code == _native.HP_E_ARG
         assert srv.pending == 0
+
+
+def test_batch_serves_2000_atoms():
+    """The largest request of the admitted range: a 2000-atom protein-like
+    aggregate (16 x 26 points per atom, 832,000 points) served alone equals
+    the single-molecule path, and served beside small requests equals itself
+    alone (4 outer iterations, FP64; the tail bound culls the far atoms)."""
+    require_gpu()
+    from configs import PROTEIN_PROBS
+
+    _say("case: batch with a 2000-atom request (protein-like, 16x26, maxiter 4), alone, single path and mix")
+    w = workloads.cluster(natom=2000, seed=1003, nrad=16, nang=26, probs=PROTEIN_PROBS, becke=False, device=0)
+
+    def big():
+        return MBISWPart(w.coordinates, w.numbers, w.pseudo_numbers, w.grid, w.density, threshold=1e-6, maxiter=4)
+
+    single = big()
+    single.do_partitioning()
+    q1, p1, n1 = single["charges"].copy(), single["propars"].copy(), int(single["niter"])
+    single.close()
+    caps = dict(point_capacity=1_000_000, atom_capacity=2100, shell_capacity=33_000, function_capacity=6200,
+                max_molecule_points=900_000)
+    with _server(**caps) as srv:
+        srv.submit(big(), precision="fp64")
+        (alone,) = srv.run_until_done(timeout=240)
+    dq = np.max(np.abs(alone.charges - q1))
+    dp = np.max(np.abs(alone.propars - p1))
+    _say(f"  niter single={n1} batch={alone.niter} event={alone.event} max|dq|={dq:.1e} max|dp|={dp:.1e}")
+    assert alone.niter == n1
+    assert dq <= EXACT
+    assert dp <= EXACT * max(1.0, np.abs(p1).max())
+    with _server(**caps) as srv:
+        tb = srv.submit(big(), precision="fp64")
+        srv.submit(_part("mbis", "chain5"), precision="fp64")
+        srv.step()
+        srv.submit(_part("gisa", "water"), precision="mixed")
+        done = {d.ticket: d for d in srv.run_until_done(timeout=240)}
+    d = done[tb]
+    dq = np.max(np.abs(d.charges - alone.charges))
+    _say(f"  in the mix: niter={d.niter} max|dq| vs alone={dq:.1e}")
+    assert len(done) == 3
+    assert d.niter == alone.niter
+    assert dq <= EXACT
+    assert np.max(np.abs(d.propars - alone.propars)) <= EXACT * max(1.0, np.abs(alone.propars).max())
