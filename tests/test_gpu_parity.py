@@ -547,3 +547,60 @@ def test_becke_weights_device_matches_host(shape):
     host = BeckeWeights().generate_weights(g.points, w.coordinates, w.numbers, indices=g.indices)
     dev = BeckeWeights(device=0).generate_weights(g.points, w.coordinates, w.numbers, indices=g.indices)
     np.testing.assert_allclose(dev, host, rtol=1e-12, atol=1e-15)
+
+
+def _wide_shell_water(nrad=12):
+    """Water on atom grids whose radial shells hold 604 points each: the
+    302-point Lebedev rule and the same rule rotated, each at half weight (an
+    angular rule as exact as either).  Shells of more than 512 points take a
+    weights workgroup each, two points per thread."""
+    from horton_part_amd.grid import AtomGrid, BeckeWeights, MolGrid, exp_radial_grid
+    from horton_part_amd.lebedev import lebedev_rule
+
+    geo = workloads.water(seed=1000, nrad=4, nang=6)  # the geometry
+    coords, numbers = geo.coordinates, geo.numbers
+    rg = exp_radial_grid(nrad)
+    u, wl = lebedev_rule(302)
+    c1, s1, c2, s2 = np.cos(0.3), np.sin(0.3), np.cos(0.7), np.sin(0.7)
+    rot = np.array([[c1, -s1, 0.0], [s1, c1, 0.0], [0.0, 0.0, 1.0]]) @ np.array(
+        [[c2, 0.0, s2], [0.0, 1.0, 0.0], [-s2, 0.0, c2]]
+    )
+    u2 = np.concatenate([u, u @ rot.T])
+    w2 = np.concatenate([wl, wl]) / 2.0
+    r, wr = rg.points, rg.weights
+    atgrids = []
+    for c in coords:
+        g = AtomGrid(rg, sizes=[302], center=c)
+        g._points = (c + r[:, None, None] * u2[None, :, :]).reshape(-1, 3)
+        g._weights = ((wr * r**2 * 4.0 * np.pi)[:, None] * w2[None, :]).ravel()
+        g._sph_weights = np.tile(w2, nrad)
+        g._sizes = np.full(nrad, len(w2))
+        g._indices = np.concatenate([[0], np.cumsum(g._sizes)])
+        atgrids.append(g)
+    grid = MolGrid(numbers, atgrids, BeckeWeights())
+    rng = np.random.default_rng(7)
+    shells = workloads.density_shells(rng, numbers, lambda z: 2 if z == 8 else 1)
+    rho = workloads.eval_density(grid.points, coords, shells, rng)
+    return coords, numbers, geo.pseudo_numbers, grid, rho
+
+
+@pytest.mark.parametrize("scheme", ["mbis", "gisa"])
+def test_shells_above_512_points(scheme):
+    """Radial shells of 604 points against the oracle (FP64)."""
+    require_gpu()
+    _say(f"case: {scheme} water, 12 radial shells of 604 points per atom (302-point rule and a rotated copy)")
+    args = _wide_shell_water()
+    assert int(np.diff(args[3].atgrids[0].indices).max()) > 512
+    if scheme == "mbis":
+        part = MBISWPart(*args, maxiter=60)
+        part.do_partitioning()
+        oracle = OracleMBIS(*args, maxiter=60)
+        cache = oracle.do_partitioning()
+        _compare(part, cache, oracle)
+    else:
+        part = GaussianISAWPart(*args, maxiter=60)
+        part.do_partitioning()
+        bs = part.bs_helper
+        oracle = OracleGISA(*args, maxiter=60, basis=OracleBasis(bs.orders, bs.exponents, bs.initials))
+        cache = oracle.do_partitioning()
+        _compare(part, cache, oracle, check_inner=False)
