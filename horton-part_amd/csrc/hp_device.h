@@ -234,6 +234,72 @@ __device__ __forceinline__ void wave_allsum_n(double *v) {
   for (int i = 0; i < N; ++i) v[i] = swap16_sum(v[i]);
 }
 
+// Four sums in one register.  v_permlane32_swap exchanges lanes 32-63 of its
+// first operand with lanes 0-31 of its second, so for two values a and b the
+// sum of the swapped pair holds a_i + a_(i+32) in lanes 0-31 and
+// b_i + b_(i+32) in lanes 32-63.  v_permlane16_swap exchanges the odd rows
+// (16 lanes) of its first operand with the even rows of its second: on two
+// such registers (a|b and c|d) the swapped sum holds the quarter sums of a,
+// c, b, d in rows 0, 1, 2, 3, and the row reductions finish all four at once.
+__device__ __forceinline__ double pair_swap32_sum(double a, double b) {
+  const auto l = __builtin_amdgcn_permlane32_swap((unsigned)__double2loint(a), (unsigned)__double2loint(b), false,
+                                                  false);
+  const auto h = __builtin_amdgcn_permlane32_swap((unsigned)__double2hiint(a), (unsigned)__double2hiint(b), false,
+                                                  false);
+  return __hiloint2double((int)h[0], (int)l[0]) + __hiloint2double((int)h[1], (int)l[1]);
+}
+
+__device__ __forceinline__ double pair_swap16_sum(double p, double q) {
+  const auto l = __builtin_amdgcn_permlane16_swap((unsigned)__double2loint(p), (unsigned)__double2loint(q), false,
+                                                  false);
+  const auto h = __builtin_amdgcn_permlane16_swap((unsigned)__double2hiint(p), (unsigned)__double2hiint(q), false,
+                                                  false);
+  return __hiloint2double((int)h[0], (int)l[0]) + __hiloint2double((int)h[1], (int)l[1]);
+}
+
+__device__ __forceinline__ double lane_value(double v, int lane) {
+  return __hiloint2double(__builtin_amdgcn_readlane(__double2hiint(v), lane),
+                          __builtin_amdgcn_readlane(__double2loint(v), lane));
+}
+
+// Wave-wide sums of N values (every lane of the wavefront active), four per
+// register through the swaps above, then the row reductions; the sums come
+// back as wave-uniform values.  Fewer instructions than N separate trees:
+// the swaps need no copies, one addition serves two values.
+template <int N>
+__device__ __forceinline__ void wave_allsum_packed(double *v) {
+  constexpr int G = (N + 3) / 4;
+  double r[G];
+#pragma unroll
+  for (int g = 0; g < G; ++g) {
+    const int i = 4 * g;
+    const double p = i + 1 < N ? pair_swap32_sum(v[i], v[i + 1]) : pair_swap32_sum(v[i], 0.0);
+    if (i + 2 < N) {
+      const double q = i + 3 < N ? pair_swap32_sum(v[i + 2], v[i + 3]) : pair_swap32_sum(v[i + 2], 0.0);
+      r[g] = pair_swap16_sum(p, q);
+    } else {
+      r[g] = pair_swap16_sum(p, 0.0);
+    }
+  }
+#pragma unroll
+  for (int g = 0; g < G; ++g) r[g] += dpp_row_d<0xB1>(r[g]);  // quad_perm [1,0,3,2]
+#pragma unroll
+  for (int g = 0; g < G; ++g) r[g] += dpp_row_d<0x4E>(r[g]);  // quad_perm [2,3,0,1]
+#pragma unroll
+  for (int g = 0; g < G; ++g) r[g] += dpp_row_d<0x141>(r[g]);  // row_half_mirror
+#pragma unroll
+  for (int g = 0; g < G; ++g) r[g] += dpp_row_d<0x140>(r[g]);  // row_mirror
+  // rows 0, 1, 2, 3 hold values 4g, 4g + 2, 4g + 1, 4g + 3
+#pragma unroll
+  for (int g = 0; g < G; ++g) {
+    const int i = 4 * g;
+    v[i] = lane_value(r[g], 0);
+    if (i + 1 < N) v[i + 1] = lane_value(r[g], 32);
+    if (i + 2 < N) v[i + 2] = lane_value(r[g], 16);
+    if (i + 3 < N) v[i + 3] = lane_value(r[g], 48);
+  }
+}
+
 __device__ __forceinline__ double wave_sum(double v) {
   wave_sum_n<1>(&v);
   return v;
@@ -1136,8 +1202,10 @@ template <int NV, int PPL>
 __device__ __forceinline__ void solver_allsum(double *v, double *lds, int parity) {
   if constexpr (PPL == 1)
     block_allsum<NV>(v, lds, parity);
+  else if constexpr (NV == 1)
+    wave_allsum_n<1>(v);
   else
-    wave_allsum_n<NV>(v);
+    wave_allsum_packed<NV>(v);
 }
 
 // The solve for an atom of exactly KC shells (no per-shell guards in the
