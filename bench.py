@@ -463,6 +463,8 @@ def run_serving(args):
     over = [{"request": k, "scheme": stream[k]["scheme"], "natom": stream[k]["natom"], "dq": dqs[j],
              "niter": res[k][2].niter, "niter_fp64": fp64[j]["niter"]}
             for j, k in enumerate(sample_idx) if dqs[j] > 1e-6]
+    per_sample = [[k, stream[k]["scheme"], stream[k]["natom"], res[k][2].niter, fp64[j]["niter"], dqs[j]]
+                  for j, k in enumerate(sample_idx)]
     log(f"  {fp64_value:.3f} partitions/s (run only), {fp64_e2e:.3f} with upload; max|dq| served vs FP64 {dq:.2e}")
 
     d = {k: st1[k] - st0[k] for k in st1}
@@ -528,6 +530,7 @@ def run_serving(args):
         "max_dq_vs_fp64": dq,
         "max_dq_scope": f"the {len(fp64)} sampled requests: served result vs the FP64 path one at a time",
         "dq_over_1e-6": over,
+        "dq_per_sample": {"columns": ["request", "scheme", "natom", "niter", "niter_fp64", "dq"], "rows": per_sample},
         "host_loop": serve_stream.host,
         "batch": {
             "ticks": d["ticks"],
