@@ -604,3 +604,29 @@ def test_shells_above_512_points(scheme):
         oracle = OracleGISA(*args, maxiter=60, basis=OracleBasis(bs.orders, bs.exponents, bs.initials))
         cache = oracle.do_partitioning()
         _compare(part, cache, oracle, check_inner=False)
+
+
+def _heavy_chain(nrad):
+    """K, O, Rb, Cs, Fr in a row (4, 2, 5, 6 and 7 MBIS shells)."""
+    rng = np.random.default_rng(21)
+    numbers = np.array([19, 8, 37, 55, 87], dtype=np.int64)
+    coords = np.array([[0.0, 0.0, 0.0], [2.6, 0.0, 0.0], [5.8, 0.5, 0.0], [9.5, 0.0, 0.4], [13.4, 0.3, 0.0]])
+    return workloads._build("heavy5", coords, numbers, nrad, 26, rng, lambda z: 2)
+
+
+@pytest.mark.parametrize("nrad", [40, 150])
+def test_mbis_four_to_seven_shells(nrad):
+    """Atoms of 4-7 MBIS shells: 9-15 sums per inner iteration, four to a
+    register in up to four groups, with two (40 radial points) and three
+    (150) points per lane; FP64 against the oracle."""
+    require_gpu()
+    _say(f"case: MBIS K-O-Rb-Cs-Fr chain, {nrad}x26, maxiter 12")
+    w = _heavy_chain(nrad)
+    args = (w.coordinates, w.numbers, w.pseudo_numbers, w.grid, w.density)
+    part = MBISWPart(*args, maxiter=12)
+    part.do_partitioning()
+    oracle = OracleMBIS(*args, maxiter=12)
+    cache = oracle.do_partitioning()
+    # several inner solves run to their 2000-iteration cap; the bar is the
+    # plain FP64 one (measured: |dq| 3e-14, parameters 2e-10 relative)
+    _compare(part, cache, oracle)
